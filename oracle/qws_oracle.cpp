@@ -1,0 +1,733 @@
+// qws_oracle.cpp -- CPU restatement of QWS's domain-decomposed Wilson-clover
+// solve, written from the mathematics of the reference, not from its code.
+//
+// TEST INFRASTRUCTURE ONLY.  This library is the checker for the HIP product:
+// only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may load
+// it.  Nothing in qws_amd/ links, loads or calls it.
+//
+// Layout (the "plain" layout of the oracle; every array is owned by the caller):
+//   site index     s = x + Lx*(y + Ly*(z + Lz*t))      (global lattice)
+//   spinor         R psi[V][3][4][2]      colour, spin, re/im   (qws.h:170-177 order)
+//   link           R U[4][V][3][3][2]     U[mu][s] = QWS glud[..].c[i][j][ri] of the
+//                                          link from s to s+mu (qws.h:119-125)
+//   clover inverse R C[V][2][36]          QWS clvd[..].c[chir][k]   (qws.h:127-130)
+// Block (SAP domain) geometry: blocks of extent B[mu] tile the lattice; the block
+// colour is (bx+by+bz+bt)%2, which is QWS's domain colour
+// domain_e=(py+pz+pt)%2 with bx = 2*px + d (qws.cc:275-276, main.cc:426).
+//
+// Parity status of each function (how it is pinned) is listed in DESIGN.md and in
+// tests/test_oracle.py; every function here is checked against data/CASE* golden
+// norms and/or the reference library built into oracle/_ref.
+
+#include <cmath>
+#include <complex>
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <random>
+#include <vector>
+
+namespace {
+
+template <typename R> using cplx = std::complex<R>;
+
+struct Geom {
+  int L[4], B[4], NB[4];
+  long V;
+  int tbc;        // +1 periodic, -1 antiperiodic in t (QWS fbc_f[3])
+  bool improper;  // some of y/z/t has a single block: QWS's npe[mu]==1 branch
+};
+
+inline void coords(const Geom& g, long s, int c[4]) {
+  c[0] = (int)(s % g.L[0]); s /= g.L[0];
+  c[1] = (int)(s % g.L[1]); s /= g.L[1];
+  c[2] = (int)(s % g.L[2]); s /= g.L[2];
+  c[3] = (int)s;
+}
+inline long index(const Geom& g, const int c[4]) {
+  return c[0] + (long)g.L[0] * (c[1] + (long)g.L[1] * (c[2] + (long)g.L[2] * c[3]));
+}
+inline int colour(const Geom& g, const int c[4]) {
+  return (c[0] / g.B[0] + c[1] / g.B[1] + c[2] / g.B[2] + c[3] / g.B[3]) & 1;
+}
+inline long neighbour(const Geom& g, const int c[4], int mu, int dir) {
+  int n[4] = {c[0], c[1], c[2], c[3]};
+  n[mu] = (n[mu] + dir + g.L[mu]) % g.L[mu];
+  return index(g, n);
+}
+// A hop stays inside the block iff it neither leaves the block nor wraps round
+// it (ddd_in_d.cc:95-188: interior hops only; the rest are face terms,
+// ddd_out_d.cc:118-134,206-216).
+inline bool hop_inside(const Geom& g, const int c[4], int mu, int dir) {
+  int l = c[mu] % g.B[mu];
+  return dir > 0 ? (l < g.B[mu] - 1) : (l > 0);
+}
+// Factor carried by the t-hops: 2 (unnormalised projector) or 2*tbc across the
+// global t edge (qws.cc:298-310, wilson_d.h:514-537).
+inline double tfactor(const Geom& g, const int c[4], int dir) {
+  if (dir > 0 && c[3] == g.L[3] - 1) return 2.0 * g.tbc;
+  if (dir < 0 && c[3] == 0) return 2.0 * g.tbc;
+  return 2.0;
+}
+
+// ---------------------------------------------------------------- site algebra
+// Spin projection P and reconstruction R for the Dirac basis of wilson_d.h
+// (gamma_4 = diag(1,1,-1,-1)); forward hops use (1-gamma_mu), backward (1+gamma_mu).
+// Projection tables: wilson_d.h:540-657; reconstruction: wilson_d.h:362-537.
+template <typename R>
+inline void project(int mu, int dir, const cplx<R> s[4], cplx<R> a[2]) {
+  const cplx<R> I(0, 1);
+  const R sg = dir > 0 ? R(-1) : R(1);  // forward: minus sign
+  switch (mu) {
+    case 0: a[0] = s[0] + sg * I * s[3]; a[1] = s[1] + sg * I * s[2]; break;
+    case 1: a[0] = s[0] + sg * s[3];     a[1] = s[1] - sg * s[2];     break;
+    case 2: a[0] = s[0] + sg * I * s[2]; a[1] = s[1] - sg * I * s[3]; break;
+    default:
+      if (dir > 0) { a[0] = s[2]; a[1] = s[3]; } else { a[0] = s[0]; a[1] = s[1]; }
+  }
+}
+template <typename R>
+inline void reconstruct(int mu, int dir, const cplx<R> b[2], R tf, cplx<R> o[4]) {
+  const cplx<R> I(0, 1);
+  const R sg = dir > 0 ? R(1) : R(-1);
+  switch (mu) {
+    case 0: o[0] += b[0]; o[1] += b[1]; o[2] += sg * I * b[1]; o[3] += sg * I * b[0]; break;
+    case 1: o[0] += b[0]; o[1] += b[1]; o[2] += sg * b[1];     o[3] -= sg * b[0];     break;
+    case 2: o[0] += b[0]; o[1] += b[1]; o[2] += sg * I * b[0]; o[3] -= sg * I * b[1]; break;
+    default:
+      if (dir > 0) { o[2] += tf * b[0]; o[3] += tf * b[1]; }
+      else         { o[0] += tf * b[0]; o[1] += tf * b[1]; }
+  }
+}
+
+// One hop contribution to site x:  forward  R^-( W_mu(x)      P^- psi(x+mu) )
+//                                   backward R^+( W_mu(x-mu)^+ P^+ psi(x-mu) )
+// with W[i][j] = U.c[j][i]: __m_fg computes b_j = sum_i c[i][j] a_i
+// (wilson_d.h:110-135) and the dagger load conjugates and transposes
+// (wilson_d.h:176-187).
+template <typename R>
+inline void hop(int mu, int dir, const R* u, const R* psi_n, R tf, cplx<R> acc[3][4]) {
+  cplx<R> a[3][2], b[3][2];
+  for (int c = 0; c < 3; ++c) {
+    cplx<R> s[4];
+    for (int sp = 0; sp < 4; ++sp) s[sp] = cplx<R>(psi_n[(c * 4 + sp) * 2], psi_n[(c * 4 + sp) * 2 + 1]);
+    project(mu, dir, s, a[c]);
+  }
+  for (int j = 0; j < 3; ++j)
+    for (int h = 0; h < 2; ++h) {
+      cplx<R> v(0, 0);
+      for (int i = 0; i < 3; ++i) {
+        cplx<R> w = dir > 0 ? cplx<R>(u[(i * 3 + j) * 2], u[(i * 3 + j) * 2 + 1])
+                            : std::conj(cplx<R>(u[(j * 3 + i) * 2], u[(j * 3 + i) * 2 + 1]));
+        v += w * a[i][h];
+      }
+      b[j][h] = v;
+    }
+  for (int c = 0; c < 3; ++c) reconstruct(mu, dir, b[c], tf, acc[c]);
+}
+
+// Clover-inverse multiply (clover_def.h:88-284): per chirality i the 6-vector
+// x_i = [(s0 +- s2)_c, (s1 +- s3)_c] is multiplied by the packed Hermitian A_i
+// (k=0..5 real diagonal, then the upper triangle row by row as (re,im)), and
+// s0/s2 = y0 +- y1 (rows 0-2), s1/s3 = y0 +- y1 (rows 3-5).
+template <typename R>
+inline void clover_mult(const R* A, cplx<R> v[3][4]) {
+  cplx<R> x[2][6], y[2][6];
+  for (int c = 0; c < 3; ++c) {
+    x[0][c] = v[c][0] + v[c][2]; x[0][3 + c] = v[c][1] + v[c][3];
+    x[1][c] = v[c][0] - v[c][2]; x[1][3 + c] = v[c][1] - v[c][3];
+  }
+  for (int i = 0; i < 2; ++i) {
+    const R* a = A + 36 * i;
+    cplx<R> M[6][6];
+    for (int k = 0; k < 6; ++k) M[k][k] = cplx<R>(a[k], 0);
+    int p = 6;
+    for (int r = 0; r < 6; ++r)
+      for (int c = r + 1; c < 6; ++c) {
+        M[r][c] = cplx<R>(a[p], a[p + 1]);
+        M[c][r] = std::conj(M[r][c]);
+        p += 2;
+      }
+    for (int r = 0; r < 6; ++r) {
+      cplx<R> acc(0, 0);
+      for (int c = 0; c < 6; ++c) acc += M[r][c] * x[i][c];
+      y[i][r] = acc;
+    }
+  }
+  for (int c = 0; c < 3; ++c) {
+    v[c][0] = y[0][c] + y[1][c];         v[c][2] = y[0][c] - y[1][c];
+    v[c][1] = y[0][3 + c] + y[1][3 + c]; v[c][3] = y[0][3 + c] - y[1][3 + c];
+  }
+}
+
+template <typename R>
+struct Fields {
+  const R* U;  // [4][V][18]
+  const R* C;  // [V][72]
+};
+
+enum HopSet { HOPS_ALL = 0, HOPS_INSIDE = 1, HOPS_FACE = 2 };
+
+// Sum of hops of the chosen set at site s, then T^-1 applied: returns T^-1 H psi.
+template <typename R>
+inline void clover_hops(const Geom& g, const Fields<R>& f, const R* in, long s, HopSet set,
+                        cplx<R> acc[3][4]) {
+  int c[4];
+  coords(g, s, c);
+  for (int cc = 0; cc < 3; ++cc)
+    for (int sp = 0; sp < 4; ++sp) acc[cc][sp] = cplx<R>(0, 0);
+  for (int mu = 0; mu < 4; ++mu)
+    for (int dir = 1; dir >= -1; dir -= 2) {
+      bool inside = hop_inside(g, c, mu, dir);
+      if (set == HOPS_INSIDE && !inside) continue;
+      if (set == HOPS_FACE && inside) continue;
+      long n = neighbour(g, c, mu, dir);
+      long ls = dir > 0 ? s : n;  // link lives at the lower end of the hop
+      R tf = (R)tfactor(g, c, dir);
+      hop(mu, dir, f.U + ((long)mu * g.V + ls) * 18, in + n * 24, tf, acc);
+    }
+  clover_mult(f.C + s * 72, acc);
+}
+
+// out[s] = in[s] + factor * T^-1 H_set in   for the sites selected by `colour`
+// (colour < 0: every site).  With set=ALL and factor=-kappa this is the full
+// D = 1 - kappa T^-1 H (ddd_d_ qws.cc:625-632, ddd_s_ qws.cc:641-644).
+template <typename R>
+void apply_d(const Geom& g, const Fields<R>& f, R factor, R* out, const R* in, HopSet set,
+             int col) {
+#pragma omp parallel for schedule(static)
+  for (long s = 0; s < g.V; ++s) {
+    int c[4];
+    coords(g, s, c);
+    if (col >= 0 && colour(g, c) != col) continue;
+    cplx<R> acc[3][4];
+    clover_hops(g, f, in, s, set, acc);
+    for (int cc = 0; cc < 3; ++cc)
+      for (int sp = 0; sp < 4; ++sp) {
+        long k = s * 24 + (cc * 4 + sp) * 2;
+        out[k] = in[k] + factor * acc[cc][sp].real();
+        out[k + 1] = in[k + 1] + factor * acc[cc][sp].imag();
+      }
+  }
+}
+
+// out[s] += factor * T^-1 H_face in  for sites of colour `col`
+// (ddd_out_pos_s_ ddd_out_s.cc:341-626 with the faces packed by ddd_out_pre_s_;
+// the neighbour values are those of `in` at the physical neighbour sites).
+template <typename R>
+void add_face(const Geom& g, const Fields<R>& f, R factor, R* out, const R* in, int col) {
+#pragma omp parallel for schedule(static)
+  for (long s = 0; s < g.V; ++s) {
+    int c[4];
+    coords(g, s, c);
+    if (col >= 0 && colour(g, c) != col) continue;
+    cplx<R> acc[3][4];
+    clover_hops(g, f, in, s, HOPS_FACE, acc);
+    for (int cc = 0; cc < 3; ++cc)
+      for (int sp = 0; sp < 4; ++sp) {
+        long k = s * 24 + (cc * 4 + sp) * 2;
+        out[k] += factor * acc[cc][sp].real();
+        out[k + 1] += factor * acc[cc][sp].imag();
+      }
+  }
+}
+
+template <typename R>
+void for_colour(const Geom& g, int col, R* out, const R* a, const R* b, int op) {
+  // op 0: out = a; 1: out = 0; 2: out += a - b; 3: out -= a; 4: out += a
+#pragma omp parallel for schedule(static)
+  for (long s = 0; s < g.V; ++s) {
+    int c[4];
+    coords(g, s, c);
+    if (col >= 0 && colour(g, c) != col) continue;
+    for (int k = 0; k < 24; ++k) {
+      long i = s * 24 + k;
+      switch (op) {
+        case 0: out[i] = a[i]; break;
+        case 1: out[i] = 0; break;
+        case 2: out[i] += a[i] - b[i]; break;
+        case 3: out[i] -= a[i]; break;
+        case 4: out[i] += a[i]; break;
+      }
+    }
+  }
+}
+
+// Jacobi block solve on the blocks of one colour (static_solver.cc:65-140):
+//   q = D_in b ; x = 2b - q ; repeat nm-1 times: q = D_in x ; x = x + (b - q)
+template <typename R>
+void jinv(const Geom& g, const Fields<R>& f, R kappa, int col, int nm, R* x, const R* b, R* scratch) {
+  apply_d(g, f, -kappa, scratch, b, HOPS_INSIDE, col);
+#pragma omp parallel for schedule(static)
+  for (long s = 0; s < g.V; ++s) {
+    int c[4];
+    coords(g, s, c);
+    if (colour(g, c) != col) continue;
+    for (int k = 0; k < 24; ++k) x[s * 24 + k] = R(2) * b[s * 24 + k] - scratch[s * 24 + k];
+  }
+  for (int it = 1; it < nm; ++it) {
+    apply_d(g, f, -kappa, scratch, x, HOPS_INSIDE, col);
+#pragma omp parallel for schedule(static)
+    for (long s = 0; s < g.V; ++s) {
+      int c[4];
+      coords(g, s, c);
+      if (colour(g, c) != col) continue;
+      for (int k = 0; k < 24; ++k) x[s * 24 + k] += b[s * 24 + k] - scratch[s * 24 + k];
+    }
+  }
+}
+
+// SAP sweeps shared by prec_ddd_s_ and prec_s_ (qws.cc:809-841, 955-966).
+// Colour 0 is QWS's domain_e, colour 1 its domain_o.
+template <typename R>
+void sap_sweeps(const Geom& g, const Fields<R>& f, R kappa, int nsap, int nm, const R* in, R* s,
+                R* q, R* tmp) {
+  const int E = 0, O = 1;
+  for_colour<R>(g, -1, s, in, nullptr, 0);
+  if (g.improper) for_colour<R>(g, O, q, nullptr, nullptr, 1);
+  for (int isap = 0; isap < nsap; ++isap) {
+    for (int half = 0; half < 2; ++half) {
+      int c = half == 0 ? E : O, o = 1 - c;
+      jinv(g, f, kappa, c, nm, q, s, tmp);
+      apply_d(g, f, -kappa, tmp, q, HOPS_INSIDE, c);  // tmp_c = D_in q_c
+      for_colour<R>(g, c, s, in, tmp, 2);               // s_c += in_c - tmp_c
+      add_face(g, f, kappa, s, q, o);                   // s_o -= D_face q
+    }
+  }
+}
+
+// prec_ddd_s_ (qws.cc:776-926): out = D M_SAP in, both final branches.
+template <typename R>
+void prec_ddd(const Geom& g, const Fields<R>& f, R kappa, int nsap, int nm, R* out, const R* in) {
+  const int E = 0, O = 1;
+  std::vector<R> s(g.V * 24), q(g.V * 24, R(0)), tmp(g.V * 24);
+  sap_sweeps(g, f, kappa, nsap, nm, in, s.data(), q.data(), tmp.data());
+  if (g.improper) for_colour<R>(g, O, q.data(), nullptr, nullptr, 1);
+  jinv(g, f, kappa, E, nm, q.data(), s.data(), tmp.data());
+  apply_d(g, f, -kappa, out, q.data(), HOPS_INSIDE, E);  // out_e = D_in q_e
+  if (g.improper) {
+    add_face(g, f, kappa, s.data(), q.data(), O);         // s_o -= D_face q  (q_o = 0)
+    jinv(g, f, kappa, O, nm, q.data(), s.data(), tmp.data());
+    apply_d(g, f, -kappa, out, q.data(), HOPS_INSIDE, O);
+    add_face(g, f, -kappa, out, q.data(), E);
+    add_face(g, f, -kappa, out, q.data(), O);
+  } else {
+    for_colour<R>(g, O, out, nullptr, nullptr, 1);
+    add_face(g, f, -kappa, out, q.data(), O);             // out_o = D_oe q_e
+    for_colour<R>(g, O, s.data(), out, nullptr, 3);       // s_o -= out_o
+    jinv(g, f, kappa, O, nm, q.data(), s.data(), tmp.data());
+    apply_d(g, f, -kappa, tmp.data(), q.data(), HOPS_INSIDE, O);
+    for_colour<R>(g, O, out, tmp.data(), nullptr, 4);     // out_o += D_in q_o
+    add_face(g, f, -kappa, out, q.data(), E);             // out_e += D_eo q_o
+  }
+}
+
+// prec_s_ (qws.cc:929-975): out = M_SAP in.
+template <typename R>
+void prec_s(const Geom& g, const Fields<R>& f, R kappa, int nsap, int nm, R* out, const R* in) {
+  const int E = 0, O = 1;
+  std::vector<R> s(g.V * 24), q(g.V * 24, R(0)), tmp(g.V * 24);
+  sap_sweeps(g, f, kappa, nsap, nm, in, s.data(), q.data(), tmp.data());
+  if (g.improper) for_colour<R>(g, O, out, nullptr, nullptr, 1);
+  jinv(g, f, kappa, E, nm, out, s.data(), tmp.data());
+  add_face(g, f, kappa, s.data(), out, O);
+  jinv(g, f, kappa, O, nm, out, s.data(), tmp.data());
+}
+
+// Conjugated inner product <a,b> = sum conj(a) b (bicgstab_dd_mix.cc:180-181).
+// Reductions accumulate in long double so that the oracle's norms are
+// insensitive to summation order at the 1e-14 level check.sh asks for.
+template <typename R>
+cplx<double> dot(long n, const R* a, const R* b) {
+  long double re = 0, im = 0;
+#pragma omp parallel for reduction(+ : re, im) schedule(static)
+  for (long i = 0; i < n; i += 2) {
+    re += (long double)a[i] * b[i] + (long double)a[i + 1] * b[i + 1];
+    im += (long double)a[i] * b[i + 1] - (long double)a[i + 1] * b[i];
+  }
+  return cplx<double>((double)re, (double)im);
+}
+template <typename R>
+double norm2(long n, const R* a) {
+  long double r = 0;
+#pragma omp parallel for reduction(+ : r) schedule(static)
+  for (long i = 0; i < n; ++i) r += (long double)a[i] * a[i];
+  return (double)r;
+}
+// y = y + alpha x (complex alpha acting on interleaved re/im)
+template <typename R>
+void caxpy(long n, cplx<double> al, const R* x, R* y) {
+  const R ar = (R)al.real(), ai = (R)al.imag();
+#pragma omp parallel for schedule(static)
+  for (long i = 0; i < n; i += 2) {
+    R xr = x[i], xi = x[i + 1];
+    y[i] = y[i] + ar * xr - ai * xi;
+    y[i + 1] = y[i + 1] + ar * xi + ai * xr;
+  }
+}
+
+// Right-preconditioned BiCGStab on A = D M_SAP in FP32, x0 = 0
+// (bicgstab_precdd_s.cc:85-559).  The coefficients follow the reference's
+// update order; reductions are accumulated in double.
+void bicgstab_precdd(const Geom& g, const Fields<float>& f, float kappa, float* x, const float* b,
+                     double tol, int maxiter, int nsap, int nm, int* iters) {
+  const long n = g.V * 24;
+  std::vector<float> q(n), r(n), p(n), t(n), r0(n);
+  for (long i = 0; i < n; ++i) { x[i] = 0; r[i] = p[i] = r0[i] = b[i]; }
+  double bnorm = norm2(n, b);
+  cplx<double> rho0(bnorm, 0), alpha, omega, rho, beta;
+  int iter;
+  for (iter = 0; iter < maxiter; ++iter) {
+    prec_ddd(g, f, kappa, nsap, nm, q.data(), p.data());
+    cplx<double> c = dot(n, r0.data(), q.data());
+    alpha = std::abs(c) == 0 ? cplx<double>(0) : rho0 / c;
+    caxpy(n, alpha, p.data(), x);
+    caxpy(n, -alpha, q.data(), r.data());
+    double rnorm = norm2(n, r.data());
+    if (std::sqrt(rnorm / bnorm) < tol) { ++iter; break; }
+    prec_ddd(g, f, kappa, nsap, nm, t.data(), r.data());
+    double tt = norm2(n, t.data());
+    cplx<double> tr = dot(n, t.data(), r.data());
+    omega = tt == 0 ? cplx<double>(0) : tr / tt;
+    caxpy(n, omega, r.data(), x);
+    caxpy(n, -omega, t.data(), r.data());
+    rnorm = norm2(n, r.data());
+    rho = dot(n, r0.data(), r.data());
+    if (std::sqrt(rnorm / bnorm) < tol) { ++iter; break; }
+    beta = alpha * rho / (rho0 * omega);
+    rho0 = rho;
+    if (std::abs(omega) == 0 || std::abs(rho0) == 0) beta = 0;
+    // p = r + beta (p - omega q)
+    const float orr = (float)omega.real(), oi = (float)omega.imag();
+    const float br = (float)beta.real(), bi = (float)beta.imag();
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < n; i += 2) {
+      float cr = p[i] - orr * q[i] + oi * q[i + 1];
+      float ci = p[i + 1] - orr * q[i + 1] - oi * q[i];
+      p[i] = br * cr - bi * ci + r[i];
+      p[i + 1] = br * ci + bi * cr + r[i + 1];
+    }
+  }
+  if (iters) *iters = iter;
+}
+
+struct Ctx {
+  Geom g;
+  double kappa;
+  Fields<double> fd;
+  Fields<float> fs;
+  std::vector<float> Us, Cs;
+};
+
+// u = M p in FP64: normalise, FP32 inner solve, prec_s_, rescale
+// (approx_inv_dirac_op, bicgstab_dd_mix.cc:76-102; reordering.cc:108-204).
+void approx_inv(Ctx& cx, double* u, const double* p, double tol_s, int maxiter_s, int nsap, int nm,
+                int* inner) {
+  const long n = cx.g.V * 24;
+  std::vector<float> bs(n), ts(n), xs(n);
+  double nrm = std::sqrt(norm2(n, p));
+  double inv = 1.0 / nrm;
+  for (long i = 0; i < n; ++i) bs[i] = (float)(p[i] * inv);
+  int it = 0;
+  bicgstab_precdd(cx.g, cx.fs, (float)cx.kappa, ts.data(), bs.data(), tol_s, maxiter_s, nsap, nm, &it);
+  prec_s(cx.g, cx.fs, (float)cx.kappa, nsap, nm, xs.data(), ts.data());
+  for (long i = 0; i < n; ++i) u[i] = (double)xs[i] * nrm;
+  if (inner) *inner += it;
+}
+
+// FP64 outer BiCGStab on D with x as the initial guess (bicgstab_dd_mix.cc:105-388).
+// Loop iter = 1..maxiter-1; breaks on the recursive residual.
+int bicgstab_dd_mix(Ctx& cx, double* x, const double* b, double tol, int maxiter, double tol_s,
+                    int maxiter_s, int nsap, int nm, int* inner_total) {
+  const Geom& g = cx.g;
+  const long n = g.V * 24;
+  std::vector<double> q(n), r(n), p(n), t(n), u(n), r0(n);
+  apply_d(g, cx.fd, -cx.kappa, q.data(), x, HOPS_ALL, -1);
+  for (long i = 0; i < n; ++i) { r[i] = b[i] - q[i]; p[i] = r0[i] = r[i]; }
+  double bnorm = norm2(n, b);
+  cplx<double> rho0(norm2(n, r.data()), 0), alpha, omega, rho, beta;
+  int iter;
+  for (iter = 1; iter < maxiter; ++iter) {
+    approx_inv(cx, u.data(), p.data(), tol_s, maxiter_s, nsap, nm, inner_total);
+    apply_d(g, cx.fd, -cx.kappa, q.data(), u.data(), HOPS_ALL, -1);
+    cplx<double> c = dot(n, r0.data(), q.data());
+    alpha = std::abs(c) == 0 ? cplx<double>(0) : rho0 / c;
+    caxpy(n, alpha, u.data(), x);
+    caxpy(n, -alpha, q.data(), r.data());
+    double rnorm = norm2(n, r.data());
+    if (std::sqrt(rnorm / bnorm) < tol) break;
+    approx_inv(cx, u.data(), r.data(), tol_s, maxiter_s, nsap, nm, inner_total);
+    apply_d(g, cx.fd, -cx.kappa, t.data(), u.data(), HOPS_ALL, -1);
+    double tt = norm2(n, t.data());
+    cplx<double> tr = dot(n, t.data(), r.data());
+    omega = tt == 0 ? cplx<double>(0) : tr / tt;
+    caxpy(n, omega, u.data(), x);
+    caxpy(n, -omega, t.data(), r.data());
+    rnorm = norm2(n, r.data());
+    if (std::sqrt(rnorm / bnorm) < tol) break;
+    rho = dot(n, r0.data(), r.data());
+    beta = alpha * rho / (rho0 * omega);
+    rho0 = rho;
+    if (std::abs(omega) == 0 || std::abs(rho0) == 0) beta = 0;
+    for (long i = 0; i < n; i += 2) {
+      double cr = p[i] - omega.real() * q[i] + omega.imag() * q[i + 1];
+      double ci = p[i + 1] - omega.real() * q[i + 1] - omega.imag() * q[i];
+      p[i] = beta.real() * cr - beta.imag() * ci + r[i];
+      p[i + 1] = beta.real() * ci + beta.imag() * cr + r[i + 1];
+    }
+  }
+  return iter;
+}
+
+}  // namespace
+
+// ============================================================================ C API
+extern "C" {
+
+// Geometry: L = global extents, B = block (SAP domain) extents, tbc = +-1.
+void* orc_create(const int* L, const int* B, int tbc, double kappa) {
+  Ctx* cx = new Ctx();
+  long V = 1;
+  bool improper = false;
+  for (int m = 0; m < 4; ++m) {
+    cx->g.L[m] = L[m];
+    cx->g.B[m] = B[m];
+    cx->g.NB[m] = L[m] / B[m];
+    V *= L[m];
+    if (m > 0 && cx->g.NB[m] == 1) improper = true;
+  }
+  cx->g.V = V;
+  cx->g.tbc = tbc;
+  cx->g.improper = improper;
+  cx->kappa = kappa;
+  cx->fd = {nullptr, nullptr};
+  cx->fs = {nullptr, nullptr};
+  return cx;
+}
+void orc_destroy(void* h) { delete (Ctx*)h; }
+long orc_volume(void* h) { return ((Ctx*)h)->g.V; }
+int orc_improper(void* h) { return ((Ctx*)h)->g.improper ? 1 : 0; }
+
+// Fields are borrowed (caller keeps them alive).  FP32 copies are casts, as in
+// the reference driver (main.cc:446-474,510-514).
+void orc_set_fields(void* h, const double* U, const double* C) {
+  Ctx* cx = (Ctx*)h;
+  const long V = cx->g.V;
+  cx->fd = {U, C};
+  cx->Us.resize(4 * V * 18);
+  cx->Cs.resize(V * 72);
+  for (long i = 0; i < 4 * V * 18; ++i) cx->Us[i] = (float)U[i];
+  for (long i = 0; i < V * 72; ++i) cx->Cs[i] = (float)C[i];
+  cx->fs = {cx->Us.data(), cx->Cs.data()};
+}
+// The FP32 fields can also be set directly (e.g. rounded separately).
+void orc_set_fields_s(void* h, const float* U, const float* C) {
+  Ctx* cx = (Ctx*)h;
+  cx->fs = {U, C};
+}
+
+// ---- operators (set: 0 all hops, 1 inside-block hops, 2 face hops; col<0: all sites)
+void orc_dslash_d(void* h, double* out, const double* in, int set, int col) {
+  Ctx* cx = (Ctx*)h;
+  apply_d(cx->g, cx->fd, -cx->kappa, out, in, (HopSet)set, col);
+}
+void orc_dslash_s(void* h, float* out, const float* in, int set, int col) {
+  Ctx* cx = (Ctx*)h;
+  apply_d(cx->g, cx->fs, (float)-cx->kappa, out, in, (HopSet)set, col);
+}
+void orc_face_d(void* h, double* out, const double* in, double factor, int col) {
+  Ctx* cx = (Ctx*)h;
+  add_face(cx->g, cx->fd, factor, out, in, col);
+}
+void orc_face_s(void* h, float* out, const float* in, float factor, int col) {
+  Ctx* cx = (Ctx*)h;
+  add_face(cx->g, cx->fs, factor, out, in, col);
+}
+// out = T^-1 in on every site (clvd_vm2_, qws.cc:429-443)
+void orc_clover_d(void* h, double* out, const double* in) {
+  Ctx* cx = (Ctx*)h;
+  const long V = cx->g.V;
+#pragma omp parallel for
+  for (long s = 0; s < V; ++s) {
+    cplx<double> v[3][4];
+    for (int c = 0; c < 3; ++c)
+      for (int sp = 0; sp < 4; ++sp) v[c][sp] = cplx<double>(in[s * 24 + (c * 4 + sp) * 2], in[s * 24 + (c * 4 + sp) * 2 + 1]);
+    clover_mult(cx->fd.C + s * 72, v);
+    for (int c = 0; c < 3; ++c)
+      for (int sp = 0; sp < 4; ++sp) {
+        out[s * 24 + (c * 4 + sp) * 2] = v[c][sp].real();
+        out[s * 24 + (c * 4 + sp) * 2 + 1] = v[c][sp].imag();
+      }
+  }
+}
+void orc_jinv_s(void* h, float* x, const float* b, int col, int nm) {
+  Ctx* cx = (Ctx*)h;
+  std::vector<float> tmp(cx->g.V * 24);
+  jinv(cx->g, cx->fs, (float)cx->kappa, col, nm, x, b, tmp.data());
+}
+void orc_prec_ddd_s(void* h, float* out, const float* in, int nsap, int nm) {
+  Ctx* cx = (Ctx*)h;
+  prec_ddd(cx->g, cx->fs, (float)cx->kappa, nsap, nm, out, in);
+}
+void orc_prec_s(void* h, float* out, const float* in, int nsap, int nm) {
+  Ctx* cx = (Ctx*)h;
+  prec_s(cx->g, cx->fs, (float)cx->kappa, nsap, nm, out, in);
+}
+// Same algorithms in FP64 (used to build the all-FP64 comparison solve).
+void orc_prec_ddd_d(void* h, double* out, const double* in, int nsap, int nm) {
+  Ctx* cx = (Ctx*)h;
+  prec_ddd(cx->g, cx->fd, cx->kappa, nsap, nm, out, in);
+}
+int orc_bicgstab_precdd_s(void* h, float* x, const float* b, double tol, int maxiter, int nsap, int nm) {
+  Ctx* cx = (Ctx*)h;
+  int it = 0;
+  bicgstab_precdd(cx->g, cx->fs, (float)cx->kappa, x, b, tol, maxiter, nsap, nm, &it);
+  return it;
+}
+int orc_bicgstab_dd_mix(void* h, double* x, const double* b, double tol, int maxiter, double tol_s,
+                        int maxiter_s, int nsap, int nm, int* inner_total) {
+  Ctx* cx = (Ctx*)h;
+  if (inner_total) *inner_total = 0;
+  return bicgstab_dd_mix(*cx, x, b, tol, maxiter, tol_s, maxiter_s, nsap, nm, inner_total);
+}
+// The reference test driver's FP32 face probe (main.cc:781-784) calls
+// ddd_out_pre_s_/ddd_out_pos_s_ with the same x-domain index d on every rank.
+// ddd_out_pre_s_ then packs the y/z/t faces from x-domain (1-d) of the sending
+// rank whenever npe[mu] > 1 (ddd_out_s.cc:128-131), so a hop that crosses a
+// rank boundary reads the neighbour (and, backward, its link) in the other
+// x-half of that rank.  Hops inside one rank, x hops and the t boundary factor
+// are physical.  nloc = per-rank extents; out[s] += factor T^-1 H_face(in) for
+// sites in x-domain d.
+void orc_face_qws_uniform_s(void* h, const int* nloc, const int* npe, int d, float factor, float* out,
+                            const float* in) {
+  Ctx* cx = (Ctx*)h;
+  const Geom& g = cx->g;
+  const int nxh = nloc[0] / 2;
+#pragma omp parallel for schedule(static)
+  for (long s = 0; s < g.V; ++s) {
+    int c[4];
+    coords(g, s, c);
+    if ((c[0] % nloc[0]) / nxh != d) continue;
+    cplx<float> acc[3][4];
+    for (int cc = 0; cc < 3; ++cc)
+      for (int sp = 0; sp < 4; ++sp) acc[cc][sp] = 0;
+    for (int mu = 0; mu < 4; ++mu)
+      for (int dir = 1; dir >= -1; dir -= 2) {
+        if (hop_inside(g, c, mu, dir)) continue;
+        int n[4] = {c[0], c[1], c[2], c[3]};
+        n[mu] = (n[mu] + dir + g.L[mu]) % g.L[mu];
+        if (mu > 0 && npe[mu] > 1) {  // crossed a rank boundary: other x-half
+          int px = n[0] / nloc[0], xl = n[0] % nloc[0];
+          n[0] = px * nloc[0] + (xl + nxh) % nloc[0];
+        }
+        long ns = index(g, n);
+        long ls = dir > 0 ? s : ns;
+        float tf = (float)tfactor(g, c, dir);
+        hop(mu, dir, cx->fs.U + ((long)mu * g.V + ls) * 18, in + ns * 24, tf, acc);
+      }
+    clover_mult(cx->fs.C + s * 72, acc);
+    for (int cc = 0; cc < 3; ++cc)
+      for (int sp = 0; sp < 4; ++sp) {
+        long k = s * 24 + (cc * 4 + sp) * 2;
+        out[k] += factor * acc[cc][sp].real();
+        out[k + 1] += factor * acc[cc][sp].imag();
+      }
+  }
+}
+
+double orc_norm2_d(long n, const double* a) { return norm2(n, a); }
+double orc_norm2_s(long n, const float* a) { return norm2(n, a); }
+
+// ---------------------------------------------------------------- driver inputs
+// The reference test driver's inputs (main.cc:406-551): std::mt19937(12345),
+// Box-Muller with r = sqrt(-ln u) (variance 1/2 per real component).  Links: per
+// site (t,z,y,x order, both x-halves), per mu, rows 0 and 1 are drawn in the
+// order c1 = 0..2 (column), c2 = 0..1 (row); row 2 is the driver's conjugated
+// cross product including its '+' in place of '*' in the imaginary parts
+// (main.cc:459-474).  Clover: 2 chiralities x 18 complex Gaussians / 10, with
+// +1 on k = 0..5 (main.cc:481-516).  Then the source b and the initial guess x0,
+// both Gaussian (main.cc:200,264).  Every rank of a (npe) grid draws the same
+// sequence, so the global fields are the local fields tiled (main.cc:129).
+void orc_case_inputs(const int* nloc, const int* npe, double* U, double* C, double* b, double* x0) {
+  std::mt19937 mt(12345);
+  std::uniform_real_distribution<double> rand01(0.0, 1.0);
+  auto normal = [&](double* r0, double* r1) {
+    double r = std::sqrt(-std::log(rand01(mt)));
+    double t = 2.0 * 3.1415926535897932384626433832795028841971 * rand01(mt);
+    *r0 = r * std::cos(t);
+    *r1 = r * std::sin(t);
+  };
+  const int nx = nloc[0], ny = nloc[1], nz = nloc[2], nt = nloc[3];
+  const long vloc = (long)nx * ny * nz * nt;
+  std::vector<double> Ul(4 * vloc * 18), Cl(vloc * 72), bl(vloc * 24), xl(vloc * 24);
+  auto lidx = [&](int x, int y, int z, int t) { return x + (long)nx * (y + (long)ny * (z + (long)nz * t)); };
+  for (int t = 0; t < nt; ++t)
+    for (int z = 0; z < nz; ++z)
+      for (int y = 0; y < ny; ++y)
+        for (int x = 0; x < nx; ++x) {
+          long s = lidx(x, y, z, t);
+          for (int mu = 0; mu < 4; ++mu) {
+            double* u = &Ul[((long)mu * vloc + s) * 18];
+            for (int c1 = 0; c1 < 3; ++c1)
+              for (int c2 = 0; c2 < 2; ++c2) {
+                double r0, r1;
+                normal(&r0, &r1);
+                u[(c2 * 3 + c1) * 2] = r0;
+                u[(c2 * 3 + c1) * 2 + 1] = r1;
+              }
+            auto R = [&](int i, int j) { return u[(i * 3 + j) * 2]; };
+            auto I = [&](int i, int j) { return u[(i * 3 + j) * 2 + 1]; };
+            double u00r = R(0, 0), u00i = I(0, 0), u01r = R(0, 1), u01i = I(0, 1), u02r = R(0, 2), u02i = I(0, 2);
+            double u10r = R(1, 0), u10i = I(1, 0), u11r = R(1, 1), u11i = I(1, 1), u12r = R(1, 2), u12i = I(1, 2);
+            u[(6 + 0) * 2] = u01r * u12r - u01i * u12i - u02r * u11r + u02i * u11i;
+            u[(6 + 1) * 2] = u02r * u10r - u02i * u10i - u00r * u12r + u00i * u12i;
+            u[(6 + 2) * 2] = u00r * u11r - u00i * u11i - u01r * u10r + u01i * u10i;
+            u[(6 + 0) * 2 + 1] = -u01r * u12i - u01i + u12r + u02r * u11i + u02i * u11r;
+            u[(6 + 1) * 2 + 1] = -u02r * u10i - u02i + u10r + u00r * u12i + u00i * u12r;
+            u[(6 + 2) * 2 + 1] = -u00r * u11i - u00i + u11r + u01r * u10i + u01i * u10r;
+          }
+        }
+  for (int t = 0; t < nt; ++t)
+    for (int z = 0; z < nz; ++z)
+      for (int y = 0; y < ny; ++y)
+        for (int x = 0; x < nx; ++x) {
+          double* c = &Cl[lidx(x, y, z, t) * 72];
+          for (int j = 0; j < 2; ++j) {
+            for (int k = 0; k < 18; ++k) {
+              double r0, r1;
+              normal(&r0, &r1);
+              c[j * 36 + 2 * k] = r0 / 10.0;
+              c[j * 36 + 2 * k + 1] = r1 / 10.0;
+            }
+            for (int k = 0; k < 6; ++k) c[j * 36 + k] += 1;
+          }
+        }
+  for (double* dst : {bl.data(), xl.data()})
+    for (int t = 0; t < nt; ++t)
+      for (int z = 0; z < nz; ++z)
+        for (int y = 0; y < ny; ++y)
+          for (int x = 0; x < nx; ++x) {
+            double* p = dst + lidx(x, y, z, t) * 24;
+            for (int c = 0; c < 3; ++c)
+              for (int s = 0; s < 4; ++s) normal(&p[(c * 4 + s) * 2], &p[(c * 4 + s) * 2 + 1]);
+          }
+  // tile over the rank grid
+  const int L[4] = {nx * npe[0], ny * npe[1], nz * npe[2], nt * npe[3]};
+  const long V = (long)L[0] * L[1] * L[2] * L[3];
+  for (long s = 0; s < V; ++s) {
+    long r = s;
+    int X = (int)(r % L[0]); r /= L[0];
+    int Y = (int)(r % L[1]); r /= L[1];
+    int Z = (int)(r % L[2]); r /= L[2];
+    int T = (int)r;
+    long ls = lidx(X % nx, Y % ny, Z % nz, T % nt);
+    for (int mu = 0; mu < 4; ++mu)
+      std::memcpy(&U[((long)mu * V + s) * 18], &Ul[((long)mu * vloc + ls) * 18], 18 * sizeof(double));
+    std::memcpy(&C[s * 72], &Cl[ls * 72], 72 * sizeof(double));
+    if (b) std::memcpy(&b[s * 24], &bl[ls * 24], 24 * sizeof(double));
+    if (x0) std::memcpy(&x0[s * 24], &xl[ls * 24], 24 * sizeof(double));
+  }
+}
+
+}  // extern "C"

@@ -1,0 +1,281 @@
+"""CPU oracle checks (no GPU).
+
+The oracle is pinned three ways:
+  * golden norms of the reference's own test driver (data/CASE*, check.sh tolerances);
+  * full-vector agreement with the reference library compiled into oracle/_ref
+    (skipped when it is not built);
+  * closed forms: free-field plane waves, Clifford algebra of the gamma matrices
+    implied by the projector tables, dense clover construction, Neumann series
+    of the Jacobi block solve, textbook multiplicative Schwarz, and
+    prec_ddd_s(in) == D prec_s(in).
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from oracle import ref_worker as RW
+from tests import casekit as K
+
+OPERATOR_CASES = ["CASE0", "CASE_4111_1111", "CASE1", "CASE2", "CASE3", "CASE4", "CASE5", "CASE6",
+                  "CASE7", "CASE8", "CASE9"]
+
+
+def _probe_norms(case):
+    """The driver's DD-operator probes (main.cc:584-795) on the CASE inputs."""
+    nloc, npe, L, B = K.case_geometry(case)
+    _, U, C, b, _ = O.case_inputs(nloc, npe)
+    orc = O.Oracle(L, B, -1, 0.05, U, C)
+    dom0 = K.xdomain0_mask(L, nloc)
+    bs = b.astype(np.float32)
+    got = {"input": O.norm2(b), "input_s_": O.norm2(bs)}
+    cl = orc.clover(b)
+    cl[~dom0] = 0
+    got["clvd_vm_"] = got["clvd_vm2_"] = O.norm2(cl)
+    cls = cl.astype(np.float32)  # clv_s_ = same map in FP32
+    got["clv_s_"] = O.norm2(cls)
+    face = np.zeros_like(b)
+    orc.face(face, b, -0.05, -1)
+    face[~dom0] = 0
+    got["ddd_out_pre|pos_d_"] = O.norm2(face)
+    din = orc.dslash(b, O.HOPS_INSIDE)
+    din[~dom0] = 0
+    got["ddd_in_d_"] = O.norm2(din)
+    got["ddd_d_"] = O.norm2(orc.dslash(b))
+    fs = np.zeros_like(bs)
+    orc.face_qws_uniform_s(fs, bs, nloc, npe, 0, -0.05)
+    got["ddd_out_pre|pos_s_"] = O.norm2(fs)
+    dins = orc.dslash(bs, O.HOPS_INSIDE)
+    dins[~dom0] = 0
+    got["ddd_in_s_"] = O.norm2(dins)
+    got["ddd_s_"] = O.norm2(orc.dslash(bs))
+    return got
+
+
+@pytest.mark.parametrize("case", OPERATOR_CASES)
+def test_case_operator_norms(case):
+    got = _probe_norms(case)
+    for name, value in got.items():
+        want = K.expected(case, name)
+        assert K.check_value(name, value, want), (case, name, value, want)
+
+
+@pytest.mark.parametrize("case", ["CASE0", "CASE_4111_1111", "CASE1"])
+def test_case_dd_mixed_solve(case):
+    """bicgstab_dd_mix_ with nsap=4, nm=2, 5 outer x 2 x 50 inner (main.cc:262-312)."""
+    nloc, npe, L, B = K.case_geometry(case)
+    _, U, C, b, x0 = O.case_inputs(nloc, npe)
+    orc = O.Oracle(L, B, -1, 0.05, U, C)
+    x, it, inner = orc.bicgstab_dd_mix(x0, b, -1.0, 6, -1.0, 50, 4, 2)
+    assert it == 6 and inner == 500
+    r = b - orc.dslash(x)
+    assert K.check_value("bnorm^2", O.norm2(b), K.expected(case, "bnorm^2", "DD"))
+    assert O.norm2(r) < 1e-15
+    assert K.check_value("xnorm^2", O.norm2(x), K.expected(case, "xnorm^2", "DD"))
+
+
+# ---------------------------------------------------------------- reference library
+needs_ref = pytest.mark.skipif(not RW.available(), reason="oracle/_ref/libqws_ref.so not built")
+
+
+def _rel(a, b):
+    return np.linalg.norm((a - b).ravel()) / np.linalg.norm(b.ravel())
+
+
+@needs_ref
+@pytest.mark.parametrize("nloc,kappa", [([32, 6, 4, 3], 0.05), ([32, 4, 2, 4], 0.11)])
+def test_against_reference_library(nloc, kappa):
+    """Full vectors of every hot-path function vs the reference compiled from its sources."""
+    L, U, C, b, x0 = O.case_inputs(nloc)
+    if kappa != 0.05:  # a second field set: random but with the same structure
+        rng = np.random.default_rng(7)
+        U, C = K.random_su3_like(b.shape[0], rng)
+    B = [nloc[0] // 2] + nloc[1:]
+    orc = O.Oracle(L, B, -1, kappa, U, C)
+    ops = ["ddd_d", "ddd_s", "ddd_in_s0", "ddd_in_s1", "ddd_out_s0", "ddd_out_s1", "jinv0", "jinv1",
+           "prec_ddd_s", "prec_s", "bicgstab_precdd_s", "bicgstab_dd_mix"]
+    ref = RW.call(nloc, kappa, U, C, b, ops, x0=x0, maxiter_s=4, maxiter=3)
+    bs = b.astype(np.float32)
+    assert _rel(orc.dslash(b), ref["ddd_d"]) < 1e-14
+    assert _rel(orc.dslash(bs), ref["ddd_s"]) < 1e-6
+    for d in (0, 1):
+        o = np.zeros_like(bs)
+        orc.dslash(bs, O.HOPS_INSIDE, d, out=o)
+        assert _rel(o, ref["ddd_in_s%d" % d]) < 1e-6
+        o = np.zeros_like(bs)
+        orc.face(o, bs, -kappa, d)
+        assert _rel(o, ref["ddd_out_s%d" % d]) < 1e-6
+        o = np.zeros_like(bs)
+        orc.jinv_s(bs, d, 2, x=o)
+        assert _rel(o, ref["jinv%d" % d]) < 1e-6
+    assert _rel(orc.prec_ddd_s(bs, 4, 2), ref["prec_ddd_s"]) < 5e-6
+    assert _rel(orc.prec_s(bs, 4, 2), ref["prec_s"]) < 5e-6
+    x, _ = orc.bicgstab_precdd_s(bs, -1, 4, 4, 2)
+    assert _rel(x, ref["bicgstab_precdd_s"]) < 1e-4
+    x, it, _ = orc.bicgstab_dd_mix(x0, b, -1, 3, -1, 4, 4, 2)
+    assert int(ref["bicgstab_dd_mix_iter"]) == it
+    assert _rel(x, ref["bicgstab_dd_mix"]) < 1e-4
+
+
+# ---------------------------------------------------------------- closed forms
+def gammas():
+    """Euclidean gamma matrices implied by the projector tables: gamma_4 =
+    diag(1,1,-1,-1), gamma_k = [[0, i sigma_k], [-i sigma_k, 0]] (wilson_d.h:540-657)."""
+    s = [np.array([[0, 1], [1, 0]], complex), np.array([[0, -1j], [1j, 0]]), np.array([[1, 0], [0, -1]], complex)]
+    z = np.zeros((2, 2), complex)
+    g = [np.block([[z, 1j * sk], [-1j * sk, z]]) for sk in s]
+    g.append(np.diag([1, 1, -1, -1]).astype(complex))
+    return g
+
+
+def test_gamma_clifford():
+    g = gammas()
+    for m in range(4):
+        for n in range(4):
+            ac = g[m] @ g[n] + g[n] @ g[m]
+            assert np.allclose(ac, 2 * np.eye(4) * (m == n))
+
+
+def _to_c(psi):
+    return psi[..., 0] + 1j * psi[..., 1]  # (V, 3, 4) complex
+
+
+def _from_c(z):
+    return np.stack([z.real, z.imag], axis=-1)
+
+
+def test_free_field_plane_wave():
+    """U = 1, T^-1 = 1: D e^{ipx} chi = D(p) e^{ipx} chi with
+    D(p) = 1 - kappa sum_mu [(1-g_mu) e^{ip_mu} + (1+g_mu) e^{-ip_mu}], antiperiodic t."""
+    L = [4, 6, 2, 8]
+    V = int(np.prod(L))
+    U = np.zeros((4, V, 3, 3, 2))
+    for c in range(3):
+        U[:, :, c, c, 0] = 1.0
+    C = np.zeros((V, 2, 36))
+    C[:, :, :6] = 0.5
+    kappa = 0.13
+    orc = O.Oracle(L, [2, 3, 2, 4], -1, kappa, U, C)
+    g = gammas()
+    rng = np.random.default_rng(1)
+    n = [1, 2, 1, 3]
+    p = [2 * np.pi * n[0] / L[0], 2 * np.pi * n[1] / L[1], 2 * np.pi * n[2] / L[2], (2 * n[3] + 1) * np.pi / L[3]]
+    chi = rng.standard_normal((3, 4)) + 1j * rng.standard_normal((3, 4))
+    idx = np.arange(V)
+    x = [idx % L[0], (idx // L[0]) % L[1], (idx // (L[0] * L[1])) % L[2], idx // (L[0] * L[1] * L[2])]
+    phase = np.exp(1j * sum(p[m] * x[m] for m in range(4)))
+    psi = phase[:, None, None] * chi[None]
+    Dp = np.eye(4, dtype=complex)
+    for m in range(4):
+        Dp -= kappa * ((np.eye(4) - g[m]) * np.exp(1j * p[m]) + (np.eye(4) + g[m]) * np.exp(-1j * p[m]))
+    want = phase[:, None, None] * (chi @ Dp.T)[None]
+    got = _to_c(orc.dslash(_from_c(psi)))
+    assert np.abs(got - want).max() < 1e-12
+
+
+def test_clover_dense_and_identity():
+    """Packed clover multiply == dense 12x12 built from the packing rules of the
+    chirality blocks (rotation to (s0+-s2, s1+-s3)); A = 1/2 gives the identity."""
+    V = 5
+    rng = np.random.default_rng(2)
+    C = rng.standard_normal((V, 2, 36))
+    U = np.zeros((4, V, 3, 3, 2))
+    orc = O.Oracle([5, 1, 1, 1], [5, 1, 1, 1], 1, 0.1, U, C)
+    psi = rng.standard_normal((V, 3, 4, 2))
+    got = _to_c(orc.clover(psi))
+    pairs = [(r, c) for r in range(6) for c in range(r + 1, 6)]
+    for s in range(V):
+        M = np.zeros((12, 12), complex)  # basis index = colour*4 + spin
+        for i, sg in enumerate((1, -1)):
+            a = C[s, i]
+            A = np.diag(a[:6]).astype(complex)
+            for k, (r, c) in enumerate(pairs):
+                A[r, c] = a[6 + 2 * k] + 1j * a[7 + 2 * k]
+                A[c, r] = np.conj(A[r, c])
+            # x_i = R_i psi, out += R_i^T A x
+            R = np.zeros((6, 12))
+            for c in range(3):
+                R[c, c * 4 + 0] = 1
+                R[c, c * 4 + 2] = sg
+                R[3 + c, c * 4 + 1] = 1
+                R[3 + c, c * 4 + 3] = sg
+            M += R.T @ A @ R
+        want = M @ _to_c(psi[s]).reshape(12)
+        assert np.abs(got[s].reshape(12) - want).max() < 1e-12
+    C[:] = 0
+    C[:, :, :6] = 0.5
+    orc2 = O.Oracle([5, 1, 1, 1], [5, 1, 1, 1], 1, 0.1, U, C)
+    assert np.abs(orc2.clover(psi) - psi).max() < 1e-15
+
+
+def _small_problem(L, B, seed=3, kappa=0.12):
+    V = int(np.prod(L))
+    rng = np.random.default_rng(seed)
+    U, C = K.random_su3_like(V, rng)
+    psi = rng.standard_normal((V, 3, 4, 2))
+    return O.Oracle(L, B, -1, kappa, U, C), psi
+
+
+def test_inside_plus_face_is_full():
+    orc, psi = _small_problem([8, 4, 4, 4], [4, 2, 2, 2])
+    full = orc.dslash(psi)
+    inside = orc.dslash(psi, O.HOPS_INSIDE)
+    face = np.zeros_like(psi)
+    orc.face(face, psi, -orc.kappa, -1)
+    assert np.abs(inside + face - full).max() < 1e-13
+
+
+def test_jinv_neumann_series():
+    """x = sum_{k=0}^{nm} (1 - D_in)^k b on the blocks of one colour."""
+    orc, psi = _small_problem([8, 4, 4, 4], [4, 2, 2, 2])
+    psi = psi.astype(np.float32)
+    col = 1
+    mask = orc.dslash(np.zeros_like(psi), O.HOPS_INSIDE, col, out=np.full_like(psi, np.nan))
+    sel = ~np.isnan(mask[:, 0, 0, 0])
+    for nm in (1, 3):
+        x = orc.jinv_s(psi, col, nm)
+        term = psi.astype(np.float64).copy()
+        term[~sel] = 0
+        acc = term.copy()
+        for _ in range(nm):
+            din = orc.dslash(term, O.HOPS_INSIDE, col, out=np.zeros_like(term))
+            term = term - din
+            term[~sel] = 0
+            acc += term
+        assert np.abs(x[sel] - acc[sel]).max() < 1e-5
+
+
+def _schwarz_textbook(orc, b, nsap, nm):
+    """Multiplicative Schwarz x_c += A_c (b - D x)_c, nsap+1 sweeps over colours
+    0,1, with A_c the Jacobi block inverse (proper colouring)."""
+    x = np.zeros_like(b)
+    for _ in range(nsap + 1):
+        for c in (0, 1):
+            r = b - orc.dslash(x)
+            dx = orc.jinv_s(r.astype(np.float32), c, nm).astype(np.float64)
+            sel = _colour_mask(orc, c)
+            x[sel] += dx[sel]
+    return x
+
+
+def _colour_mask(orc, c):
+    m = orc.dslash(np.zeros((orc.V, 3, 4, 2)), O.HOPS_INSIDE, c, out=np.full((orc.V, 3, 4, 2), np.nan))
+    return ~np.isnan(m[:, 0, 0, 0])
+
+
+def test_sap_is_multiplicative_schwarz_proper():
+    orc, psi = _small_problem([8, 4, 4, 8], [2, 2, 2, 2], kappa=0.1)
+    assert not orc.improper
+    got = orc.prec_s(psi.astype(np.float32), 3, 2)
+    want = _schwarz_textbook(orc, psi, 3, 2)
+    assert _rel(got, want) < 2e-5
+
+
+@pytest.mark.parametrize("L,B", [([8, 4, 4, 8], [2, 2, 2, 2]), ([32, 6, 4, 3], [16, 6, 4, 3]),
+                                 ([8, 4, 2, 4], [4, 2, 2, 4])])
+def test_prec_ddd_equals_d_prec(L, B):
+    """prec_ddd_s_(in) == ddd_s_(prec_s_(in)) (test_qws.F90:168,181-182), both branches."""
+    orc, psi = _small_problem(L, B, kappa=0.1)
+    ps = psi.astype(np.float32)
+    a = orc.prec_ddd_s(ps, 4, 2)
+    bb = orc.dslash(orc.prec_s(ps, 4, 2))
+    assert _rel(a, bb) < 2e-5
