@@ -1,0 +1,183 @@
+/*
+ * qws_mi355x.h -- C ABI of libqws_mi355x.so, the MI355X (gfx950) Wilson-clover
+ * domain-decomposed solver.
+ *
+ * Two interfaces:
+ *
+ *  1. QWS drop-in.  The Fortran-callable symbols (trailing underscore, scalars by
+ *     pointer) of QWS's DD solver path, with QWS's host data layout (VLEND = 8,
+ *     VLENS = 16 AoSoA, qws.h:111-192) and QWS's exported globals.  A caller
+ *     written against QWS (its test driver main.cc, or BQCD through
+ *     qws_bqcd.cc) relinks against this library unchanged.  Arguments, argument
+ *     meaning and error behaviour follow the reference; each declaration cites
+ *     the reference interface it replaces.  Single rank: npe must be (1,1,1,1).
+ *
+ *  2. Device-resident interface (qwsg_*).  Fields and spinors live in HBM in the
+ *     library's blocked site-major layout; nothing crosses PCIe inside a solve.
+ *     Plain pointers and sizes only; every function returns 0 on success or a
+ *     nonzero error code, with qwsg_last_error() describing the failure.
+ */
+#ifndef QWS_MI355X_H
+#define QWS_MI355X_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------- QWS host types
+ * Same memory layout as qws.h:111-192 with VLEND = 8, VLENS = 16. */
+#define QWS_VLEND 8
+#define QWS_VLENS 16
+typedef struct { double c[3][4][2][QWS_VLEND]; } scd_t;   /* qws.h:155-160 */
+typedef struct { float c[3][4][2][QWS_VLENS]; } scs_t;    /* qws.h:183-189 */
+typedef struct { double c[3][3][2][QWS_VLEND]; } glud_t;  /* qws.h:119-125 */
+typedef struct { float c[3][3][2][QWS_VLENS]; } glus_t;   /* qws.h:144-152 */
+typedef struct { double c[2][36][QWS_VLEND]; } clvd_t;    /* qws.h:127-130 */
+typedef struct { float c[2][36][QWS_VLENS]; } clvs_t;     /* qws.h:154-158 */
+
+/* ---------------------------------------------------------------- QWS drop-in: globals
+ * Host mirrors of QWS's exported data symbols (qws.cc:74-124).  qws_init_
+ * allocates the four field arrays; a caller may write them directly, as the
+ * reference driver does (main.cc:439-516).  The device copy is refreshed on the
+ * next solver/operator call after qws_init_, after any qws_load*_ call, or after
+ * qws_mi355x_fields_modified_(). */
+extern glud_t* glud;
+extern glus_t* glus;
+extern clvd_t* clvd;
+extern clvs_t* clvs;
+extern double kappa, kappa2, mkappa;
+extern int nx, ny, nz, nt, nxh, nxd, nxs, vold, vols;
+extern int npe[4];
+extern double fbc[4][2];
+extern int rank, domain_e, domain_o;
+
+/* qws.cc:130 -- geometry, field allocation, t boundary (fbc_f[3] = -1: antiperiodic).
+ * block_size is ignored, as in the reference; the SAP domains are the two x-halves
+ * unless qws_mi355x_set_sap_blocks_ is called. */
+void qws_init_(int* lx, int* ly, int* lz, int* lt, int* npe_f, int* fbc_f, int* pce_f, int* pco_f, int* block_size);
+/* qws.cc:319 */
+void qws_finalize_(void);
+
+/* qws_bqcd.cc:125 -- BQCD-ordered links (needs BQCD's std_xyzt2i_/e_o_); sets kappa */
+void qws_loadg_dd_(double* u, int* volh_tot, double* in_kappa);
+/* qws_bqcd.cc:284 -- BQCD-ordered clover inverse (72 reals per site) */
+void qws_loadc_dd_(double* c, int* volh);
+/* qws_bqcd.cc:500,536,572,608 -- BQCD even/odd spinors <-> DD layout */
+void qws_loadfd_dd_(scd_t* out, double* ine, double* ino);
+void qws_storefd_dd_(double* oute, double* outo, scd_t* in);
+void qws_loadfs_dd_(scs_t* out, double* ine, double* ino);
+void qws_storefs_dd_(double* oute, double* outo, scs_t* in);
+
+/* bicgstab_dd_mix.cc:105 -- FP64 outer BiCGStab with FP32 SAP-preconditioned inner solve.
+ * x: initial guess in, solution out.  tol < 0: run maxiter-1 outer iterations. */
+void bicgstab_dd_mix_(scd_t* x, scd_t* b, double* tol, int* conviter, int* maxiter, double* tol_s, int* maxiter_s,
+                      int* nsap, int* nm);
+/* bicgstab_precdd_s.cc:85 -- FP32 right-preconditioned BiCGStab on D M_SAP, x0 = 0 */
+void bicgstab_precdd_s_(scs_t* x, scs_t* b, double* tol, int* conviter, int* maxiter, int* nsap, int* nm);
+/* qws.cc:776 -- out = D M_SAP in */
+void prec_ddd_s_(scs_t* out, scs_t* in, int* nsap, int* nm);
+/* qws.cc:929 -- out = M_SAP in */
+void prec_s_(scs_t* out, scs_t* in, int* nsap, int* nm);
+/* static_solver.cc:141 -- Jacobi block solve on one domain (x, b point at that domain) */
+void jinv_ddd_in_s_(scs_t* x, scs_t* b, int* domain, int* maxiter);
+/* qws.cc:625,641 -- full operator D = 1 - kappa T^-1 H */
+void ddd_d_(scd_t* out, scd_t* in);
+void ddd_s_(scs_t* out, scs_t* in);
+/* ddd_in_d.cc:66, ddd_in_s.cc:376 -- domain-local operator (out, in point at that domain) */
+void ddd_in_d_(scd_t* out, scd_t* in, int* domain);
+void ddd_in_s_(scs_t* out, scs_t* in, int* domain);
+/* ddd_out_d.cc:84,164, ddd_out_s.cc:639,661 -- face terms of a domain.  pre records
+ * the faces of `in`; pos adds factor * T^-1 H_face (FP64: factor -kappa) to out,
+ * which points at that domain. */
+void ddd_out_pre_d_(scd_t* in, int* domain);
+void ddd_out_pos_d_(scd_t* out, scd_t* in, int* domain);
+void ddd_out_pre_s_(scs_t* in, int* domain);
+void ddd_out_pos_s_(scs_t* out, scs_t* in, int* domain, float factor);
+/* qws.cc:390,429 -- clover inverse multiply on one parity/domain block of vold sites */
+void clvd_vm_(int* pe, scd_t* inout);
+void clvd_vm2_(int* pe, scd_t* out, scd_t* in);
+/* reordering.cc:141,108 -- FP64 <-> FP32 with normalisation */
+void fermi_reorder_d2s_dd_renorm_(scs_t* out, const scd_t* in, double* dnorm);
+void fermi_reorder_s2d_dd_renorm_(scd_t* out, const scs_t* in, const double* dnorm);
+
+/* No counterpart in the reference: mark the host field arrays as modified, so
+ * that the next call re-uploads them; and choose SAP domains finer than the two
+ * x-halves (extents must divide nx/2, ny, nz, nt).  For callers that write
+ * glud/glus/clvd/clvs themselves or want a GPU-sized SAP block. */
+void qws_mi355x_fields_modified_(void);
+void qws_mi355x_set_sap_blocks_(int* block_extents);
+
+/* ---------------------------------------------------------------- device-resident interface */
+typedef struct qwsg_lattice qwsg_lattice;
+
+enum { QWSG_F64 = 0, QWSG_F32 = 1, QWSG_F16 = 2 };
+enum { QWSG_HOPS_ALL = 0, QWSG_HOPS_INSIDE = 1, QWSG_HOPS_FACE = 2 };
+
+typedef struct {
+  int nsap, nm;            /* SAP sweeps and Jacobi steps per block solve (each <= 10) */
+  double tol;              /* outer: recursive residual sqrt(|r|^2/|b|^2) target (<0: fixed count) */
+  int maxiter;             /* outer: iterations run are 1..maxiter-1 (bicgstab_dd_mix.cc:165) */
+  double tol_s;            /* inner FP32 tolerance (<0: fixed count) */
+  int maxiter_s;           /* inner iterations */
+  double true_tol;         /* > 0: restart until the FP64 true residual |b-Dx|/|b| <= true_tol */
+  int max_restarts;        /* restarts allowed for true_tol */
+  int sap_precision;       /* QWSG_F32 or QWSG_F16 (FP16 storage, FP32 arithmetic) */
+} qwsg_solve_params;
+
+typedef struct {
+  int outer_iters;         /* outer BiCGStab iterations, summed over restarts */
+  int inner_iters;         /* inner BiCGStab iterations, summed */
+  int restarts;
+  double true_residual;    /* |b - D x| / |b| recomputed in FP64 on the GPU */
+  double seconds;          /* wall time of the solve */
+} qwsg_solve_stats;
+
+const char* qwsg_last_error(void);
+int qwsg_device_count(int* n);
+
+/* L: lattice extents, B: SAP block extents (must divide L; each of the
+ * y/z/t block counts even or 1, x block count even), tbc: +1 periodic / -1
+ * antiperiodic in t, links32: 18 (all 9 complex entries) or 12 (two rows;
+ * third row rebuilt, valid for SU(3) links only). */
+int qwsg_create(const int* L, const int* B, int tbc, int links32, qwsg_lattice** out);
+int qwsg_destroy(qwsg_lattice* lat);
+int qwsg_set_kappa(qwsg_lattice* lat, double kappa);
+int qwsg_volume(qwsg_lattice* lat, long long* V);
+/* host plain layout: links [4][V][3][3][2] (QWS c[i][j] of the link x -> x+mu),
+ * clover inverse [V][2][36] (QWS packing), site x + Lx*(y + Ly*(z + Lz*t)) */
+int qwsg_load_gauge(qwsg_lattice* lat, const double* U);
+int qwsg_load_clover(qwsg_lattice* lat, const double* C);
+/* FP32 fields rounded from separate FP32 host arrays (QWS's glus/clvs) */
+int qwsg_load_gauge_f32(qwsg_lattice* lat, const float* U);
+int qwsg_load_clover_f32(qwsg_lattice* lat, const float* C);
+
+/* spinors: device buffers of V sites in the given precision */
+int qwsg_spinor_alloc(qwsg_lattice* lat, int prec, void** dev);
+int qwsg_spinor_free(void* dev);
+int qwsg_spinor_upload(qwsg_lattice* lat, int prec, void* dev, const double* host_plain);
+int qwsg_spinor_download(qwsg_lattice* lat, int prec, const void* dev, double* host_plain);
+int qwsg_spinor_zero(qwsg_lattice* lat, int prec, void* dev);
+
+/* out = in - kappa T^-1 H_hops in on the sites of blocks of `colour` (-1: all) */
+int qwsg_dslash(qwsg_lattice* lat, int prec, void* out, const void* in, int hops, int colour);
+/* out += factor T^-1 H_face in on the sites of blocks of `colour` (-1: all) */
+int qwsg_face(qwsg_lattice* lat, int prec, void* out, const void* in, double factor, int colour);
+int qwsg_clover(qwsg_lattice* lat, int prec, void* out, const void* in);
+int qwsg_jinv(qwsg_lattice* lat, int prec, void* x, const void* b, int colour, int nm);
+int qwsg_prec_ddd(qwsg_lattice* lat, int prec, void* out, const void* in, int nsap, int nm);
+int qwsg_prec_s(qwsg_lattice* lat, int prec, void* out, const void* in, int nsap, int nm);
+int qwsg_bicgstab_precdd(qwsg_lattice* lat, int prec, void* x, const void* b, double tol, int maxiter, int nsap,
+                         int nm, int* iters);
+/* FP64 outer solve; x (FP64) holds the initial guess */
+int qwsg_solve(qwsg_lattice* lat, void* x, const void* b, const qwsg_solve_params* p, qwsg_solve_stats* st);
+int qwsg_norm2(qwsg_lattice* lat, int prec, const void* a, double* out);
+int qwsg_synchronize(qwsg_lattice* lat);
+/* average device time (ms) of `reps` full FP32/FP16/FP64 dslash launches on the
+ * lattice's stream, measured with HIP events */
+int qwsg_time_dslash(qwsg_lattice* lat, int prec, int reps, double* ms_per_call);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* QWS_MI355X_H */

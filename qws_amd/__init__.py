@@ -1,0 +1,227 @@
+"""qws_amd -- MI355X (gfx950) Wilson-clover domain-decomposed solver.
+
+The product is the C-ABI library qws_amd/lib/libqws_mi355x.so (HIP kernels and
+the C++ host solvers, declared in include/qws_mi355x.h).  This module is a thin
+ctypes binding of its device-resident interface (qwsg_*), used by the tests and
+the benchmark.  There is no CPU fallback: when the library is not built, or a
+call fails, an exception is raised.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(HERE, "lib", "libqws_mi355x.so")
+HEADER = os.path.join(os.path.dirname(HERE), "include", "qws_mi355x.h")
+
+F64, F32, F16 = 0, 1, 2
+HOPS_ALL, HOPS_INSIDE, HOPS_FACE = 0, 1, 2
+
+_lib = None
+
+
+class QwsError(RuntimeError):
+    pass
+
+
+class SolveParams(ctypes.Structure):
+    _fields_ = [("nsap", ctypes.c_int), ("nm", ctypes.c_int), ("tol", ctypes.c_double), ("maxiter", ctypes.c_int),
+                ("tol_s", ctypes.c_double), ("maxiter_s", ctypes.c_int), ("true_tol", ctypes.c_double),
+                ("max_restarts", ctypes.c_int), ("sap_precision", ctypes.c_int)]
+
+
+class SolveStats(ctypes.Structure):
+    _fields_ = [("outer_iters", ctypes.c_int), ("inner_iters", ctypes.c_int), ("restarts", ctypes.c_int),
+                ("true_residual", ctypes.c_double), ("seconds", ctypes.c_double)]
+
+
+def lib():
+    """Load libqws_mi355x.so (raises if it is not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIBPATH):
+            raise QwsError(f"{LIBPATH} is not built: run `python -m qws_amd.build`")
+        L = ctypes.CDLL(LIBPATH)
+        vp, ip = ctypes.c_void_p, ctypes.POINTER(ctypes.c_int)
+        L.qwsg_last_error.restype = ctypes.c_char_p
+        L.qwsg_create.argtypes = [vp, vp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(vp)]
+        L.qwsg_destroy.argtypes = [vp]
+        L.qwsg_set_kappa.argtypes = [vp, ctypes.c_double]
+        L.qwsg_volume.argtypes = [vp, ctypes.POINTER(ctypes.c_longlong)]
+        for n in ("qwsg_load_gauge", "qwsg_load_clover", "qwsg_load_gauge_f32", "qwsg_load_clover_f32"):
+            getattr(L, n).argtypes = [vp, vp]
+        L.qwsg_spinor_alloc.argtypes = [vp, ctypes.c_int, ctypes.POINTER(vp)]
+        L.qwsg_spinor_free.argtypes = [vp]
+        L.qwsg_spinor_upload.argtypes = [vp, ctypes.c_int, vp, vp]
+        L.qwsg_spinor_download.argtypes = [vp, ctypes.c_int, vp, vp]
+        L.qwsg_spinor_zero.argtypes = [vp, ctypes.c_int, vp]
+        L.qwsg_dslash.argtypes = [vp, ctypes.c_int, vp, vp, ctypes.c_int, ctypes.c_int]
+        L.qwsg_face.argtypes = [vp, ctypes.c_int, vp, vp, ctypes.c_double, ctypes.c_int]
+        L.qwsg_clover.argtypes = [vp, ctypes.c_int, vp, vp]
+        L.qwsg_jinv.argtypes = [vp, ctypes.c_int, vp, vp, ctypes.c_int, ctypes.c_int]
+        L.qwsg_prec_ddd.argtypes = [vp, ctypes.c_int, vp, vp, ctypes.c_int, ctypes.c_int]
+        L.qwsg_prec_s.argtypes = [vp, ctypes.c_int, vp, vp, ctypes.c_int, ctypes.c_int]
+        L.qwsg_bicgstab_precdd.argtypes = [vp, ctypes.c_int, vp, vp, ctypes.c_double, ctypes.c_int, ctypes.c_int,
+                                           ctypes.c_int, ip]
+        L.qwsg_solve.argtypes = [vp, vp, vp, ctypes.POINTER(SolveParams), ctypes.POINTER(SolveStats)]
+        L.qwsg_norm2.argtypes = [vp, ctypes.c_int, vp, ctypes.POINTER(ctypes.c_double)]
+        L.qwsg_synchronize.argtypes = [vp]
+        L.qwsg_time_dslash.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
+        L.qwsg_device_count.argtypes = [ip]
+        _lib = L
+    return _lib
+
+
+def check(rc):
+    if rc != 0:
+        raise QwsError(lib().qwsg_last_error().decode())
+
+
+def declared_functions():
+    """Function names declared in include/qws_mi355x.h."""
+    import re
+    txt = open(HEADER).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    names = re.findall(r"\b([A-Za-z_][A-Za-z0-9_]*)\s*\(", txt)
+    skip = {"if", "for", "while", "sizeof", "return", "defined"}
+    return sorted({n for n in names if n not in skip and not n.isupper()})
+
+
+def _p(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+class Spinor:
+    """A device spinor field (precision F64/F32/F16) owned by the library."""
+
+    def __init__(self, lat, prec):
+        self.lat, self.prec = lat, prec
+        p = ctypes.c_void_p()
+        check(lib().qwsg_spinor_alloc(lat.h, prec, ctypes.byref(p)))
+        self.ptr = p
+
+    def __del__(self):
+        try:
+            if self.ptr:
+                lib().qwsg_spinor_free(self.ptr)
+        except Exception:
+            pass
+
+    def upload(self, host):
+        a = np.ascontiguousarray(host, dtype=np.float64)
+        assert a.size == self.lat.V * 24
+        check(lib().qwsg_spinor_upload(self.lat.h, self.prec, self.ptr, _p(a)))
+        return self
+
+    def download(self):
+        out = np.empty((self.lat.V, 3, 4, 2))
+        check(lib().qwsg_spinor_download(self.lat.h, self.prec, self.ptr, _p(out)))
+        return out
+
+    def zero(self):
+        check(lib().qwsg_spinor_zero(self.lat.h, self.prec, self.ptr))
+        return self
+
+    def norm2(self):
+        v = ctypes.c_double()
+        check(lib().qwsg_norm2(self.lat.h, self.prec, self.ptr, ctypes.byref(v)))
+        return v.value
+
+
+class Lattice:
+    """One lattice with its gauge and clover fields resident on the GPU.
+
+    L: extents (x, y, z, t); B: SAP block extents; tbc: t boundary (+1/-1);
+    links32: FP32 link storage, 18 reals or 12 (third row rebuilt; SU(3) only)."""
+
+    def __init__(self, L, B, tbc=-1, links32=18, kappa=None):
+        self.L, self.B = list(L), list(B)
+        h = ctypes.c_void_p()
+        Li = (ctypes.c_int * 4)(*L)
+        Bi = (ctypes.c_int * 4)(*B)
+        check(lib().qwsg_create(Li, Bi, int(tbc), int(links32), ctypes.byref(h)))
+        self.h = h
+        v = ctypes.c_longlong()
+        check(lib().qwsg_volume(h, ctypes.byref(v)))
+        self.V = v.value
+        if kappa is not None:
+            self.set_kappa(kappa)
+
+    def close(self):
+        if self.h:
+            check(lib().qwsg_destroy(self.h))
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_kappa(self, k):
+        self.kappa = float(k)
+        check(lib().qwsg_set_kappa(self.h, self.kappa))
+
+    def load_gauge(self, U):
+        check(lib().qwsg_load_gauge(self.h, _p(np.ascontiguousarray(U, dtype=np.float64))))
+
+    def load_clover(self, C):
+        check(lib().qwsg_load_clover(self.h, _p(np.ascontiguousarray(C, dtype=np.float64))))
+
+    def load_gauge_f32(self, U):
+        check(lib().qwsg_load_gauge_f32(self.h, _p(np.ascontiguousarray(U, dtype=np.float32))))
+
+    def load_clover_f32(self, C):
+        check(lib().qwsg_load_clover_f32(self.h, _p(np.ascontiguousarray(C, dtype=np.float32))))
+
+    def spinor(self, prec, host=None):
+        s = Spinor(self, prec)
+        if host is not None:
+            s.upload(host)
+        return s
+
+    def dslash(self, out, inp, hops=HOPS_ALL, colour=-1):
+        check(lib().qwsg_dslash(self.h, inp.prec, out.ptr, inp.ptr, hops, colour))
+
+    def face(self, out, inp, factor, colour=-1):
+        check(lib().qwsg_face(self.h, inp.prec, out.ptr, inp.ptr, float(factor), colour))
+
+    def clover(self, out, inp):
+        check(lib().qwsg_clover(self.h, inp.prec, out.ptr, inp.ptr))
+
+    def jinv(self, x, b, colour, nm):
+        check(lib().qwsg_jinv(self.h, b.prec, x.ptr, b.ptr, colour, nm))
+
+    def prec_ddd(self, out, inp, nsap, nm):
+        check(lib().qwsg_prec_ddd(self.h, inp.prec, out.ptr, inp.ptr, nsap, nm))
+
+    def prec_s(self, out, inp, nsap, nm):
+        check(lib().qwsg_prec_s(self.h, inp.prec, out.ptr, inp.ptr, nsap, nm))
+
+    def bicgstab_precdd(self, x, b, tol, maxiter, nsap, nm):
+        it = ctypes.c_int()
+        check(lib().qwsg_bicgstab_precdd(self.h, b.prec, x.ptr, b.ptr, float(tol), maxiter, nsap, nm,
+                                         ctypes.byref(it)))
+        return it.value
+
+    def solve(self, x, b, nsap=4, nm=2, tol=-1.0, maxiter=6, tol_s=-1.0, maxiter_s=50, true_tol=-1.0,
+              max_restarts=0, sap_precision=F32):
+        p = SolveParams(nsap, nm, tol, maxiter, tol_s, maxiter_s, true_tol, max_restarts, sap_precision)
+        st = SolveStats()
+        check(lib().qwsg_solve(self.h, x.ptr, b.ptr, ctypes.byref(p), ctypes.byref(st)))
+        return {k: getattr(st, k) for k, _ in SolveStats._fields_}
+
+    def sync(self):
+        check(lib().qwsg_synchronize(self.h))
+
+    def time_dslash(self, prec, reps):
+        ms = ctypes.c_double()
+        check(lib().qwsg_time_dslash(self.h, prec, reps, ctypes.byref(ms)))
+        return ms.value
+
+
+def device_count():
+    n = ctypes.c_int()
+    check(lib().qwsg_device_count(ctypes.byref(n)))
+    return n.value

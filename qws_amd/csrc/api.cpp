@@ -1,0 +1,661 @@
+// api.cpp -- the C ABI of libqws_mi355x.so: the device-resident qwsg_*
+// interface and the QWS drop-in entry points with their host mirrors.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/qws_mi355x.h"
+#include "lattice.h"
+
+using qws::Lattice;
+
+struct qwsg_lattice {
+  Lattice* lat;
+};
+
+namespace {
+thread_local std::string g_err;
+
+template <typename F> int guarded(F&& f) {
+  try {
+    f();
+    return 0;
+  } catch (const qws::HipError& e) {
+    g_err = e.what();
+    return 2;
+  } catch (const std::invalid_argument& e) {
+    g_err = e.what();
+    return 1;
+  } catch (const std::exception& e) {
+    g_err = e.what();
+    return 3;
+  }
+}
+}  // namespace
+
+extern "C" {
+
+const char* qwsg_last_error(void) { return g_err.c_str(); }
+
+int qwsg_device_count(int* n) {
+  return guarded([&] { QWS_HIP(hipGetDeviceCount(n)); });
+}
+
+int qwsg_create(const int* L, const int* B, int tbc, int links32, qwsg_lattice** out) {
+  return guarded([&] {
+    Lattice* l = new Lattice(L, B, tbc, links32);
+    *out = new qwsg_lattice{l};
+  });
+}
+int qwsg_destroy(qwsg_lattice* h) {
+  return guarded([&] {
+    if (h) {
+      delete h->lat;
+      delete h;
+    }
+  });
+}
+int qwsg_set_kappa(qwsg_lattice* h, double k) {
+  return guarded([&] { h->lat->set_kappa(k); });
+}
+int qwsg_volume(qwsg_lattice* h, long long* V) {
+  return guarded([&] { *V = h->lat->volume(); });
+}
+int qwsg_load_gauge(qwsg_lattice* h, const double* U) {
+  return guarded([&] { h->lat->load_gauge(U, 0, 1, 3); });
+}
+int qwsg_load_clover(qwsg_lattice* h, const double* C) {
+  return guarded([&] { h->lat->load_clover(C, 0, 1, 3); });
+}
+int qwsg_load_gauge_f32(qwsg_lattice* h, const float* U) {
+  return guarded([&] { h->lat->load_gauge(U, 0, 0, 2); });
+}
+int qwsg_load_clover_f32(qwsg_lattice* h, const float* C) {
+  return guarded([&] { h->lat->load_clover(C, 0, 0, 2); });
+}
+int qwsg_spinor_alloc(qwsg_lattice* h, int prec, void** dev) {
+  return guarded([&] { *dev = h->lat->alloc_spinor(prec); });
+}
+int qwsg_spinor_free(void* dev) {
+  return guarded([&] { QWS_HIP(hipFree(dev)); });
+}
+int qwsg_spinor_upload(qwsg_lattice* h, int prec, void* dev, const double* host) {
+  return guarded([&] { h->lat->upload_spinor(prec, dev, host, 0, 1); });
+}
+int qwsg_spinor_download(qwsg_lattice* h, int prec, const void* dev, double* host) {
+  return guarded([&] { h->lat->download_spinor(prec, dev, host, 0, 1); });
+}
+int qwsg_spinor_zero(qwsg_lattice* h, int prec, void* dev) {
+  return guarded([&] {
+    QWS_HIP(hipMemsetAsync(dev, 0, h->lat->spinor_bytes(prec), h->lat->stream()));
+  });
+}
+int qwsg_dslash(qwsg_lattice* h, int prec, void* out, const void* in, int hops, int colour) {
+  return guarded([&] { h->lat->dslash(prec, out, in, hops, colour); });
+}
+int qwsg_face(qwsg_lattice* h, int prec, void* out, const void* in, double factor, int colour) {
+  return guarded([&] { h->lat->face(prec, out, in, factor, colour); });
+}
+int qwsg_clover(qwsg_lattice* h, int prec, void* out, const void* in) {
+  return guarded([&] { h->lat->clover(prec, out, in); });
+}
+int qwsg_jinv(qwsg_lattice* h, int prec, void* x, const void* b, int colour, int nm) {
+  return guarded([&] { h->lat->jinv(prec, x, b, colour, nm); });
+}
+int qwsg_prec_ddd(qwsg_lattice* h, int prec, void* out, const void* in, int nsap, int nm) {
+  return guarded([&] { h->lat->prec_ddd(prec, out, in, nsap, nm); });
+}
+int qwsg_prec_s(qwsg_lattice* h, int prec, void* out, const void* in, int nsap, int nm) {
+  return guarded([&] { h->lat->prec_s(prec, out, in, nsap, nm); });
+}
+int qwsg_bicgstab_precdd(qwsg_lattice* h, int prec, void* x, const void* b, double tol, int maxiter, int nsap, int nm,
+                         int* iters) {
+  return guarded([&] {
+    int it = h->lat->bicgstab_precdd(prec, x, b, tol, maxiter, nsap, nm);
+    if (iters) *iters = it;
+  });
+}
+int qwsg_solve(qwsg_lattice* h, void* x, const void* b, const qwsg_solve_params* p, qwsg_solve_stats* st) {
+  return guarded([&] {
+    qws::SolveParams sp;
+    sp.nsap = p->nsap;
+    sp.nm = p->nm;
+    sp.tol = p->tol;
+    sp.maxiter = p->maxiter;
+    sp.tol_s = p->tol_s;
+    sp.maxiter_s = p->maxiter_s;
+    sp.true_tol = p->true_tol;
+    sp.max_restarts = p->max_restarts;
+    sp.sap_prec = p->sap_precision;
+    if (sp.nsap > 10 || sp.nm > 10) throw std::invalid_argument("nsap and nm must be <= 10");
+    qws::SolveStats s = h->lat->solve((double*)x, (const double*)b, sp);
+    if (st) {
+      st->outer_iters = s.outer_iters;
+      st->inner_iters = s.inner_iters;
+      st->restarts = s.restarts;
+      st->true_residual = s.true_residual;
+      st->seconds = s.seconds;
+    }
+  });
+}
+int qwsg_norm2(qwsg_lattice* h, int prec, const void* a, double* out) {
+  return guarded([&] { *out = h->lat->norm2(prec, a); });
+}
+int qwsg_synchronize(qwsg_lattice* h) {
+  return guarded([&] { h->lat->sync(); });
+}
+int qwsg_time_dslash(qwsg_lattice* h, int prec, int reps, double* ms) {
+  return guarded([&] { *ms = h->lat->time_dslash(prec, reps); });
+}
+
+// ============================================================================ QWS drop-in
+// Host mirrors of QWS's globals (qws.cc:74-124).
+glud_t* glud = nullptr;
+glus_t* glus = nullptr;
+clvd_t* clvd = nullptr;
+clvs_t* clvs = nullptr;
+double kappa = 0, kappa2 = 0, mkappa = 0;
+int nx = 0, ny = 0, nz = 0, nt = 0, nxh = 0, nxd = 0, nxs = 0, vold = 0, vols = 0;
+int npe[4] = {1, 1, 1, 1};
+double fbc[4][2] = {{2, 2}, {2, 2}, {2, 2}, {2, 2}};
+int rank = 0, domain_e = 0, domain_o = 1;
+
+// BQCD provides these (qws_bqcd.cc:56-57); weak so that the library loads without BQCD.
+int std_xyzt2i_(int*) __attribute__((weak));
+int e_o_(int*) __attribute__((weak));
+
+}  // extern "C"
+
+namespace {
+
+struct DropIn {
+  Lattice* lat = nullptr;
+  bool dirty = true;
+  bool default_blocks = true;
+  int tbc = -1;
+  int* pce = nullptr;
+  int* pco = nullptr;
+  void* dev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};  // d0 d1 s0 s1 pre_d pre_s
+  std::vector<char> host_tmp[2];
+};
+DropIn g_qws;
+
+[[noreturn]] void fatal(const char* where, const std::string& what) {
+  std::printf("ERROR at %s : %s\n", where, what.c_str());
+  std::fflush(stdout);
+  std::exit(1);
+}
+
+template <typename F> void drop_in(const char* where, F&& f) {
+  try {
+    f();
+  } catch (const std::exception& e) {
+    fatal(where, e.what());
+  }
+}
+
+Lattice& lat_sync(const char* where) {
+  if (!g_qws.lat) fatal(where, "qws_init_ has not been called");
+  if (g_qws.dirty) {
+    g_qws.lat->load_gauge(glud, 1, 1, 1);
+    g_qws.lat->load_gauge(glus, 1, 0, 2);
+    g_qws.lat->load_clover(clvd, 1, 1, 1);
+    g_qws.lat->load_clover(clvs, 1, 0, 2);
+    g_qws.dirty = false;
+  }
+  g_qws.lat->set_kappa(kappa);  // the caller may set kappa directly (main.cc:173-175)
+  return *g_qws.lat;
+}
+
+void* dbuf(int i, int prec) {
+  if (!g_qws.dev[i]) g_qws.dev[i] = g_qws.lat->alloc_spinor(prec);
+  return g_qws.dev[i];
+}
+
+size_t dom_bytes_d() { return sizeof(scd_t) * (size_t)vold; }
+size_t dom_bytes_s() { return sizeof(scs_t) * (size_t)vols; }
+
+// a full-lattice host AoSoA array holding `part` (one domain) at domain d, zero elsewhere
+const void* embed(int slot, const void* part, size_t dom_bytes, int d) {
+  auto& buf = g_qws.host_tmp[slot];
+  buf.assign(2 * dom_bytes, 0);
+  std::memcpy(buf.data() + d * dom_bytes, part, dom_bytes);
+  return buf.data();
+}
+
+void require_domain_blocks(const char* where) {
+  if (!g_qws.default_blocks) fatal(where, "per-domain entry points need the default SAP domains (x-halves)");
+}
+
+void release_drop_in() {
+  for (auto& p : g_qws.dev)
+    if (p) {
+      hipFree(p);
+      p = nullptr;
+    }
+  delete g_qws.lat;
+  g_qws.lat = nullptr;
+}
+
+void create_drop_in_lattice(const int B[4]) {
+  int L[4] = {nx, ny, nz, nt};
+  release_drop_in();
+  g_qws.lat = new Lattice(L, B, g_qws.tbc, 18);
+  g_qws.dirty = true;
+}
+
+}  // namespace
+
+extern "C" {
+
+void qws_init_(int* lx, int* ly, int* lz, int* lt, int* npe_f, int* fbc_f, int* pce_f, int* pco_f, int* block_size) {
+  (void)block_size;
+  g_qws.pce = pce_f;  // kept, as the reference does (qws.cc:134-135)
+  g_qws.pco = pco_f;
+  for (int i = 0; i < 4; ++i) npe[i] = npe_f[i];
+  if (npe[0] * npe[1] * npe[2] * npe[3] != 1)
+    fatal(__func__, "this library runs one rank on one GPU: npe must be (1,1,1,1)");
+  nx = *lx;
+  ny = *ly;
+  nz = *lz;
+  nt = *lt;
+  nxh = nx / 2;
+  nxd = nx / 2 / QWS_VLEND;
+  nxs = nx / 2 / QWS_VLENS;
+  if (nxd == 0) { std::printf("ERROR at %s : nx/2/VLEND must be >=1\n", __func__); std::exit(1); }
+  if (nxs == 0) { std::printf("ERROR at %s : nx/2/VLENS must be >=1\n", __func__); std::exit(1); }
+  if (nxh % QWS_VLENS != 0) fatal(__func__, "nx/2 must be a multiple of 16");
+  vold = nxd * ny * nz * nt;
+  vols = nxs * ny * nz * nt;
+  rank = 0;
+  domain_e = 0;  // (py+pz+pt)%2 on the single rank (qws.cc:275-276)
+  domain_o = 1;
+  // t boundary (qws.cc:298-310, single rank: pt == 0 == npe[3]-1)
+  if (fbc_f[3] != 1) {
+    fbc[3][0] = fbc[3][1] = (double)(fbc_f[3] * 2);
+  } else {
+    fbc[3][0] = fbc[3][1] = 2;
+  }
+  if (fbc_f[3] != 1 && fbc_f[3] != -1) fatal(__func__, "fbc_f[3] must be +1 or -1");
+  g_qws.tbc = fbc_f[3];
+  auto alloc = [](size_t bytes) {
+    void* p = nullptr;
+    if (posix_memalign(&p, 256, bytes) != 0) fatal("qws_init_", "host allocation failed");
+    std::memset(p, 0, bytes);
+    return p;
+  };
+  std::free(glud);
+  std::free(glus);
+  std::free(clvd);
+  std::free(clvs);
+  glud = (glud_t*)alloc(sizeof(glud_t) * (size_t)vold * 4 * 2);
+  clvd = (clvd_t*)alloc(sizeof(clvd_t) * (size_t)vold * 2);
+  glus = (glus_t*)alloc(sizeof(glus_t) * (size_t)vols * 4 * 2);
+  clvs = (clvs_t*)alloc(sizeof(clvs_t) * (size_t)vols * 2);
+  int B[4] = {nxh, ny, nz, nt};
+  g_qws.default_blocks = true;
+  drop_in(__func__, [&] { create_drop_in_lattice(B); });
+}
+
+void qws_finalize_(void) {
+  release_drop_in();
+  std::free(glud);
+  std::free(glus);
+  std::free(clvd);
+  std::free(clvs);
+  glud = nullptr;
+  glus = nullptr;
+  clvd = nullptr;
+  clvs = nullptr;
+}
+
+void qws_mi355x_fields_modified_(void) { g_qws.dirty = true; }
+
+void qws_mi355x_set_sap_blocks_(int* b) {
+  if (!g_qws.lat) fatal(__func__, "qws_init_ has not been called");
+  int B[4] = {b[0], b[1], b[2], b[3]};
+  g_qws.default_blocks = (B[0] == nxh && B[1] == ny && B[2] == nz && B[3] == nt);
+  drop_in(__func__, [&] { create_drop_in_lattice(B); });
+}
+
+// ---- operators on host AoSoA arrays
+void ddd_d_(scd_t* out, scd_t* in) {
+  drop_in(__func__, [&] {
+    Lattice& L = lat_sync("ddd_d_");
+    void* a = dbuf(0, qws::P64);
+    void* b = dbuf(1, qws::P64);
+    L.upload_spinor(qws::P64, a, in, 1, 1);
+    L.dslash(qws::P64, b, a, qws::HOPS_ALL, -1);
+    L.download_spinor(qws::P64, b, out, 1, 1);
+  });
+}
+
+void ddd_s_(scs_t* out, scs_t* in) {
+  drop_in(__func__, [&] {
+    Lattice& L = lat_sync("ddd_s_");
+    void* a = dbuf(2, qws::P32);
+    void* b = dbuf(3, qws::P32);
+    L.upload_spinor(qws::P32, a, in, 1, 0);
+    L.dslash(qws::P32, b, a, qws::HOPS_ALL, -1);
+    L.download_spinor(qws::P32, b, out, 1, 0);
+  });
+}
+
+// domain-local operator: out, in point at domain d's data (ddd_in_s.cc:72-84)
+static void ddd_in_generic(int prec, void* out, const void* in, int d, const char* where) {
+  require_domain_blocks(where);
+  Lattice& L = lat_sync(where);
+  const bool dbl = prec == qws::P64;
+  const size_t db = dbl ? dom_bytes_d() : dom_bytes_s();
+  void* a = dbuf(dbl ? 0 : 2, prec);
+  void* b = dbuf(dbl ? 1 : 3, prec);
+  L.upload_spinor(prec, a, embed(0, in, db, d), 1, dbl);
+  L.dslash(prec, b, a, qws::HOPS_INSIDE, d);
+  auto& full = g_qws.host_tmp[1];
+  full.resize(2 * db);
+  L.download_spinor(prec, b, full.data(), 1, dbl);
+  std::memcpy(out, full.data() + d * db, db);
+}
+
+void ddd_in_s_(scs_t* out, scs_t* in, int* domain) {
+  drop_in(__func__, [&] { ddd_in_generic(qws::P32, out, in, *domain, "ddd_in_s_"); });
+}
+void ddd_in_d_(scd_t* out, scd_t* in, int* domain) {
+  drop_in(__func__, [&] { ddd_in_generic(qws::P64, out, in, *domain, "ddd_in_d_"); });
+}
+
+// pre: record the faces of `in` (the whole field), pos: out (domain d's data) +=
+// factor T^-1 H_face(recorded field) (ddd_out_s.cc:112-626, ddd_out_d.cc:84-235)
+static void ddd_out_pre_generic(int prec, const void* in, const char* where) {
+  require_domain_blocks(where);
+  Lattice& L = lat_sync(where);
+  const bool dbl = prec == qws::P64;
+  L.upload_spinor(prec, dbuf(dbl ? 4 : 5, prec), in, 1, dbl);
+}
+static void ddd_out_pos_generic(int prec, void* out, int d, double factor, const char* where) {
+  require_domain_blocks(where);
+  Lattice& L = lat_sync(where);
+  const bool dbl = prec == qws::P64;
+  const size_t db = dbl ? dom_bytes_d() : dom_bytes_s();
+  void* o = dbuf(dbl ? 0 : 2, prec);
+  L.upload_spinor(prec, o, embed(0, out, db, d), 1, dbl);
+  L.face(prec, o, dbuf(dbl ? 4 : 5, prec), factor, d);
+  auto& full = g_qws.host_tmp[1];
+  full.resize(2 * db);
+  L.download_spinor(prec, o, full.data(), 1, dbl);
+  std::memcpy(out, full.data() + d * db, db);
+}
+void ddd_out_pre_s_(scs_t* in, int* domain) {
+  (void)domain;
+  drop_in(__func__, [&] { ddd_out_pre_generic(qws::P32, in, "ddd_out_pre_s_"); });
+}
+void ddd_out_pos_s_(scs_t* out, scs_t* in, int* domain, float factor) {
+  (void)in;
+  drop_in(__func__, [&] { ddd_out_pos_generic(qws::P32, out, *domain, factor, "ddd_out_pos_s_"); });
+}
+void ddd_out_pre_d_(scd_t* in, int* domain) {
+  (void)domain;
+  drop_in(__func__, [&] { ddd_out_pre_generic(qws::P64, in, "ddd_out_pre_d_"); });
+}
+void ddd_out_pos_d_(scd_t* out, scd_t* in, int* domain) {
+  (void)in;
+  drop_in(__func__, [&] { ddd_out_pos_generic(qws::P64, out, *domain, mkappa, "ddd_out_pos_d_"); });
+}
+
+void jinv_ddd_in_s_(scs_t* x, scs_t* b, int* domain, int* maxiter) {
+  drop_in(__func__, [&] {
+    require_domain_blocks("jinv_ddd_in_s_");
+    Lattice& L = lat_sync("jinv_ddd_in_s_");
+    const int d = *domain;
+    const size_t db = dom_bytes_s();
+    void* bb = dbuf(2, qws::P32);
+    void* xx = dbuf(3, qws::P32);
+    L.upload_spinor(qws::P32, bb, embed(0, b, db, d), 1, 0);
+    L.jinv(qws::P32, xx, bb, d, *maxiter);
+    auto& full = g_qws.host_tmp[1];
+    full.resize(2 * db);
+    L.download_spinor(qws::P32, xx, full.data(), 1, 0);
+    std::memcpy(x, full.data() + d * db, db);
+  });
+}
+
+void prec_ddd_s_(scs_t* out, scs_t* in, int* nsap, int* nm) {
+  if (*nsap > 10) { std::printf("prec_ddd_s_ nsap > 10\n"); std::exit(1); }
+  if (*nm > 10) { std::printf("prec_ddd_s_ nn   > 10\n"); std::exit(1); }
+  drop_in(__func__, [&] {
+    Lattice& L = lat_sync("prec_ddd_s_");
+    void* a = dbuf(2, qws::P32);
+    void* b = dbuf(3, qws::P32);
+    L.upload_spinor(qws::P32, a, in, 1, 0);
+    L.prec_ddd(qws::P32, b, a, *nsap, *nm);
+    L.download_spinor(qws::P32, b, out, 1, 0);
+  });
+}
+
+void prec_s_(scs_t* out, scs_t* in, int* nsap, int* nm) {
+  drop_in(__func__, [&] {
+    Lattice& L = lat_sync("prec_s_");
+    void* a = dbuf(2, qws::P32);
+    void* b = dbuf(3, qws::P32);
+    L.upload_spinor(qws::P32, a, in, 1, 0);
+    L.prec_s(qws::P32, b, a, *nsap, *nm);
+    L.download_spinor(qws::P32, b, out, 1, 0);
+  });
+}
+
+void bicgstab_precdd_s_(scs_t* x, scs_t* b, double* tol, int* conviter, int* maxiter, int* nsap, int* nm) {
+  drop_in(__func__, [&] {
+    Lattice& L = lat_sync("bicgstab_precdd_s_");
+    void* bb = dbuf(2, qws::P32);
+    void* xx = dbuf(3, qws::P32);
+    L.upload_spinor(qws::P32, bb, b, 1, 0);
+    int it = L.bicgstab_precdd(qws::P32, xx, bb, *tol, *maxiter, *nsap, *nm);
+    L.download_spinor(qws::P32, xx, x, 1, 0);
+    if (conviter) *conviter = it;
+  });
+}
+
+void bicgstab_dd_mix_(scd_t* x, scd_t* b, double* tol, int* conviter, int* maxiter, double* tol_s, int* maxiter_s,
+                      int* nsap, int* nm) {
+  drop_in(__func__, [&] {
+    Lattice& L = lat_sync("bicgstab_dd_mix_");
+    double* xx = (double*)dbuf(0, qws::P64);
+    double* bb = (double*)dbuf(1, qws::P64);
+    L.upload_spinor(qws::P64, xx, x, 1, 1);
+    L.upload_spinor(qws::P64, bb, b, 1, 1);
+    qws::SolveParams p;
+    p.tol = *tol;
+    p.maxiter = *maxiter;
+    p.tol_s = *tol_s;
+    p.maxiter_s = *maxiter_s;
+    p.nsap = *nsap;
+    p.nm = *nm;
+    int inner = 0;
+    int it = L.bicgstab_dd_mix(xx, bb, p, &inner);
+    L.download_spinor(qws::P64, xx, x, 1, 1);
+    if (conviter) *conviter = it;
+  });
+}
+
+// clover inverse on the first vold elements with the clover of block `pe`
+// (qws.cc:390-443; in the DD layout pe selects the x-domain)
+static void clvd_generic(int pe, scd_t* out, const scd_t* in) {
+  Lattice& L = lat_sync("clvd_vm_");
+  const size_t db = dom_bytes_d();
+  void* a = dbuf(0, qws::P64);
+  void* b = dbuf(1, qws::P64);
+  L.upload_spinor(qws::P64, a, embed(0, in, db, pe), 1, 1);
+  L.clover(qws::P64, b, a);
+  auto& full = g_qws.host_tmp[1];
+  full.resize(2 * db);
+  L.download_spinor(qws::P64, b, full.data(), 1, 1);
+  std::memcpy(out, full.data() + pe * db, db);
+}
+void clvd_vm_(int* pe, scd_t* inout) {
+  drop_in(__func__, [&] { clvd_generic(*pe, inout, inout); });
+}
+void clvd_vm2_(int* pe, scd_t* out, scd_t* in) {
+  drop_in(__func__, [&] { clvd_generic(*pe, out, in); });
+}
+
+// ---- FP64 <-> FP32 with normalisation (reordering.cc:108-204), host side
+void fermi_reorder_d2s_dd_renorm_(scs_t* out, const scd_t* in, double* dnorm) {
+  const size_t n = (size_t)vold * 2;
+  double acc = 0;
+  for (size_t i = 0; i < n; ++i)
+    for (int c = 0; c < 3; ++c)
+      for (int s = 0; s < 4; ++s)
+        for (int ri = 0; ri < 2; ++ri)
+          for (int j = 0; j < QWS_VLEND; ++j) acc += in[i].c[c][s][ri][j] * in[i].c[c][s][ri][j];
+  acc = std::sqrt(acc);
+  *dnorm = acc;
+  const double inv = 1.0 / acc;
+  for (int x = 0; x < nx; ++x)
+    for (int t = 0; t < nt; ++t)
+      for (int z = 0; z < nz; ++z)
+        for (int y = 0; y < ny; ++y) {
+          const int id = (x % nxh) / QWS_VLEND + nxd * (y + ny * (z + nz * t)) + vold * (x / nxh);
+          const int jd = (x % nxh) % QWS_VLEND;
+          const int is = (x % nxh) / QWS_VLENS + nxs * (y + ny * (z + nz * t)) + vols * (x / nxh);
+          const int js = (x % nxh) % QWS_VLENS;
+          for (int c = 0; c < 3; ++c)
+            for (int s = 0; s < 4; ++s)
+              for (int ri = 0; ri < 2; ++ri) out[is].c[c][s][ri][js] = (float)(in[id].c[c][s][ri][jd] * inv);
+        }
+}
+
+void fermi_reorder_s2d_dd_renorm_(scd_t* out, const scs_t* in, const double* dnorm) {
+  for (int x = 0; x < nx; ++x)
+    for (int t = 0; t < nt; ++t)
+      for (int z = 0; z < nz; ++z)
+        for (int y = 0; y < ny; ++y) {
+          const int id = (x % nxh) / QWS_VLEND + nxd * (y + ny * (z + nz * t)) + vold * (x / nxh);
+          const int jd = (x % nxh) % QWS_VLEND;
+          const int is = (x % nxh) / QWS_VLENS + nxs * (y + ny * (z + nz * t)) + vols * (x / nxh);
+          const int js = (x % nxh) % QWS_VLENS;
+          for (int c = 0; c < 3; ++c)
+            for (int s = 0; s < 4; ++s)
+              for (int ri = 0; ri < 2; ++ri)
+                out[id].c[c][s][ri][jd] = (double)in[is].c[c][s][ri][js] * (*dnorm);
+        }
+}
+
+// ---- BQCD loaders (qws_bqcd.cc:125-642); site order and parity come from BQCD.
+static void need_bqcd(const char* where) {
+  if (!std_xyzt2i_ || !e_o_) fatal(where, "BQCD's std_xyzt2i_/e_o_ are not linked");
+}
+
+// QWS clover index k <- BQCD index (qws_bqcd.cc:301-337)
+static int bqcd_clover_index(int k) {
+  if (k < 2) return k;
+  if (k < 4) return 20 + (k - 2);
+  if (k < 6) return 32 + (k - 4);
+  if (k < 24) return 2 + (k - 6);
+  if (k < 34) return 22 + (k - 24);
+  return k;
+}
+
+void qws_loadg_dd_(double* u, int* volh_tot, double* in_kappa) {
+  need_bqcd(__func__);
+  kappa = *in_kappa;
+  mkappa = -kappa;
+  kappa2 = kappa * kappa;
+  for (int t = 0; t < nt; ++t)
+    for (int z = 0; z < nz; ++z)
+      for (int y = 0; y < ny; ++y)
+        for (int x = 0; x < nx; ++x) {
+          int j[4] = {x, y, z, t};
+          const int i = std_xyzt2i_(j);
+          const int eo = e_o_(j);
+          const long ihh = (long)(i - 1) * 18 + (long)eo * (*volh_tot) * 18;
+          const int iwsd = (x % nxh) / QWS_VLEND + nxd * y + nxd * ny * z + nxd * ny * nz * t + 4 * vold * (x / nxh);
+          const int ixxd = (x % nxh) % QWS_VLEND;
+          const int iwss = (x % nxh) / QWS_VLENS + nxs * y + nxs * ny * z + nxs * ny * nz * t + 4 * vols * (x / nxh);
+          const int ixxs = (x % nxh) % QWS_VLENS;
+          for (int mu = 0; mu < 4; ++mu)
+            for (int c1 = 0; c1 < 3; ++c1)
+              for (int c2 = 0; c2 < 3; ++c2)
+                for (int ri = 0; ri < 2; ++ri) {
+                  const double v = u[ri + 2 * c1 + 6 * c2 + ihh + (long)mu * (*volh_tot) * 36];
+                  glud[iwsd + vold * mu].c[c2][c1][ri][ixxd] = v;
+                  glus[iwss + vols * mu].c[c2][c1][ri][ixxs] = (float)v;
+                }
+        }
+  g_qws.dirty = true;
+}
+
+void qws_loadc_dd_(double* c, int* volh) {
+  need_bqcd(__func__);
+  for (int t = 0; t < nt; ++t)
+    for (int z = 0; z < nz; ++z)
+      for (int y = 0; y < ny; ++y)
+        for (int x = 0; x < nx; ++x) {
+          int j[4] = {x, y, z, t};
+          const int i = std_xyzt2i_(j);
+          const int eo = e_o_(j);
+          const long ihh = (long)(i - 1) * 72 + (long)eo * (*volh) * 72;
+          const int iwsd = (x % nxh) / QWS_VLEND + nxd * y + nxd * ny * z + nxd * ny * nz * t + vold * (x / nxh);
+          const int ixxd = (x % nxh) % QWS_VLEND;
+          const int iwss = (x % nxh) / QWS_VLENS + nxs * y + nxs * ny * z + nxs * ny * nz * t + vols * (x / nxh);
+          const int ixxs = (x % nxh) % QWS_VLENS;
+          for (int ud = 0; ud < 2; ++ud)
+            for (int k = 0; k < 36; ++k) {
+              const double v = c[bqcd_clover_index(k) + ud * 36 + ihh];
+              clvd[iwsd].c[ud][k][ixxd] = v;
+              clvs[iwss].c[ud][k][ixxs] = (float)v;
+            }
+        }
+  g_qws.dirty = true;
+}
+
+}  // extern "C"
+
+template <typename T, int VL>
+static void bqcd_spinor(T* dd, double* ine, double* ino, bool load) {
+  for (int t = 0; t < nt; ++t)
+    for (int z = 0; z < nz; ++z)
+      for (int y = 0; y < ny; ++y)
+        for (int x = 0; x < nx; ++x) {
+          int j[4] = {x, y, z, t};
+          const int i = std_xyzt2i_(j);
+          const int eo = e_o_(j);
+          const long ihh = (long)(i - 1) * 24;
+          const int nxv = nxh / VL;
+          const int vv = nxv * ny * nz * nt;
+          const int iws = (x % nxh) / VL + nxv * y + nxv * ny * z + nxv * ny * nz * t + vv * (x / nxh);
+          const int ixx = (x % nxh) % VL;
+          double* h = eo == 0 ? ine : ino;
+          for (int c = 0; c < 3; ++c)
+            for (int s = 0; s < 4; ++s)
+              for (int ri = 0; ri < 2; ++ri) {
+                if (load) dd[iws].c[c][s][ri][ixx] = h[ri + 2 * s + 8 * c + ihh];
+                else h[ri + 2 * s + 8 * c + ihh] = (double)dd[iws].c[c][s][ri][ixx];
+              }
+        }
+}
+
+extern "C" {
+
+void qws_loadfd_dd_(scd_t* out, double* ine, double* ino) {
+  need_bqcd(__func__);
+  bqcd_spinor<scd_t, QWS_VLEND>(out, ine, ino, true);
+}
+void qws_storefd_dd_(double* oute, double* outo, scd_t* in) {
+  need_bqcd(__func__);
+  bqcd_spinor<scd_t, QWS_VLEND>(in, oute, outo, false);
+}
+void qws_loadfs_dd_(scs_t* out, double* ine, double* ino) {
+  need_bqcd(__func__);
+  bqcd_spinor<scs_t, QWS_VLENS>(out, ine, ino, true);
+}
+void qws_storefs_dd_(double* oute, double* outo, scs_t* in) {
+  need_bqcd(__func__);
+  bqcd_spinor<scs_t, QWS_VLENS>(in, oute, outo, false);
+}
+
+}  // extern "C"

@@ -1,0 +1,362 @@
+// blas.hip -- fused BiCGStab vector updates and reductions.
+//
+// Vectors are flat arrays of interleaved (re,im) pairs; the same kernels serve
+// the FP32 inner solver (bicgstab_precdd_s.cc:85-559) and the FP64 outer solver
+// (bicgstab_dd_mix.cc:105-388).  Every reduction accumulates in FP64 and writes
+// one partial per workgroup; launch_finalize sums the partials in a fixed order
+// (bit-reproducible) and updates the solver scalars in device memory, so a
+// BiCGStab iteration needs no host round trip unless the caller asks for the
+// residual.
+#include <math.h>
+
+#include "kernels.h"
+
+namespace qws {
+
+template <typename R> struct V2;
+template <> struct V2<float> { using T = float2; };
+template <> struct V2<double> { using T = double2; };
+
+// Block-wide sum of NV doubles; thread 0 writes part[v * RED_BLOCKS + blockIdx.x].
+template <int NV>
+__device__ __forceinline__ void block_reduce_store(double (&v)[NV], double* part) {
+  __shared__ double sh[NV][RED_THREADS / 64];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < NV; ++k) {
+    double x = v[k];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
+    if (lane == 0) sh[k][wv] = x;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+      double x = 0;
+      for (int w = 0; w < RED_THREADS / 64; ++w) x += sh[k][w];
+      part[k * RED_BLOCKS + blockIdx.x] = x;
+    }
+  }
+}
+
+#define GRID_STRIDE(i, n) for (long long i = blockIdx.x * (long long)RED_THREADS + threadIdx.x; i < (n); i += (long long)RED_BLOCKS * RED_THREADS)
+
+template <typename R>
+__global__ void __launch_bounds__(RED_THREADS) k_init_inner(long long n, R* x, R* r, R* p, R* r0, const R* b, double* part) {
+  using T = typename V2<R>::T;
+  double acc[1] = {0};
+  GRID_STRIDE(i, n) {
+    T bv = reinterpret_cast<const T*>(b)[i];
+    T z; z.x = 0; z.y = 0;
+    reinterpret_cast<T*>(x)[i] = z;
+    reinterpret_cast<T*>(r)[i] = bv;
+    reinterpret_cast<T*>(p)[i] = bv;
+    reinterpret_cast<T*>(r0)[i] = bv;
+    acc[0] += (double)bv.x * bv.x + (double)bv.y * bv.y;
+  }
+  block_reduce_store<1>(acc, part);
+}
+
+template <typename R>
+__global__ void __launch_bounds__(RED_THREADS) k_init_outer(long long n, R* r, R* p, R* r0, const R* b, const R* q, double* part) {
+  using T = typename V2<R>::T;
+  double acc[2] = {0, 0};
+  GRID_STRIDE(i, n) {
+    T bv = reinterpret_cast<const T*>(b)[i];
+    T qv = reinterpret_cast<const T*>(q)[i];
+    T rv;
+    rv.x = bv.x - qv.x;
+    rv.y = bv.y - qv.y;
+    reinterpret_cast<T*>(r)[i] = rv;
+    reinterpret_cast<T*>(p)[i] = rv;
+    reinterpret_cast<T*>(r0)[i] = rv;
+    acc[0] += (double)bv.x * bv.x + (double)bv.y * bv.y;
+    acc[1] += (double)rv.x * rv.x + (double)rv.y * rv.y;
+  }
+  block_reduce_store<2>(acc, part);
+}
+
+// <a,b> = sum conj(a) b  (bicgstab_dd_mix.cc:180-181)
+template <typename R>
+__global__ void __launch_bounds__(RED_THREADS) k_dot(long long n, const R* a, const R* b, double* part) {
+  using T = typename V2<R>::T;
+  double acc[2] = {0, 0};
+  GRID_STRIDE(i, n) {
+    T av = reinterpret_cast<const T*>(a)[i];
+    T bv = reinterpret_cast<const T*>(b)[i];
+    acc[0] += (double)av.x * bv.x + (double)av.y * bv.y;
+    acc[1] += (double)av.x * bv.y - (double)av.y * bv.x;
+  }
+  block_reduce_store<2>(acc, part);
+}
+
+template <typename R>
+__global__ void __launch_bounds__(RED_THREADS) k_norm_dot(long long n, const R* t, const R* r, double* part) {
+  using T = typename V2<R>::T;
+  double acc[3] = {0, 0, 0};
+  GRID_STRIDE(i, n) {
+    T tv = reinterpret_cast<const T*>(t)[i];
+    T rv = reinterpret_cast<const T*>(r)[i];
+    acc[0] += (double)tv.x * tv.x + (double)tv.y * tv.y;
+    acc[1] += (double)tv.x * rv.x + (double)tv.y * rv.y;
+    acc[2] += (double)tv.x * rv.y - (double)tv.y * rv.x;
+  }
+  block_reduce_store<3>(acc, part);
+}
+
+template <typename R>
+__global__ void __launch_bounds__(RED_THREADS) k_norm(long long n, const R* a, double* part) {
+  using T = typename V2<R>::T;
+  double acc[1] = {0};
+  GRID_STRIDE(i, n) {
+    T av = reinterpret_cast<const T*>(a)[i];
+    acc[0] += (double)av.x * av.x + (double)av.y * av.y;
+  }
+  block_reduce_store<1>(acc, part);
+}
+
+// x += c v1 ; r -= c v2 ; |r|^2 [, <r0,r>]   (bicgstab_precdd_s.cc:281-295,427-440)
+template <typename R, bool WITH_R0>
+__global__ void __launch_bounds__(RED_THREADS)
+k_update_xr(long long n, R* x, const R* v1, R* r, const R* v2, const double* c, const R* r0, double* part) {
+  using T = typename V2<R>::T;
+  const R cr = (R)c[0], ci = (R)c[1];
+  double acc[WITH_R0 ? 3 : 1];
+#pragma unroll
+  for (int k = 0; k < (WITH_R0 ? 3 : 1); ++k) acc[k] = 0;
+  GRID_STRIDE(i, n) {
+    T xv = reinterpret_cast<T*>(x)[i];
+    T a = reinterpret_cast<const T*>(v1)[i];
+    T rv = reinterpret_cast<T*>(r)[i];
+    T q = reinterpret_cast<const T*>(v2)[i];
+    xv.x = xv.x + cr * a.x - ci * a.y;
+    xv.y = xv.y + cr * a.y + ci * a.x;
+    rv.x = rv.x - cr * q.x + ci * q.y;
+    rv.y = rv.y - cr * q.y - ci * q.x;
+    reinterpret_cast<T*>(x)[i] = xv;
+    reinterpret_cast<T*>(r)[i] = rv;
+    acc[0] += (double)rv.x * rv.x + (double)rv.y * rv.y;
+    if (WITH_R0) {
+      T z = reinterpret_cast<const T*>(r0)[i];
+      acc[1] += (double)z.x * rv.x + (double)z.y * rv.y;
+      acc[2] += (double)z.x * rv.y - (double)z.y * rv.x;
+    }
+  }
+  block_reduce_store<WITH_R0 ? 3 : 1>(acc, part);
+}
+
+// p = r + beta (p - omega q)   (bicgstab_precdd_s.cc:529-543)
+template <typename R>
+__global__ void __launch_bounds__(RED_THREADS) k_update_p(long long n, R* p, const R* r, const R* q, const Scalars* sc) {
+  using T = typename V2<R>::T;
+  const R ar = (R)sc->omega[0], ai = (R)sc->omega[1], br = (R)sc->beta[0], bi = (R)sc->beta[1];
+  GRID_STRIDE(i, n) {
+    T pv = reinterpret_cast<T*>(p)[i];
+    T qv = reinterpret_cast<const T*>(q)[i];
+    T rv = reinterpret_cast<const T*>(r)[i];
+    R cr = pv.x - ar * qv.x + ai * qv.y;
+    R ci = pv.y - ar * qv.y - ai * qv.x;
+    pv.x = br * cr - bi * ci + rv.x;
+    pv.y = br * ci + bi * cr + rv.y;
+    reinterpret_cast<T*>(p)[i] = pv;
+  }
+}
+
+__device__ __forceinline__ void cdiv(double ar, double ai, double br, double bi, double* o) {
+  // (a / b) with scaling (std::complex semantics up to rounding)
+  if (fabs(br) >= fabs(bi)) {
+    double t = bi / br, d = br + bi * t;
+    o[0] = (ar + ai * t) / d;
+    o[1] = (ai - ar * t) / d;
+  } else {
+    double t = br / bi, d = bi + br * t;
+    o[0] = (ar * t + ai) / d;
+    o[1] = (ai * t - ar) / d;
+  }
+}
+
+// Deterministic sum of the partials, then the scalar recurrences of BiCGStab
+// with the reference's breakdown guards (bicgstab_dd_mix.cc:192,279,359).
+__global__ void __launch_bounds__(RED_THREADS) k_finalize(const double* part, int nval, Scalars* sc, int mode) {
+  __shared__ double sh[4][RED_THREADS];
+  for (int v = 0; v < nval; ++v) {
+    double x = 0;
+    for (int i = threadIdx.x; i < RED_BLOCKS; i += RED_THREADS) x += part[v * RED_BLOCKS + i];
+    sh[v][threadIdx.x] = x;
+  }
+  __syncthreads();
+  for (int w = RED_THREADS / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w)
+      for (int v = 0; v < nval; ++v) sh[v][threadIdx.x] += sh[v][threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x != 0) return;
+  double red[4] = {0, 0, 0, 0};
+  for (int v = 0; v < nval; ++v) red[v] = sh[v][0];
+  for (int v = 0; v < 4; ++v) sc->red[v] = red[v];
+  switch (mode) {
+    case FIN_INIT_INNER:
+      sc->bnorm = red[0];
+      sc->rnorm = red[0];
+      sc->rho0[0] = red[0];
+      sc->rho0[1] = 0;
+      break;
+    case FIN_INIT_OUTER:
+      sc->bnorm = red[0];
+      sc->rnorm = red[1];
+      sc->rho0[0] = red[1];
+      sc->rho0[1] = 0;
+      break;
+    case FIN_ALPHA:
+      if (red[0] == 0 && red[1] == 0) { sc->alpha[0] = 0; sc->alpha[1] = 0; }
+      else cdiv(sc->rho0[0], sc->rho0[1], red[0], red[1], sc->alpha);
+      break;
+    case FIN_RNORM:
+      sc->rnorm = red[0];
+      break;
+    case FIN_OMEGA:
+      sc->tt = red[0];
+      if (red[0] == 0) { sc->omega[0] = 0; sc->omega[1] = 0; }
+      else { sc->omega[0] = red[1] / red[0]; sc->omega[1] = red[2] / red[0]; }
+      break;
+    case FIN_RHO: {
+      sc->rnorm = red[0];
+      sc->rho[0] = red[1];
+      sc->rho[1] = red[2];
+      // beta = alpha * rho / (rho0 * omega)
+      double nr = sc->alpha[0] * red[1] - sc->alpha[1] * red[2];
+      double ni = sc->alpha[0] * red[2] + sc->alpha[1] * red[1];
+      double dr = sc->rho0[0] * sc->omega[0] - sc->rho0[1] * sc->omega[1];
+      double di = sc->rho0[0] * sc->omega[1] + sc->rho0[1] * sc->omega[0];
+      double bt[2] = {0, 0};
+      if (dr != 0 || di != 0) cdiv(nr, ni, dr, di, bt);
+      sc->rho0[0] = red[1];
+      sc->rho0[1] = red[2];
+      bool om0 = (sc->omega[0] == 0 && sc->omega[1] == 0);
+      bool rh0 = (red[1] == 0 && red[2] == 0);
+      if (om0 || rh0) { bt[0] = 0; bt[1] = 0; }
+      sc->beta[0] = bt[0];
+      sc->beta[1] = bt[1];
+      break;
+    }
+    case FIN_NRM:
+      sc->nrm = sqrt(red[0]);
+      break;
+    default:
+      break;
+  }
+}
+
+template <typename S>
+__global__ void __launch_bounds__(RED_THREADS) k_d2s_scaled(long long V, S* out, const double* in, const Scalars* sc) {
+  using R = typename Store<S>::R;
+  const double inv = 1.0 / sc->nrm;
+  GRID_STRIDE(s, V) {
+    Cx<double> v[12];
+    Cx<R> w[12];
+    SpinorIO<double>::load(in, s, V, v);
+#pragma unroll
+    for (int k = 0; k < 12; ++k) w[k] = {(R)(v[k].re * inv), (R)(v[k].im * inv)};
+    SpinorIO<S>::store(out, s, V, w);
+  }
+}
+
+template <typename S>
+__global__ void __launch_bounds__(RED_THREADS) k_s2d_scaled(long long V, double* out, const S* in, const Scalars* sc) {
+  using R = typename Store<S>::R;
+  const double nrm = sc->nrm;
+  GRID_STRIDE(s, V) {
+    Cx<R> w[12];
+    Cx<double> v[12];
+    SpinorIO<S>::load(in, s, V, w);
+#pragma unroll
+    for (int k = 0; k < 12; ++k) v[k] = {(double)w[k].re * nrm, (double)w[k].im * nrm};
+    SpinorIO<double>::store(out, s, V, v);
+  }
+}
+
+template <typename SO, typename SI>
+__global__ void __launch_bounds__(RED_THREADS) k_convert(long long V, SO* out, const SI* in) {
+  using RI = typename Store<SI>::R;
+  using RO = typename Store<SO>::R;
+  GRID_STRIDE(s, V) {
+    Cx<RI> w[12];
+    Cx<RO> v[12];
+    SpinorIO<SI>::load(in, s, V, w);
+#pragma unroll
+    for (int k = 0; k < 12; ++k) v[k] = {(RO)w[k].re, (RO)w[k].im};
+    SpinorIO<SO>::store(out, s, V, v);
+  }
+}
+
+template <typename R>
+__global__ void __launch_bounds__(RED_THREADS) k_sub(long long n, R* out, const R* a, const R* b) {
+  GRID_STRIDE(i, n) out[i] = a[i] - b[i];
+}
+
+// ------------------------------------------------------------------ launchers
+#define LAUNCH(kern, ...) hipLaunchKernelGGL(kern, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, st, __VA_ARGS__)
+
+template <typename R> void launch_bicg_init_inner(long long n, R* x, R* r, R* p, R* r0, const R* b, double* part, hipStream_t st) {
+  LAUNCH((k_init_inner<R>), n, x, r, p, r0, b, part);
+}
+template <typename R> void launch_bicg_init_outer(long long n, R* r, R* p, R* r0, const R* b, const R* q, double* part, hipStream_t st) {
+  LAUNCH((k_init_outer<R>), n, r, p, r0, b, q, part);
+}
+template <typename R> void launch_dot(long long n, const R* a, const R* b, double* part, hipStream_t st) {
+  LAUNCH((k_dot<R>), n, a, b, part);
+}
+template <typename R> void launch_norm_dot(long long n, const R* t, const R* r, double* part, hipStream_t st) {
+  LAUNCH((k_norm_dot<R>), n, t, r, part);
+}
+template <typename R>
+void launch_update_xr(long long n, R* x, const R* v1, R* r, const R* v2, const double* c, const R* r0, double* part,
+                      hipStream_t st) {
+  if (r0) LAUNCH((k_update_xr<R, true>), n, x, v1, r, v2, c, r0, part);
+  else LAUNCH((k_update_xr<R, false>), n, x, v1, r, v2, c, r0, part);
+}
+template <typename R> void launch_update_p(long long n, R* p, const R* r, const R* q, const Scalars* sc, hipStream_t st) {
+  LAUNCH((k_update_p<R>), n, p, r, q, sc);
+}
+template <typename R> void launch_norm(long long n, const R* a, double* part, hipStream_t st) {
+  LAUNCH((k_norm<R>), n, a, part);
+}
+void launch_finalize(const double* part, int nval, Scalars* sc, int mode, hipStream_t st) {
+  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(RED_THREADS), 0, st, part, nval, sc, mode);
+}
+template <typename S> void launch_d2s_scaled(long long V, S* out, const double* in, const Scalars* sc, hipStream_t st) {
+  LAUNCH((k_d2s_scaled<S>), V, out, in, sc);
+}
+template <typename S> void launch_s2d_scaled(long long V, double* out, const S* in, const Scalars* sc, hipStream_t st) {
+  LAUNCH((k_s2d_scaled<S>), V, out, in, sc);
+}
+template <typename SO, typename SI> void launch_convert(long long V, SO* out, const SI* in, hipStream_t st) {
+  LAUNCH((k_convert<SO, SI>), V, out, in);
+}
+template void launch_d2s_scaled<float>(long long, float*, const double*, const Scalars*, hipStream_t);
+template void launch_d2s_scaled<__half>(long long, __half*, const double*, const Scalars*, hipStream_t);
+template void launch_s2d_scaled<float>(long long, double*, const float*, const Scalars*, hipStream_t);
+template void launch_s2d_scaled<__half>(long long, double*, const __half*, const Scalars*, hipStream_t);
+template void launch_convert<float, double>(long long, float*, const double*, hipStream_t);
+template void launch_convert<double, float>(long long, double*, const float*, hipStream_t);
+template void launch_convert<__half, float>(long long, __half*, const float*, hipStream_t);
+template void launch_convert<float, __half>(long long, float*, const __half*, hipStream_t);
+template <typename R> void launch_sub(long long n, R* out, const R* a, const R* b, hipStream_t st) {
+  LAUNCH((k_sub<R>), n, out, a, b);
+}
+
+#define INST(R)                                                                                                     \
+  template void launch_bicg_init_inner<R>(long long, R*, R*, R*, R*, const R*, double*, hipStream_t);              \
+  template void launch_bicg_init_outer<R>(long long, R*, R*, R*, const R*, const R*, double*, hipStream_t);        \
+  template void launch_dot<R>(long long, const R*, const R*, double*, hipStream_t);                                \
+  template void launch_norm_dot<R>(long long, const R*, const R*, double*, hipStream_t);                           \
+  template void launch_update_xr<R>(long long, R*, const R*, R*, const R*, const double*, const R*, double*,       \
+                                    hipStream_t);                                                                   \
+  template void launch_update_p<R>(long long, R*, const R*, const R*, const Scalars*, hipStream_t);                \
+  template void launch_norm<R>(long long, const R*, double*, hipStream_t);                                         \
+  template void launch_sub<R>(long long, R*, const R*, const R*, hipStream_t);
+INST(float)
+INST(double)
+
+}  // namespace qws

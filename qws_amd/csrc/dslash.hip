@@ -1,0 +1,116 @@
+// dslash.hip -- hopping-term kernels: D, block-local D_in, face terms and the
+// fused SAP updates, all as  out = a + b - c + gamma * T^-1 H_set(in).
+//
+// Reference semantics: ddd_in_s.cc:72-374 / ddd_in_d.cc:66-196 (hops inside a
+// domain, then T^-1, then out = in - kappa * (...)), ddd_out_s.cc:112-626 and
+// ddd_out_d.cc:84-235 (face hops of a domain, T^-1, out += factor * (...)),
+// qws.cc:625-644 (full operator).  On one GPU the faces are read in place from
+// the neighbouring block instead of being packed and exchanged.
+#include "kernels.h"
+
+namespace qws {
+
+constexpr int HOP_THREADS = 256;
+
+template <typename S, int NL, int SET>
+__global__ void __launch_bounds__(HOP_THREADS)
+k_hop(Geo g, const int* __restrict__ list, int wg_per_blk, typename Store<S>::R gamma, S* out, const S* a,
+      const S* b, const S* c, const S* __restrict__ in, const S* __restrict__ U, const S* __restrict__ C,
+      int skip_interior) {
+  using R = typename Store<S>::R;
+  const int bi = blockIdx.x / wg_per_blk;
+  const int blk = list ? list[bi] : bi;
+  const int loc = (blockIdx.x - bi * wg_per_blk) * HOP_THREADS + threadIdx.x;
+  if (loc >= g.Vb) return;
+  const long long s = (long long)blk * g.Vb + loc;
+  SiteC sc;
+  site_coords(g, s, sc);
+  if (SET == HOPS_FACE && skip_interior) {
+    bool face = false;
+#pragma unroll
+    for (int m = 0; m < 4; ++m) face |= (sc.l[m] == 0) | (sc.l[m] == g.B[m] - 1);
+    if (!face) return;
+  }
+  Cx<R> acc[12];
+  hop_sum<S, NL, SET>(g, s, sc, in, U, acc);
+  clover_apply<S>(C, s, g.V, acc);
+  Cx<R> v[12];
+  if (a) {
+    SpinorIO<S>::load(a, s, g.V, v);
+  } else {
+#pragma unroll
+    for (int k = 0; k < 12; ++k) v[k] = {0, 0};
+  }
+  if (b) {
+    Cx<R> w[12];
+    SpinorIO<S>::load(b, s, g.V, w);
+#pragma unroll
+    for (int k = 0; k < 12; ++k) v[k] = cadd(v[k], w[k]);
+  }
+  if (c) {
+    Cx<R> w[12];
+    SpinorIO<S>::load(c, s, g.V, w);
+#pragma unroll
+    for (int k = 0; k < 12; ++k) v[k] = csub(v[k], w[k]);
+  }
+#pragma unroll
+  for (int k = 0; k < 12; ++k) v[k] = cadd(v[k], cscale(gamma, acc[k]));
+  SpinorIO<S>::store(out, s, g.V, v);
+}
+
+template <typename S>
+__global__ void __launch_bounds__(HOP_THREADS) k_zero_blocks(Geo g, const int* list, int wg_per_blk, S* out) {
+  using R = typename Store<S>::R;
+  const int bi = blockIdx.x / wg_per_blk;
+  const int blk = list ? list[bi] : bi;
+  const int loc = (blockIdx.x - bi * wg_per_blk) * HOP_THREADS + threadIdx.x;
+  if (loc >= g.Vb) return;
+  const long long s = (long long)blk * g.Vb + loc;
+  Cx<R> v[12];
+#pragma unroll
+  for (int k = 0; k < 12; ++k) v[k] = {0, 0};
+  SpinorIO<S>::store(out, s, g.V, v);
+}
+
+template <typename S, int NL>
+void launch_hop(const Geo& g, BlockSel sel, int set, typename Store<S>::R gamma, S* out, const S* a, const S* b,
+                const S* c, const S* in, const S* U, const S* C, hipStream_t st) {
+  const int wpb = (g.Vb + HOP_THREADS - 1) / HOP_THREADS;
+  dim3 grid(sel.n * wpb), block(HOP_THREADS);
+  const int skip = (set == HOPS_FACE && out == a && b == nullptr && c == nullptr) ? 1 : 0;
+  switch (set) {
+    case HOPS_ALL:
+      hipLaunchKernelGGL((k_hop<S, NL, HOPS_ALL>), grid, block, 0, st, g, sel.list, wpb, gamma, out, a, b, c, in, U, C, skip);
+      break;
+    case HOPS_INSIDE:
+      hipLaunchKernelGGL((k_hop<S, NL, HOPS_INSIDE>), grid, block, 0, st, g, sel.list, wpb, gamma, out, a, b, c, in, U, C, skip);
+      break;
+    case HOPS_FACE:
+      hipLaunchKernelGGL((k_hop<S, NL, HOPS_FACE>), grid, block, 0, st, g, sel.list, wpb, gamma, out, a, b, c, in, U, C, skip);
+      break;
+    default:
+      hipLaunchKernelGGL((k_hop<S, NL, HOPS_NONE>), grid, block, 0, st, g, sel.list, wpb, gamma, out, a, b, c, in, U, C, 0);
+  }
+}
+
+template <typename S>
+void launch_zero_blocks(const Geo& g, BlockSel sel, S* out, hipStream_t st) {
+  const int wpb = (g.Vb + HOP_THREADS - 1) / HOP_THREADS;
+  hipLaunchKernelGGL((k_zero_blocks<S>), dim3(sel.n * wpb), dim3(HOP_THREADS), 0, st, g, sel.list, wpb, out);
+}
+
+template void launch_hop<float, 18>(const Geo&, BlockSel, int, float, float*, const float*, const float*, const float*,
+                                    const float*, const float*, const float*, hipStream_t);
+template void launch_hop<float, 12>(const Geo&, BlockSel, int, float, float*, const float*, const float*, const float*,
+                                    const float*, const float*, const float*, hipStream_t);
+template void launch_hop<double, 18>(const Geo&, BlockSel, int, double, double*, const double*, const double*,
+                                     const double*, const double*, const double*, const double*, hipStream_t);
+template void launch_hop<__half, 12>(const Geo&, BlockSel, int, float, __half*, const __half*, const __half*,
+                                     const __half*, const __half*, const __half*, const __half*, hipStream_t);
+template void launch_hop<__half, 18>(const Geo&, BlockSel, int, float, __half*, const __half*, const __half*,
+                                     const __half*, const __half*, const __half*, const __half*, hipStream_t);
+template void launch_zero_blocks<float>(const Geo&, BlockSel, float*, hipStream_t);
+template void launch_zero_blocks<double>(const Geo&, BlockSel, double*, hipStream_t);
+template void launch_zero_blocks<__half>(const Geo&, BlockSel, __half*, hipStream_t);
+
+}  // namespace qws

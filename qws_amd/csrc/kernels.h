@@ -1,0 +1,80 @@
+// kernels.h -- host-side declarations of the HIP kernel launchers.
+#pragma once
+
+#include <hip/hip_fp16.h>
+#include <hip/hip_runtime.h>
+
+#include "qws_device.h"
+
+namespace qws {
+
+// Blocks to run over: all blocks (list == nullptr) or the blocks of one colour.
+struct BlockSel {
+  const int* list;  // device array of block ids, or nullptr for every block
+  int n;            // number of blocks
+};
+
+// out = a + b - c + gamma * T^-1 H_set(in)  on the selected blocks.
+// Any of a, b, c may be nullptr (term absent).  out may alias a, b or c, never in.
+// skip_interior: for set == HOPS_FACE with out == a, sites without a face hop
+// are left untouched.
+template <typename S, int NL>
+void launch_hop(const Geo& g, BlockSel sel, int set, typename Store<S>::R gamma, S* out, const S* a,
+                const S* b, const S* c, const S* in, const S* U, const S* C, hipStream_t st);
+
+// out = 0 on the selected blocks
+template <typename S>
+void launch_zero_blocks(const Geo& g, BlockSel sel, S* out, hipStream_t st);
+
+// ---- BiCGStab vector kernels (full lattice, n = number of complex numbers)
+struct Scalars {
+  double bnorm, rnorm, nrm, tt;
+  double rho0[2], rho[2], alpha[2], omega[2], beta[2];
+  double red[4];
+};
+
+constexpr int RED_BLOCKS = 1024;  // partial sums per reduction (fixed => deterministic)
+constexpr int RED_THREADS = 256;
+
+enum Finalize {
+  FIN_INIT_INNER = 0,  // bnorm = rho0 = red0
+  FIN_INIT_OUTER,      // bnorm = red0, rho0 = red1
+  FIN_ALPHA,           // alpha = rho0 / (red0 + i red1)
+  FIN_RNORM,           // rnorm = red0
+  FIN_OMEGA,           // omega = (red1 + i red2) / red0
+  FIN_RHO,             // rnorm = red0, rho = red1 + i red2, beta, rho0 = rho
+  FIN_NRM,             // nrm = sqrt(red0)
+  FIN_RAW              // red[] only
+};
+
+template <typename R> void launch_bicg_init_inner(long long n, R* x, R* r, R* p, R* r0, const R* b, double* part, hipStream_t st);
+template <typename R> void launch_bicg_init_outer(long long n, R* r, R* p, R* r0, const R* b, const R* q, double* part, hipStream_t st);
+template <typename R> void launch_dot(long long n, const R* a, const R* b, double* part, hipStream_t st);      // <a,b>
+template <typename R> void launch_norm_dot(long long n, const R* t, const R* r, double* part, hipStream_t st);  // |t|^2, <t,r>
+// x += c v1 ; r -= c v2 ; red |r|^2 (and <r0,r> if r0)
+template <typename R>
+void launch_update_xr(long long n, R* x, const R* v1, R* r, const R* v2, const double* c, const R* r0, double* part,
+                      hipStream_t st);
+// p = r + beta (p - omega q)
+template <typename R> void launch_update_p(long long n, R* p, const R* r, const R* q, const Scalars* sc, hipStream_t st);
+template <typename R> void launch_norm(long long n, const R* a, double* part, hipStream_t st);
+void launch_finalize(const double* part, int nval, Scalars* sc, int mode, hipStream_t st);
+// Per-site precision conversion of spinors (V sites; the storage layouts of the
+// precisions differ): out = in / nrm (to lower precision), out = in * nrm (back).
+template <typename S> void launch_d2s_scaled(long long V, S* out, const double* in, const Scalars* sc, hipStream_t st);
+template <typename S> void launch_s2d_scaled(long long V, double* out, const S* in, const Scalars* sc, hipStream_t st);
+template <typename SO, typename SI> void launch_convert(long long V, SO* out, const SI* in, hipStream_t st);
+// out = a - b (real arrays of length n)
+template <typename R> void launch_sub(long long n, R* out, const R* a, const R* b, hipStream_t st);
+
+// ---- layout conversion (device site order <-> host layouts); host data staged in device memory
+// kind: 0 = spinor (24 reals), 1 = links (4 x 18 reals), 2 = clover (72 reals)
+// layout: 0 = plain lexicographic, 1 = QWS AoSoA with lane width vlen (single rank, 2 x-domains)
+template <typename S>
+void launch_import(const Geo& g, int kind, int layout, int vlen, int nl, const void* src, int src_is_double, S* dst,
+                   hipStream_t st);
+template <typename S>
+void launch_export(const Geo& g, int kind, int layout, int vlen, const S* src, void* dst, int dst_is_double,
+                   hipStream_t st);
+
+}  // namespace qws

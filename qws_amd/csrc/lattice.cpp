@@ -1,0 +1,478 @@
+// lattice.cpp -- host side of the device solver: field loading, the SAP step
+// sequences, the FP32 inner and FP64 outer BiCGStab, true residual.
+#include "lattice.h"
+
+#include <chrono>
+#include <cmath>
+#include <cstddef>
+#include <cstring>
+
+namespace qws {
+
+namespace {
+template <typename T> T* dalloc(size_t n) {
+  void* p = nullptr;
+  QWS_HIP(hipMalloc(&p, n * sizeof(T)));
+  return (T*)p;
+}
+}  // namespace
+
+Lattice::Lattice(const int L[4], const int B[4], int tbc, int links32) {
+  if (links32 != 18 && links32 != 12) throw std::invalid_argument("links32 must be 18 or 12");
+  if (tbc != 1 && tbc != -1) throw std::invalid_argument("tbc must be +1 or -1");
+  long long V = 1;
+  int Vb = 1, nblk = 1;
+  improper_ = false;
+  for (int m = 0; m < 4; ++m) {
+    if (L[m] <= 0 || B[m] <= 0 || L[m] % B[m] != 0)
+      throw std::invalid_argument("block extents must divide the lattice extents");
+    g_.L[m] = L[m];
+    g_.B[m] = B[m];
+    g_.NB[m] = L[m] / B[m];
+    V *= L[m];
+    Vb *= B[m];
+    nblk *= g_.NB[m];
+    if (m == 0 && g_.NB[m] % 2 != 0) throw std::invalid_argument("the x direction needs an even number of blocks");
+    if (m > 0 && g_.NB[m] != 1 && g_.NB[m] % 2 != 0)
+      throw std::invalid_argument("y/z/t need an even number of blocks or a single block");
+    if (m > 0 && g_.NB[m] == 1) improper_ = true;  // QWS's npe[mu]==1 branch (qws.cc:805)
+  }
+  g_.sB[0] = 1;
+  g_.sN[0] = 1;
+  for (int m = 1; m < 4; ++m) {
+    g_.sB[m] = g_.sB[m - 1] * g_.B[m - 1];
+    g_.sN[m] = g_.sN[m - 1] * g_.NB[m - 1];
+  }
+  g_.Vb = Vb;
+  g_.nblk = nblk;
+  g_.V = V;
+  g_.tbc = tbc;
+  nl32_ = links32;
+  QWS_HIP(hipStreamCreate(&st_));
+  // block lists per colour (colour = (bx+by+bz+bt)%2, qws.cc:275-276)
+  std::vector<int> lists[2];
+  for (int b = 0; b < nblk; ++b) {
+    int r = b, sum = 0;
+    for (int m = 0; m < 4; ++m) {
+      sum += r % g_.NB[m];
+      r /= g_.NB[m];
+    }
+    lists[sum & 1].push_back(b);
+  }
+  for (int c = 0; c < 2; ++c) {
+    nblk_colour_[c] = (int)lists[c].size();
+    d_blocks_[c] = dalloc<int>(lists[c].size() + 1);
+    QWS_HIP(hipMemcpy(d_blocks_[c], lists[c].data(), lists[c].size() * sizeof(int), hipMemcpyHostToDevice));
+  }
+  part_ = dalloc<double>(4 * RED_BLOCKS);
+  sc_in_ = dalloc<Scalars>(1);
+  sc_out_ = dalloc<Scalars>(1);
+  QWS_HIP(hipMemset(sc_in_, 0, sizeof(Scalars)));
+  QWS_HIP(hipMemset(sc_out_, 0, sizeof(Scalars)));
+  QWS_HIP(hipHostMalloc((void**)&sc_host_, sizeof(Scalars), hipHostMallocDefault));
+}
+
+Lattice::~Lattice() {
+  hipStreamSynchronize(st_);
+  for (auto* p : {(void*)Ud_, (void*)Cd_, (void*)Us_, (void*)Cs_, (void*)Uh_, (void*)Ch_, (void*)part_, (void*)sc_in_,
+                  (void*)sc_out_, (void*)d_blocks_[0], (void*)d_blocks_[1]})
+    if (p) hipFree(p);
+  for (auto& w : work_) hipFree(w.second.first);
+  if (sc_host_) hipHostFree(sc_host_);
+  if (st_) hipStreamDestroy(st_);
+}
+
+void* Lattice::work(const std::string& name, size_t bytes) {
+  for (auto& w : work_)
+    if (w.first == name) {
+      if (w.second.second >= bytes) return w.second.first;
+      QWS_HIP(hipStreamSynchronize(st_));
+      QWS_HIP(hipFree(w.second.first));
+      QWS_HIP(hipMalloc(&w.second.first, bytes));
+      w.second.second = bytes;
+      return w.second.first;
+    }
+  void* p = nullptr;
+  QWS_HIP(hipMalloc(&p, bytes));
+  work_.push_back({name, {p, bytes}});
+  return p;
+}
+
+size_t Lattice::spinor_bytes(int prec) const {
+  const size_t es = prec == P64 ? 8 : (prec == P32 ? 4 : 2);
+  return (size_t)g_.V * 24 * es;
+}
+
+void* Lattice::alloc_spinor(int prec) {
+  void* p = nullptr;
+  QWS_HIP(hipMalloc(&p, spinor_bytes(prec)));
+  QWS_HIP(hipMemsetAsync(p, 0, spinor_bytes(prec), st_));
+  return p;
+}
+
+// ------------------------------------------------------------------ field loading
+void Lattice::load_gauge(const void* U, int layout, int is_double, int target) {
+  const size_t es = is_double ? 8 : 4;
+  const size_t bytes = (size_t)4 * g_.V * 18 * es;
+  void* stage = work("stage", bytes);
+  QWS_HIP(hipMemcpyAsync(stage, U, bytes, hipMemcpyHostToDevice, st_));
+  const int vlen = layout == 1 ? (is_double ? 8 : 16) : 1;
+  if (target & 1) {
+    if (!Ud_) Ud_ = dalloc<double>((size_t)4 * 18 * g_.V);
+    launch_import<double>(g_, 1, layout, vlen, 18, stage, is_double, Ud_, st_);
+  }
+  if (target & 2) {
+    if (!Us_) Us_ = dalloc<float>((size_t)4 * nl32_ * g_.V);
+    launch_import<float>(g_, 1, layout, vlen, nl32_, stage, is_double, Us_, st_);
+    half_valid_ = false;
+  }
+  QWS_HIP(hipStreamSynchronize(st_));
+}
+
+void Lattice::load_clover(const void* C, int layout, int is_double, int target) {
+  const size_t es = is_double ? 8 : 4;
+  const size_t bytes = (size_t)g_.V * 72 * es;
+  void* stage = work("stage", bytes);
+  QWS_HIP(hipMemcpyAsync(stage, C, bytes, hipMemcpyHostToDevice, st_));
+  const int vlen = layout == 1 ? (is_double ? 8 : 16) : 1;
+  if (target & 1) {
+    if (!Cd_) Cd_ = dalloc<double>((size_t)72 * g_.V);
+    launch_import<double>(g_, 2, layout, vlen, 0, stage, is_double, Cd_, st_);
+  }
+  if (target & 2) {
+    if (!Cs_) Cs_ = dalloc<float>((size_t)72 * g_.V);
+    launch_import<float>(g_, 2, layout, vlen, 0, stage, is_double, Cs_, st_);
+    half_valid_ = false;
+  }
+  QWS_HIP(hipStreamSynchronize(st_));
+}
+
+void Lattice::ensure_half_fields() {
+  if (half_valid_) return;
+  throw std::runtime_error("FP16 field storage is not enabled in this build");
+}
+
+void Lattice::upload_spinor(int prec, void* dev, const void* host, int layout, int host_is_double) {
+  const size_t bytes = (size_t)g_.V * 24 * (host_is_double ? 8 : 4);
+  void* stage = work("stage_spinor", bytes);
+  QWS_HIP(hipMemcpyAsync(stage, host, bytes, hipMemcpyHostToDevice, st_));
+  const int vlen = layout == 1 ? (host_is_double ? 8 : 16) : 1;
+  if (prec == P64) launch_import<double>(g_, 0, layout, vlen, 0, stage, host_is_double, (double*)dev, st_);
+  else if (prec == P32) launch_import<float>(g_, 0, layout, vlen, 0, stage, host_is_double, (float*)dev, st_);
+  else launch_import<__half>(g_, 0, layout, vlen, 0, stage, host_is_double, (__half*)dev, st_);
+  QWS_HIP(hipStreamSynchronize(st_));
+}
+
+void Lattice::download_spinor(int prec, const void* dev, void* host, int layout, int host_is_double) {
+  const size_t bytes = (size_t)g_.V * 24 * (host_is_double ? 8 : 4);
+  void* stage = work("stage_spinor", bytes);
+  const int vlen = layout == 1 ? (host_is_double ? 8 : 16) : 1;
+  if (prec == P64) launch_export<double>(g_, 0, layout, vlen, (const double*)dev, stage, host_is_double, st_);
+  else if (prec == P32) launch_export<float>(g_, 0, layout, vlen, (const float*)dev, stage, host_is_double, st_);
+  else launch_export<__half>(g_, 0, layout, vlen, (const __half*)dev, stage, host_is_double, st_);
+  QWS_HIP(hipMemcpyAsync(host, stage, bytes, hipMemcpyDeviceToHost, st_));
+  QWS_HIP(hipStreamSynchronize(st_));
+}
+
+// ------------------------------------------------------------------ operators
+template <> Lattice::FieldSet<double> Lattice::fields<double>() const {
+  if (!Ud_ || !Cd_) throw std::runtime_error("FP64 gauge/clover fields are not loaded");
+  return {Ud_, Cd_, 18};
+}
+template <> Lattice::FieldSet<float> Lattice::fields<float>() const {
+  if (!Us_ || !Cs_) throw std::runtime_error("FP32 gauge/clover fields are not loaded");
+  return {Us_, Cs_, nl32_};
+}
+template <> Lattice::FieldSet<__half> Lattice::fields<__half>() const {
+  if (!Uh_ || !Ch_ || !half_valid_) throw std::runtime_error("FP16 gauge/clover fields are not available");
+  return {Uh_, Ch_, nl32_};
+}
+
+BlockSel Lattice::sel(int colour) const {
+  if (colour < 0) return {nullptr, g_.nblk};
+  return {d_blocks_[colour & 1], nblk_colour_[colour & 1]};
+}
+
+template <typename S>
+void Lattice::hop(BlockSel s, int set, typename Store<S>::R gamma, S* out, const S* a, const S* b, const S* c,
+                  const S* in) {
+  if (s.n == 0) return;
+  FieldSet<S> f = fields<S>();
+  if (f.nl == 18) launch_hop<S, 18>(g_, s, set, gamma, out, a, b, c, in, f.U, f.C, st_);
+  else if constexpr (!std::is_same<S, double>::value) launch_hop<S, 12>(g_, s, set, gamma, out, a, b, c, in, f.U, f.C, st_);
+}
+
+#define DISPATCH_PREC(prec, CALL)                                  \
+  do {                                                             \
+    if ((prec) == P64) { using S = double; CALL; }                 \
+    else if ((prec) == P32) { using S = float; CALL; }             \
+    else { ensure_half_fields(); using S = __half; CALL; }         \
+  } while (0)
+
+void Lattice::dslash(int prec, void* out, const void* in, int hops, int colour) {
+  DISPATCH_PREC(prec, (hop<S>(sel(colour), hops, (typename Store<S>::R)(-kappa_), (S*)out, (const S*)in, nullptr,
+                              nullptr, (const S*)in)));
+}
+
+void Lattice::face(int prec, void* out, const void* in, double factor, int colour) {
+  DISPATCH_PREC(prec, (hop<S>(sel(colour), HOPS_FACE, (typename Store<S>::R)factor, (S*)out, (const S*)out, nullptr,
+                              nullptr, (const S*)in)));
+}
+
+void Lattice::clover(int prec, void* out, const void* in) {
+  DISPATCH_PREC(prec, (hop<S>(sel(-1), HOPS_NONE, (typename Store<S>::R)1, (S*)out, nullptr, nullptr, nullptr,
+                              (const S*)in)));
+}
+
+// Jacobi block solve (static_solver.cc:65-140): x_1 = 2b - D_in b, then
+// x_{k+1} = x_k + b - D_in x_k, i.e. x_{k+1} = b + kappa T^-1 H_in x_k with x_0 = b.
+// Iterates alternate between x and tmp so that the last one lands in x.
+template <typename S>
+void Lattice::jinv_t(int colour, S* x, const S* b, S* tmp, int nm) {
+  using R = typename Store<S>::R;
+  const S* prev = b;
+  for (int k = 1; k <= nm; ++k) {
+    S* dst = ((nm - k) % 2 == 0) ? x : tmp;
+    hop<S>(sel(colour), HOPS_INSIDE, (R)kappa_, dst, b, nullptr, nullptr, prev);
+    prev = dst;
+  }
+}
+
+void Lattice::jinv(int prec, void* x, const void* b, int colour, int nm) {
+  DISPATCH_PREC(prec, ({
+    S* tmp = (S*)work("jinv_tmp", spinor_bytes(prec));
+    jinv_t<S>(colour, (S*)x, (const S*)b, tmp, nm);
+  }));
+}
+
+// SAP sweeps (qws.cc:809-841): for each colour c (0 = domain_e, 1 = domain_o):
+//   q_c = A_c s_c ;  s_c += in_c - D_in q_c ;  s_o += kappa T^-1 H_face q
+template <typename S>
+void Lattice::sap_sweeps(const S* in, S* s, S* q, S* tmp, int nsap, int nm) {
+  using R = typename Store<S>::R;
+  const R k = (R)kappa_;
+  QWS_HIP(hipMemcpyAsync(s, in, spinor_bytes(sizeof(S) == 8 ? P64 : sizeof(S) == 4 ? P32 : P16),
+                         hipMemcpyDeviceToDevice, st_));
+  if (improper_) launch_zero_blocks<S>(g_, sel(1), q, st_);
+  for (int isap = 0; isap < nsap; ++isap)
+    for (int c = 0; c < 2; ++c) {
+      jinv_t<S>(c, q, s, tmp, nm);
+      hop<S>(sel(c), HOPS_INSIDE, k, s, s, in, q, q);  // s_c = s_c + in_c - q_c + kappa T^-1 H_in q_c
+      hop<S>(sel(1 - c), HOPS_FACE, k, s, s, nullptr, nullptr, q);
+    }
+}
+
+// prec_ddd_s_ (qws.cc:776-926): out = D M_SAP in, with the final sequence of the
+// improper (npe[mu]==1) and of the proper colouring.
+template <typename S>
+void Lattice::prec_ddd_t(S* out, const S* in, int nsap, int nm) {
+  using R = typename Store<S>::R;
+  const int pr = sizeof(S) == 8 ? P64 : sizeof(S) == 4 ? P32 : P16;
+  S* s = (S*)work("sap_s", spinor_bytes(pr));
+  S* q = (S*)work("sap_q", spinor_bytes(pr));
+  S* tmp = (S*)work("sap_tmp", spinor_bytes(pr));
+  const R k = (R)kappa_;
+  sap_sweeps<S>(in, s, q, tmp, nsap, nm);
+  if (improper_) launch_zero_blocks<S>(g_, sel(1), q, st_);
+  jinv_t<S>(0, q, s, tmp, nm);
+  hop<S>(sel(0), HOPS_INSIDE, -k, out, q, nullptr, nullptr, q);  // out_e = D_in q_e
+  if (improper_) {
+    hop<S>(sel(1), HOPS_FACE, k, s, s, nullptr, nullptr, q);      // s_o -= D_face q (q_o = 0)
+    jinv_t<S>(1, q, s, tmp, nm);
+    hop<S>(sel(1), HOPS_INSIDE, -k, out, q, nullptr, nullptr, q);  // out_o = D_in q_o
+    hop<S>(sel(0), HOPS_FACE, -k, out, out, nullptr, nullptr, q);  // out_e += D_face q
+    hop<S>(sel(1), HOPS_FACE, -k, out, out, nullptr, nullptr, q);  // out_o += D_face q
+  } else {
+    hop<S>(sel(1), HOPS_FACE, -k, out, nullptr, nullptr, nullptr, q);  // out_o = D_oe q_e
+    hop<S>(sel(1), HOPS_FACE, k, s, s, nullptr, nullptr, q);           // s_o -= out_o
+    jinv_t<S>(1, q, s, tmp, nm);
+    hop<S>(sel(1), HOPS_INSIDE, -k, out, out, q, nullptr, q);          // out_o += D_in q_o
+    hop<S>(sel(0), HOPS_FACE, -k, out, out, nullptr, nullptr, q);      // out_e += D_eo q_o
+  }
+}
+
+// prec_s_ (qws.cc:929-975): out = M_SAP in
+template <typename S>
+void Lattice::prec_s_t(S* out, const S* in, int nsap, int nm) {
+  using R = typename Store<S>::R;
+  const int pr = sizeof(S) == 8 ? P64 : sizeof(S) == 4 ? P32 : P16;
+  S* s = (S*)work("sap_s", spinor_bytes(pr));
+  S* q = (S*)work("sap_q", spinor_bytes(pr));
+  S* tmp = (S*)work("sap_tmp", spinor_bytes(pr));
+  sap_sweeps<S>(in, s, q, tmp, nsap, nm);
+  if (improper_) launch_zero_blocks<S>(g_, sel(1), out, st_);
+  jinv_t<S>(0, out, s, tmp, nm);
+  hop<S>(sel(1), HOPS_FACE, (R)kappa_, s, s, nullptr, nullptr, out);
+  jinv_t<S>(1, out, s, tmp, nm);
+}
+
+void Lattice::prec_ddd(int prec, void* out, const void* in, int nsap, int nm) {
+  if (nsap > 10 || nm > 10) throw std::invalid_argument("prec_ddd_s: nsap and nm must be <= 10 (qws.cc:798-799)");
+  DISPATCH_PREC(prec, (prec_ddd_t<S>((S*)out, (const S*)in, nsap, nm)));
+}
+
+void Lattice::prec_s(int prec, void* out, const void* in, int nsap, int nm) {
+  if (nsap > 10 || nm > 10) throw std::invalid_argument("prec_s: nsap and nm must be <= 10");
+  DISPATCH_PREC(prec, (prec_s_t<S>((S*)out, (const S*)in, nsap, nm)));
+}
+
+double Lattice::read_scalar(const Scalars* sc, size_t offset) {
+  QWS_HIP(hipMemcpyAsync((char*)sc_host_ + offset, (const char*)sc + offset, sizeof(double), hipMemcpyDeviceToHost, st_));
+  QWS_HIP(hipStreamSynchronize(st_));
+  double v;
+  std::memcpy(&v, (char*)sc_host_ + offset, sizeof(double));
+  return v;
+}
+
+double Lattice::norm2(int prec, const void* a) {
+  const long long n = g_.V * 12;
+  if (prec == P64) launch_norm<double>(n, (const double*)a, part_, st_);
+  else if (prec == P32) launch_norm<float>(n, (const float*)a, part_, st_);
+  else throw std::invalid_argument("norm2: FP16 vectors are not reduced directly");
+  launch_finalize(part_, 1, sc_out_, FIN_RAW, st_);
+  return read_scalar(sc_out_, offsetof(Scalars, red));
+}
+
+// ------------------------------------------------------------------ inner BiCGStab
+// Right-preconditioned BiCGStab for (D M_SAP) x = b, x0 = 0 (bicgstab_precdd_s.cc:85-559).
+template <typename S>
+int Lattice::bicg_t(S* x, const S* b, double tol, int maxiter, int nsap, int nm) {
+  static_assert(sizeof(S) == 4, "the inner solver runs on FP32 vectors");
+  const long long n = g_.V * 12;
+  const size_t bytes = spinor_bytes(P32);
+  S* q = (S*)work("bicg_q", bytes);
+  S* r = (S*)work("bicg_r", bytes);
+  S* p = (S*)work("bicg_p", bytes);
+  S* t = (S*)work("bicg_t", bytes);
+  S* r0 = (S*)work("bicg_r0", bytes);
+  Scalars* sc = sc_in_;
+  launch_bicg_init_inner<S>(n, x, r, p, r0, b, part_, st_);
+  launch_finalize(part_, 1, sc, FIN_INIT_INNER, st_);
+  double bnorm = -1;
+  if (tol >= 0) bnorm = read_scalar(sc, offsetof(Scalars, bnorm));
+  int iter;
+  for (iter = 0; iter < maxiter; ++iter) {
+    prec_ddd_t<S>(q, p, nsap, nm);
+    launch_dot<S>(n, r0, q, part_, st_);
+    launch_finalize(part_, 2, sc, FIN_ALPHA, st_);
+    launch_update_xr<S>(n, x, p, r, q, sc->alpha, nullptr, part_, st_);
+    launch_finalize(part_, 1, sc, FIN_RNORM, st_);
+    if (tol >= 0 && std::sqrt(read_scalar(sc, offsetof(Scalars, rnorm)) / bnorm) < tol) { ++iter; break; }
+    prec_ddd_t<S>(t, r, nsap, nm);
+    launch_norm_dot<S>(n, t, r, part_, st_);
+    launch_finalize(part_, 3, sc, FIN_OMEGA, st_);
+    launch_update_xr<S>(n, x, r, r, t, sc->omega, r0, part_, st_);
+    launch_finalize(part_, 3, sc, FIN_RHO, st_);
+    if (tol >= 0 && std::sqrt(read_scalar(sc, offsetof(Scalars, rnorm)) / bnorm) < tol) { ++iter; break; }
+    launch_update_p<S>(n, p, r, q, sc, st_);
+  }
+  return iter;
+}
+
+int Lattice::bicgstab_precdd(int prec, void* x, const void* b, double tol, int maxiter, int nsap, int nm) {
+  if (prec != P32) throw std::invalid_argument("bicgstab_precdd: FP32 vectors only");
+  return bicg_t<float>((float*)x, (const float*)b, tol, maxiter, nsap, nm);
+}
+
+// ------------------------------------------------------------------ outer mixed BiCGStab
+// u = M p: normalise, FP32 inner solve, prec_s_, rescale (bicgstab_dd_mix.cc:76-102,
+// reordering.cc:108-204).
+void Lattice::approx_inv(double* u, const double* p, const SolveParams& prm, int* inner) {
+  const long long n = g_.V * 12;
+  float* bs = (float*)work("ai_b", spinor_bytes(P32));
+  float* ts = (float*)work("ai_t", spinor_bytes(P32));
+  float* xs = (float*)work("ai_x", spinor_bytes(P32));
+  launch_norm<double>(n, p, part_, st_);
+  launch_finalize(part_, 1, sc_out_, FIN_NRM, st_);
+  launch_d2s_scaled<float>(g_.V, bs, p, sc_out_, st_);
+  int it = bicg_t<float>(ts, bs, prm.tol_s, prm.maxiter_s, prm.nsap, prm.nm);
+  prec_s_t<float>(xs, ts, prm.nsap, prm.nm);
+  launch_s2d_scaled<float>(g_.V, u, xs, sc_out_, st_);
+  if (inner) *inner += it;
+}
+
+// FP64 outer BiCGStab on D, x = initial guess (bicgstab_dd_mix.cc:105-388).
+int Lattice::bicgstab_dd_mix(double* x, const double* b, const SolveParams& prm, int* inner_total) {
+  const long long n = g_.V * 12;
+  const size_t bytes = spinor_bytes(P64);
+  double* q = (double*)work("mix_q", bytes);
+  double* r = (double*)work("mix_r", bytes);
+  double* p = (double*)work("mix_p", bytes);
+  double* t = (double*)work("mix_t", bytes);
+  double* u = (double*)work("mix_u", bytes);
+  double* r0 = (double*)work("mix_r0", bytes);
+  Scalars* sc = sc_out_;
+  dslash(P64, q, x, HOPS_ALL, -1);
+  launch_bicg_init_outer<double>(n, r, p, r0, b, q, part_, st_);
+  launch_finalize(part_, 2, sc, FIN_INIT_OUTER, st_);
+  double bnorm = -1;
+  if (prm.tol >= 0) bnorm = read_scalar(sc, offsetof(Scalars, bnorm));
+  int iter;
+  for (iter = 1; iter < prm.maxiter; ++iter) {
+    approx_inv(u, p, prm, inner_total);
+    dslash(P64, q, u, HOPS_ALL, -1);
+    launch_dot<double>(n, r0, q, part_, st_);
+    launch_finalize(part_, 2, sc, FIN_ALPHA, st_);
+    launch_update_xr<double>(n, x, u, r, q, sc->alpha, nullptr, part_, st_);
+    launch_finalize(part_, 1, sc, FIN_RNORM, st_);
+    if (prm.tol >= 0 && std::sqrt(read_scalar(sc, offsetof(Scalars, rnorm)) / bnorm) < prm.tol) break;
+    approx_inv(u, r, prm, inner_total);
+    dslash(P64, t, u, HOPS_ALL, -1);
+    launch_norm_dot<double>(n, t, r, part_, st_);
+    launch_finalize(part_, 3, sc, FIN_OMEGA, st_);
+    launch_update_xr<double>(n, x, u, r, t, sc->omega, r0, part_, st_);
+    launch_finalize(part_, 3, sc, FIN_RHO, st_);
+    if (prm.tol >= 0 && std::sqrt(read_scalar(sc, offsetof(Scalars, rnorm)) / bnorm) < prm.tol) break;
+    launch_update_p<double>(n, p, r, q, sc, st_);
+  }
+  return iter;
+}
+
+// |b - D x| / |b| in FP64 on the GPU
+double Lattice::true_residual(const double* x, const double* b) {
+  const long long n = g_.V * 12;
+  double* q = (double*)work("res_q", spinor_bytes(P64));
+  dslash(P64, q, x, HOPS_ALL, -1);
+  launch_sub<double>(2 * n, q, b, q, st_);
+  double rr = norm2(P64, q);
+  double bb = norm2(P64, b);
+  return std::sqrt(rr / bb);
+}
+
+SolveStats Lattice::solve(double* x, const double* b, const SolveParams& prm) {
+  SolveStats st;
+  auto t0 = std::chrono::steady_clock::now();
+  for (int attempt = 0;; ++attempt) {
+    int inner = 0;
+    int it = bicgstab_dd_mix(x, b, prm, &inner);
+    st.outer_iters += it;
+    st.inner_iters += inner;
+    st.true_residual = true_residual(x, b);
+    if (prm.true_tol <= 0 || st.true_residual <= prm.true_tol || attempt >= prm.max_restarts) break;
+    st.restarts++;
+  }
+  sync();
+  st.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  return st;
+}
+
+double Lattice::time_dslash(int prec, int reps) {
+  void* in = work("time_in", spinor_bytes(prec));
+  void* out = work("time_out", spinor_bytes(prec));
+  QWS_HIP(hipMemsetAsync(in, 0, spinor_bytes(prec), st_));
+  dslash(prec, out, in, HOPS_ALL, -1);  // warm up
+  hipEvent_t e0, e1;
+  QWS_HIP(hipEventCreate(&e0));
+  QWS_HIP(hipEventCreate(&e1));
+  QWS_HIP(hipEventRecord(e0, st_));
+  for (int i = 0; i < reps; ++i) dslash(prec, out, in, HOPS_ALL, -1);
+  QWS_HIP(hipEventRecord(e1, st_));
+  QWS_HIP(hipEventSynchronize(e1));
+  float ms = 0;
+  QWS_HIP(hipEventElapsedTime(&ms, e0, e1));
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  return ms / reps;
+}
+
+}  // namespace qws

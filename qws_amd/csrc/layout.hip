@@ -1,0 +1,133 @@
+// layout.hip -- conversion between host field layouts (staged in device memory)
+// and the device's blocked site-major SoA layout.
+//
+// Host layouts:
+//   plain : lexicographic site index x + Lx*(y + Ly*(z + Lz*t)); per site 24
+//           (spinor), 4 x 18 (links, [mu][V][18]) or 72 (clover) reals
+//   QWS   : single-rank AoSoA of qws.h:111-192 with lane width vlen: site ->
+//           x-domain d = x / (Lx/2), vector i = (x%nxh)/vlen + nxv*(y + ny*(z + nz*t)),
+//           lane j = (x%nxh)%vlen (main.cc:426-429,485-488, reordering.cc:66-69);
+//           spinor scd_t/scs_t[i + vol*d], links glud[i + vol*mu + 4*vol*d],
+//           clover clvd[i + vol*d]
+#include "kernels.h"
+
+namespace qws {
+
+struct HostMap {
+  int layout, vlen, nxh, nxv;
+  long long volv;  // vectors per x-domain
+};
+
+__device__ __forceinline__ void global_coords(const Geo& g, long long s, int x[4]) {
+  SiteC c;
+  site_coords(g, s, c);
+#pragma unroll
+  for (int m = 0; m < 4; ++m) x[m] = c.b[m] * g.B[m] + c.l[m];
+}
+
+// offset of real k of the per-site record (nrec reals) for direction/record r
+__device__ __forceinline__ long long host_offset(const Geo& g, const HostMap& h, const int x[4], int kind, int mu,
+                                                 int k) {
+  if (h.layout == 0) {
+    long long lex = x[0] + (long long)g.L[0] * (x[1] + (long long)g.L[1] * (x[2] + (long long)g.L[2] * x[3]));
+    if (kind == 0) return lex * 24 + k;
+    if (kind == 1) return ((long long)mu * g.V + lex) * 18 + k;
+    return lex * 72 + k;
+  }
+  const int d = x[0] / h.nxh, xr = x[0] % h.nxh;
+  const long long i = xr / h.vlen + (long long)h.nxv * (x[1] + (long long)g.L[1] * (x[2] + (long long)g.L[2] * x[3]));
+  const int j = xr % h.vlen;
+  if (kind == 0) return ((i + h.volv * d) * 24 + k) * h.vlen + j;
+  if (kind == 1) return ((i + h.volv * mu + 4 * h.volv * d) * 18 + k) * h.vlen + j;
+  return ((i + h.volv * d) * 72 + k) * h.vlen + j;
+}
+
+template <typename S> __device__ __forceinline__ void store_real(S* p, long long i, double v);
+template <> __device__ __forceinline__ void store_real<float>(float* p, long long i, double v) { p[i] = (float)v; }
+template <> __device__ __forceinline__ void store_real<double>(double* p, long long i, double v) { p[i] = v; }
+template <> __device__ __forceinline__ void store_real<__half>(__half* p, long long i, double v) { p[i] = __float2half_rn((float)v); }
+
+template <typename S> __device__ __forceinline__ double load_real(const S* p, long long i);
+template <> __device__ __forceinline__ double load_real<float>(const float* p, long long i) { return p[i]; }
+template <> __device__ __forceinline__ double load_real<double>(const double* p, long long i) { return p[i]; }
+template <> __device__ __forceinline__ double load_real<__half>(const __half* p, long long i) { return __half2float(p[i]); }
+
+// device offset of real k of a site record
+template <typename S>
+__device__ __forceinline__ long long dev_offset(const Geo& g, int kind, int nl, long long s, int mu, int k) {
+  if (kind == 0) {
+    if (sizeof(S) == 4) return ((long long)(k >> 2) * g.V + s) * 4 + (k & 3);  // float4 [6][V]
+    return ((long long)(k >> 1) * g.V + s) * 2 + (k & 1);                     // Vec2 [12][V]
+  }
+  if (kind == 1) return (((long long)mu * (nl / 2) + (k >> 1)) * g.V + s) * 2 + (k & 1);
+  if (sizeof(S) == 4) return ((long long)(k >> 2) * g.V + s) * 4 + (k & 3);  // float4 [18][V]
+  return ((long long)(k >> 1) * g.V + s) * 2 + (k & 1);                       // Vec2 [36][V]
+}
+
+template <typename S>
+__global__ void k_import(Geo g, HostMap h, int kind, int nl, const void* src, int src_is_double, S* dst) {
+  const long long s = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (s >= g.V) return;
+  int x[4];
+  global_coords(g, s, x);
+  const int nmu = kind == 1 ? 4 : 1;
+  const int nrec = kind == 0 ? 24 : (kind == 1 ? nl : 72);
+  for (int mu = 0; mu < nmu; ++mu)
+    for (int k = 0; k < nrec; ++k) {
+      long long o = host_offset(g, h, x, kind, mu, k);
+      double v = src_is_double ? ((const double*)src)[o] : (double)((const float*)src)[o];
+      store_real<S>(dst, dev_offset<S>(g, kind, nl, s, mu, k), v);
+    }
+}
+
+template <typename S>
+__global__ void k_export(Geo g, HostMap h, const S* src, void* dst, int dst_is_double) {
+  const long long s = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (s >= g.V) return;
+  int x[4];
+  global_coords(g, s, x);
+  for (int k = 0; k < 24; ++k) {
+    double v = load_real<S>(src, dev_offset<S>(g, 0, 0, s, 0, k));
+    long long o = host_offset(g, h, x, 0, 0, k);
+    if (dst_is_double) ((double*)dst)[o] = v;
+    else ((float*)dst)[o] = (float)v;
+  }
+}
+
+static HostMap make_map(const Geo& g, int layout, int vlen) {
+  HostMap h;
+  h.layout = layout;
+  h.vlen = vlen > 0 ? vlen : 1;
+  h.nxh = g.L[0] / 2;
+  h.nxv = h.nxh / h.vlen;
+  h.volv = (long long)h.nxv * g.L[1] * g.L[2] * g.L[3];
+  return h;
+}
+
+template <typename S>
+void launch_import(const Geo& g, int kind, int layout, int vlen, int nl, const void* src, int src_is_double, S* dst,
+                   hipStream_t st) {
+  HostMap h = make_map(g, layout, vlen);
+  const int T = 128;
+  hipLaunchKernelGGL((k_import<S>), dim3((unsigned)((g.V + T - 1) / T)), dim3(T), 0, st, g, h, kind, nl, src,
+                     src_is_double, dst);
+}
+
+template <typename S>
+void launch_export(const Geo& g, int kind, int layout, int vlen, const S* src, void* dst, int dst_is_double,
+                   hipStream_t st) {
+  (void)kind;
+  HostMap h = make_map(g, layout, vlen);
+  const int T = 128;
+  hipLaunchKernelGGL((k_export<S>), dim3((unsigned)((g.V + T - 1) / T)), dim3(T), 0, st, g, h, src, dst,
+                     dst_is_double);
+}
+
+template void launch_import<float>(const Geo&, int, int, int, int, const void*, int, float*, hipStream_t);
+template void launch_import<double>(const Geo&, int, int, int, int, const void*, int, double*, hipStream_t);
+template void launch_import<__half>(const Geo&, int, int, int, int, const void*, int, __half*, hipStream_t);
+template void launch_export<float>(const Geo&, int, int, int, const float*, void*, int, hipStream_t);
+template void launch_export<double>(const Geo&, int, int, int, const double*, void*, int, hipStream_t);
+template void launch_export<__half>(const Geo&, int, int, int, const __half*, void*, int, hipStream_t);
+
+}  // namespace qws

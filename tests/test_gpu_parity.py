@@ -1,0 +1,244 @@
+"""GPU parity: the HIP library (through its C ABI) against the CPU oracle and the
+reference's golden outputs.
+
+Tolerances (stated by the task): operator applications within 1e-13 relative in
+FP64 and 1e-6 relative in FP32; composite FP32 operators (Jacobi, SAP) within a
+few 1e-6; solves reach the reference's golden solution norms under check.sh
+tolerances.
+"""
+import ctypes
+
+import numpy as np
+import pytest
+
+import qws_amd as Q
+from oracle import oracle as O
+from oracle import ref_worker as RW
+from tests import casekit as K
+
+pytestmark = pytest.mark.gpu
+
+TOL64 = 1e-13
+TOL32 = 1e-6
+
+
+def rel(a, b):
+    return float(np.linalg.norm((np.asarray(a) - np.asarray(b)).ravel()) / np.linalg.norm(np.asarray(b).ravel()))
+
+
+def random_su3(n, rng):
+    """n random SU(3) matrices as QWS link arrays (n,3,3,2)."""
+    z = rng.standard_normal((n, 3, 3)) + 1j * rng.standard_normal((n, 3, 3))
+    q, r = np.linalg.qr(z)
+    d = np.diagonal(r, axis1=1, axis2=2)
+    q = q * (d / np.abs(d))[:, None, :]
+    det = np.linalg.det(q)
+    q[:, :, 0] /= det[:, None]  # det(q) = 1 after scaling one column by 1/det (|det| = 1)
+    return np.stack([q.real, q.imag], axis=-1)
+
+
+class Problem:
+    def __init__(self, L, B, kappa=0.12, seed=5, su3=False, case=None):
+        self.L, self.B = L, B
+        V = int(np.prod(L))
+        rng = np.random.default_rng(seed)
+        if case is not None:
+            nloc, npe, L2, B2 = K.case_geometry(case)
+            assert L2 == L and B2 == B
+            _, U, C, b, x0 = O.case_inputs(nloc, npe)
+            self.b, self.x0 = b, x0
+        else:
+            U, C = K.random_su3_like(V, rng)
+            if su3:
+                U = random_su3(4 * V, rng).reshape(4, V, 3, 3, 2)
+            self.b = rng.standard_normal((V, 3, 4, 2))
+            self.x0 = rng.standard_normal((V, 3, 4, 2))
+        self.U, self.C, self.kappa = U, C, kappa
+        self.orc = O.Oracle(L, B, -1, kappa, U, C)
+        self.lat = Q.Lattice(L, B, -1, 12 if su3 else 18, kappa=kappa)
+        self.lat.load_gauge(U)
+        self.lat.load_clover(C)
+
+
+GEOMS = [
+    ("case0", [32, 6, 4, 3], [16, 6, 4, 3]),       # QWS single rank: improper colouring
+    ("proper", [8, 4, 4, 8], [2, 2, 2, 2]),        # every direction split: proper colouring
+    ("mixed", [8, 4, 2, 4], [4, 2, 2, 4]),         # z, t single block: improper
+    ("b4", [16, 8, 8, 8], [4, 4, 4, 4]),           # 4^4 blocks (the solver's GPU shape)
+]
+
+
+@pytest.fixture(scope="module", params=GEOMS, ids=[g[0] for g in GEOMS])
+def prob(request):
+    _, L, B = request.param
+    return Problem(L, B)
+
+
+@pytest.mark.parametrize("hops", [Q.HOPS_ALL, Q.HOPS_INSIDE, Q.HOPS_FACE])
+@pytest.mark.parametrize("colour", [-1, 0, 1])
+def test_dslash_fp64_fp32(prob, hops, colour):
+    p, lat = prob, prob.lat
+    for prec, dt, tol in ((Q.F64, np.float64, TOL64), (Q.F32, np.float32, TOL32)):
+        src = p.b.astype(dt)
+        inp = lat.spinor(prec, src)
+        out = lat.spinor(prec, np.zeros_like(p.b))
+        lat.dslash(out, inp, hops, colour)
+        want = p.orc.dslash(src, hops, colour, out=np.zeros_like(src))
+        assert rel(out.download(), want) < tol, (prec, hops, colour)
+
+
+@pytest.mark.parametrize("colour", [-1, 0, 1])
+def test_face_accumulate(prob, colour):
+    p, lat = prob, prob.lat
+    for prec, dt, tol in ((Q.F64, np.float64, TOL64), (Q.F32, np.float32, TOL32)):
+        base = (0.5 * p.x0).astype(dt)
+        out = lat.spinor(prec, base)
+        inp = lat.spinor(prec, p.b.astype(dt))
+        lat.face(out, inp, 0.37, colour)
+        want = p.orc.face(base.copy(), p.b.astype(dt), 0.37, colour)
+        assert rel(out.download(), want) < tol
+
+
+def test_clover(prob):
+    p, lat = prob, prob.lat
+    out = lat.spinor(Q.F64)
+    lat.clover(out, lat.spinor(Q.F64, p.b))
+    assert rel(out.download(), p.orc.clover(p.b)) < TOL64
+
+
+@pytest.mark.parametrize("nm", [1, 2, 3])
+def test_jinv(prob, nm):
+    p, lat = prob, prob.lat
+    bs = p.b.astype(np.float32)
+    for col in (0, 1):
+        x = lat.spinor(Q.F32, np.zeros_like(p.b))
+        lat.jinv(x, lat.spinor(Q.F32, bs), col, nm)
+        want = p.orc.jinv_s(bs, col, nm, x=np.zeros_like(bs))
+        assert rel(x.download(), want) < 3e-6
+
+
+@pytest.mark.parametrize("nsap,nm", [(4, 2), (2, 3), (1, 1)])
+def test_prec_ddd_and_prec_s(prob, nsap, nm):
+    p, lat = prob, prob.lat
+    bs = p.b.astype(np.float32)
+    inp = lat.spinor(Q.F32, bs)
+    out = lat.spinor(Q.F32)
+    lat.prec_ddd(out, inp, nsap, nm)
+    assert rel(out.download(), p.orc.prec_ddd_s(bs, nsap, nm)) < 5e-6
+    lat.prec_s(out, inp, nsap, nm)
+    assert rel(out.download(), p.orc.prec_s(bs, nsap, nm)) < 5e-6
+
+
+def test_inner_bicgstab(prob):
+    p, lat = prob, prob.lat
+    bs = (p.b / np.sqrt(O.norm2(p.b))).astype(np.float32)
+    x = lat.spinor(Q.F32)
+    it = lat.bicgstab_precdd(x, lat.spinor(Q.F32, bs), -1.0, 3, 4, 2)
+    want, it_o = p.orc.bicgstab_precdd_s(bs, -1.0, 3, 4, 2)
+    assert it == it_o == 3
+    assert rel(x.download(), want) < 1e-4
+
+
+def test_links12_reconstruction():
+    """12-real FP32 links (third row rebuilt) equal 18-real storage for SU(3) links."""
+    p = Problem([8, 8, 4, 8], [4, 4, 2, 4], su3=True, seed=11)
+    lat18 = Q.Lattice(p.L, p.B, -1, 18, kappa=p.kappa)
+    lat18.load_gauge(p.U)
+    lat18.load_clover(p.C)
+    bs = p.b.astype(np.float32)
+    outs = []
+    for lat in (p.lat, lat18):
+        o = lat.spinor(Q.F32)
+        lat.dslash(o, lat.spinor(Q.F32, bs))
+        outs.append(o.download())
+    assert rel(outs[0], outs[1]) < 1e-6
+    assert rel(outs[0], p.orc.dslash(bs)) < TOL32
+
+
+# ---------------------------------------------------------------- solves vs golden norms
+@pytest.mark.parametrize("case", ["CASE0", "CASE_4111_1111", "CASE1", "CASE7"])
+def test_dd_mixed_solve_golden(case):
+    """bicgstab_dd_mix_ with the driver's settings (main.cc:262-312) reaches the
+    reference's solution: |x|^2 at 1e-14, |b - D x|^2 < 1e-15."""
+    nloc, npe, L, B = K.case_geometry(case)
+    _, U, C, b, x0 = O.case_inputs(nloc, npe)
+    lat = Q.Lattice(L, B, -1, 18, kappa=0.05)
+    lat.load_gauge(U)
+    lat.load_clover(C)
+    x = lat.spinor(Q.F64, x0)
+    bb = lat.spinor(Q.F64, b)
+    st = lat.solve(x, bb, nsap=4, nm=2, tol=-1, maxiter=6, tol_s=-1, maxiter_s=50)
+    assert st["outer_iters"] == 6 and st["inner_iters"] == 500
+    xs = x.download()
+    orc = O.Oracle(L, B, -1, 0.05, U, C)
+    r = b - orc.dslash(xs)
+    assert O.norm2(r) < 1e-15
+    assert K.check_value("xnorm^2", O.norm2(xs), K.expected(case, "xnorm^2", "DD"))
+    assert st["true_residual"] < 1e-10
+
+
+def test_solve_to_true_residual():
+    p = Problem([8, 8, 8, 8], [4, 4, 4, 4], kappa=0.1, seed=21)
+    x = p.lat.spinor(Q.F64)
+    st = p.lat.solve(x, p.lat.spinor(Q.F64, p.b), nsap=4, nm=2, tol=1e-11, maxiter=50, tol_s=1e-3,
+                     maxiter_s=20, true_tol=1e-10, max_restarts=3)
+    assert st["true_residual"] <= 1e-10
+    r = p.b - p.orc.dslash(x.download())
+    assert np.sqrt(O.norm2(r) / O.norm2(p.b)) <= 1e-10
+
+
+# ---------------------------------------------------------------- QWS drop-in ABI
+def _ptr(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def test_dropin_abi_case0():
+    """The QWS entry points on QWS's host layout reproduce data/CASE0."""
+    L = Q.lib()
+    nloc = [32, 6, 4, 3]
+    _, U, C, b, x0 = O.case_inputs(nloc)
+    ci = lambda v: ctypes.byref(ctypes.c_int(v))
+    keep = [(ctypes.c_int * 4)(1, 1, 1, 1), (ctypes.c_int * 4)(1, 1, 1, -1), ctypes.c_int(0), ctypes.c_int(1),
+            (ctypes.c_int * 4)(1, 2, 2, 2)]
+    L.qws_init_(ci(32), ci(6), ci(4), ci(3), keep[0], keep[1], ctypes.byref(keep[2]), ctypes.byref(keep[3]), keep[4])
+    vold = ctypes.c_int.in_dll(L, "vold").value
+    vols = ctypes.c_int.in_dll(L, "vols").value
+
+    def glob(name, ct, n):
+        return np.ctypeslib.as_array((ct * n).from_address(ctypes.c_void_p.in_dll(L, name).value))
+
+    glob("glud", ctypes.c_double, vold * 8 * 144)[:] = RW.links_to_aosoa(U, nloc, 8).ravel()
+    glob("glus", ctypes.c_float, vols * 8 * 288)[:] = RW.links_to_aosoa(U.astype(np.float32), nloc, 16).ravel()
+    glob("clvd", ctypes.c_double, vold * 2 * 576)[:] = RW.clover_to_aosoa(C, nloc, 8).ravel()
+    glob("clvs", ctypes.c_float, vols * 2 * 1152)[:] = RW.clover_to_aosoa(C.astype(np.float32), nloc, 16).ravel()
+    ctypes.c_double.in_dll(L, "kappa").value = 0.05
+    ctypes.c_double.in_dll(L, "mkappa").value = -0.05
+    ctypes.c_double.in_dll(L, "kappa2").value = 0.0025
+    L.qws_mi355x_fields_modified_()
+    bd = RW.spinor_to_aosoa(b, nloc, 8)
+    bs = RW.spinor_to_aosoa(b.astype(np.float32), nloc, 16)
+    out = np.zeros_like(bd)
+    L.ddd_d_(_ptr(out), _ptr(bd))
+    assert K.check_value("ddd_d_", O.norm2(out), K.expected("CASE0", "ddd_d_"))
+    outs = np.zeros_like(bs)
+    L.ddd_s_(_ptr(outs), _ptr(bs))
+    assert K.check_value("ddd_s_", O.norm2(outs), K.expected("CASE0", "ddd_s_"))
+    outs[:] = 0
+    L.ddd_in_s_(_ptr(outs), _ptr(bs), ci(0))
+    assert K.check_value("ddd_in_s_", O.norm2(outs), K.expected("CASE0", "ddd_in_s_"))
+    outs[:] = 0
+    L.ddd_out_pre_s_(_ptr(bs), ci(0))
+    L.ddd_out_pos_s_.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_float]
+    L.ddd_out_pos_s_(_ptr(outs), _ptr(bs), ci(0), ctypes.c_float(-0.05))
+    assert K.check_value("ddd_out_pre|pos_s_", O.norm2(outs), K.expected("CASE0", "ddd_out_pre|pos_s_"))
+    # the mixed solve (main.cc:280)
+    x = RW.spinor_to_aosoa(x0, nloc, 8)
+    it = ctypes.c_int(0)
+    cd = lambda v: ctypes.byref(ctypes.c_double(v))
+    L.bicgstab_dd_mix_(_ptr(x), _ptr(bd), cd(-1.0), ctypes.byref(it), ci(6), cd(-1.0), ci(50), ci(4), ci(2))
+    assert it.value == 6
+    r = np.zeros_like(bd)
+    L.ddd_d_(_ptr(r), _ptr(x))
+    assert O.norm2(bd - r) < 1e-15
+    assert K.check_value("xnorm^2", O.norm2(x), K.expected("CASE0", "xnorm^2", "DD"))
+    L.qws_finalize_()
