@@ -1,0 +1,167 @@
+"""Benchmark: plain mixed-precision SAP-BiCGStab solve on one MI355X.
+
+Workload (one step = one solve from x = 0 to an FP64 true residual
+|b - D x| / |b| <= 1e-10): 32^3 x 64 lattice, SU(3) links U = exp(0.3 i H) (Philox
+seed 2), kappa = 0.1365, c_sw = 1.0 clover term built from the links, antiperiodic
+t, Gaussian FP64 source (seed 2); FP64 outer BiCGStab with FP32 SAP-preconditioned
+inner BiCGStab on 4^4 blocks; FP32 links stored as 12 reals.
+
+Prints one JSON line.  value = solves/s; roofline = the FP32 dslash kernel (full
+D = 1 - kappa T^-1 H), timed with HIP events on the library's stream, against the
+MI355X HBM peak; cpu_baseline = the QWS reference library (oracle/_ref) timed on
+this host on a bounded sample of the same solve.
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu-baseline]
+"""
+import argparse
+import json
+import os
+import subprocess
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+WORKLOAD = dict(L=[32, 32, 32, 64], B=[4, 4, 4, 4], eps=0.3, seed=2, kappa=0.1365, csw=1.0, tbc=-1, links32=12)
+# Solver settings of the measured workload (tuned on this hardware)
+SOLVE = dict(nsap=4, nm=2, tol=1e-11, maxiter=40, tol_s=1e-3, maxiter_s=40, true_tol=1e-10, max_restarts=4)
+# algorithmic bytes per site of one FP32 D application with 12-real links:
+# 4 links x 12 x 4 B + clover 72 x 4 B + input 24 x 4 B + output 24 x 4 B
+BYTES_PER_SITE_F32_12 = 4 * 12 * 4 + 72 * 4 + 24 * 4 + 24 * 4
+
+
+def setup(Q, w=WORKLOAD):
+    lat = Q.Lattice(w["L"], w["B"], w["tbc"], w["links32"], kappa=w["kappa"])
+    lat.generate_gauge(w["eps"], w["seed"])
+    lat.compute_clover(w["csw"])
+    b = lat.gaussian(Q.F64, w["seed"])
+    return lat, b
+
+
+def run_solves(Q, lat, b, n, params):
+    x = lat.spinor(Q.F64)
+    stats = []
+    lat.sync()
+    t0 = time.perf_counter()
+    for _ in range(n):
+        x.zero()
+        stats.append(lat.solve(x, b, **params))
+    lat.sync()
+    return time.perf_counter() - t0, stats, x
+
+
+def cpu_baseline(Q, lat, stats, workdir):
+    """QWS reference (oracle/_ref/libqws_ref.so) on this host: time per inner
+    iteration (bicgstab_precdd_s_, nsap=4, nm=2) on the same lattice and fields, in
+    QWS's single-rank geometry (two SAP domains of 16 x 32 x 32 x 64)."""
+    from oracle import ref_worker as RW
+
+    if not RW.available():
+        return None
+    U = lat.gauge()
+    C = lat.clover_field()
+    b = lat.gaussian(Q.F64, WORKLOAD["seed"]).download()
+    nloc = WORKLOAD["L"]
+    fin = os.path.join(workdir, "in.npz")
+    fout = os.path.join(workdir, "out.npz")
+    sample_iters = 2
+    np.savez(fin, nloc=np.array(nloc), kappa=np.array(WORKLOAD["kappa"]), U=U, C=C, b=b, x0=np.zeros_like(b),
+             ops=np.array(["time_inner"]), nsap=np.array(4), nm=np.array(2), tol=np.array(-1.0),
+             maxiter=np.array(2), tol_s=np.array(-1.0), maxiter_s=np.array(sample_iters))
+    del U, C
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    env = dict(os.environ, OMP_NUM_THREADS=str(cores))
+    t0 = time.perf_counter()
+    p = subprocess.run([sys.executable, "-m", "oracle.ref_worker", fin, fout], cwd=ROOT, env=env,
+                       capture_output=True, text=True, timeout=900)
+    if p.returncode != 0:
+        print("cpu baseline failed:", p.stderr[-2000:], file=sys.stderr)
+        return None
+    res = dict(np.load(fout))
+    t_iter = float(res["time_inner"]) / sample_iters
+    inner = np.mean([s["inner_iters"] for s in stats])
+    outer = np.mean([s["outer_iters"] for s in stats])
+    t_solve = t_iter * inner
+    return {"value": 1.0 / t_solve, "unit": "solves/s", "cores": cores, "kind": "reference",
+            "sample": (f"QWS bicgstab_precdd_s_ (nsap=4, nm=2) on the same 32^3x64 fields, {sample_iters} inner "
+                       f"iterations: {t_iter:.3f} s/iteration x {inner:.0f} inner iterations of the GPU solve "
+                       f"({outer:.0f} outer); QWS single-rank SAP domains 16x32x32x64; wall {time.perf_counter()-t0:.0f} s")}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--dslash-only", type=int, default=0, help="only time N dslash launches (for counter runs)")
+    ap.add_argument("--sweep", action="store_true", help="print solve statistics for a grid of solver settings")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is design-only (DESIGN.md)",
+                          "n_gpus": a.gpus}))
+        sys.exit(2)
+    import qws_amd as Q
+
+    lat, b = setup(Q)
+    V = lat.V
+    if a.dslash_only:
+        ms = lat.time_dslash(Q.F32, a.dslash_only)
+        print(json.dumps({"dslash_ms": ms, "GBs": V * BYTES_PER_SITE_F32_12 / (ms * 1e-3) / 1e9}))
+        return
+    if a.sweep:
+        t_start = time.perf_counter()
+        for nsap, nm in ((2, 2), (4, 2), (3, 3), (4, 3), (6, 2)):
+            for tol_s, mis in ((1e-2, 40), (1e-3, 60)):
+                if time.perf_counter() - t_start > 240:
+                    print(json.dumps({"sweep": "stopped at its time budget"}), flush=True)
+                    return
+                prm = dict(SOLVE, nsap=nsap, nm=nm, tol_s=tol_s, maxiter_s=mis)
+                dt, st, _ = run_solves(Q, lat, b, 1, prm)
+                print(json.dumps({"nsap": nsap, "nm": nm, "tol_s": tol_s, "s": round(dt, 3), **st[0]}), flush=True)
+        return
+
+    # warm-up and timed solves
+    run_solves(Q, lat, b, a.warmup, SOLVE)
+    dt, stats, x = run_solves(Q, lat, b, a.steps, SOLVE)
+    ms_per = dt / a.steps * 1e3
+    worst = max(s["true_residual"] for s in stats)
+    # dslash roofline: HIP events on the library stream, same fields
+    reps = 50
+    ms_d = lat.time_dslash(Q.F32, reps)
+    achieved = V * BYTES_PER_SITE_F32_12 / (ms_d * 1e-3) / 1e9
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", "dslash_f32_32c64_traffic.json")
+    if os.path.exists(tfile):
+        traffic = json.load(open(tfile)).get("bytes_per_launch")
+    line = {
+        "metric": "solves/s at 1e-10 residual (batched/served vs plain); dslash GB/s; p99 ms",
+        "value": 1e3 / ms_per, "unit": "solves/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+        "ms_per_step": ms_per, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64/f32",
+        "data": "synthetic (Philox SU(3) links exp(0.3iH), clover from links, Gaussian source)",
+        "config": {"workload": "plain single-RHS solve, 32^3x64, 4^4 SAP blocks, kappa=0.1365, csw=1.0, "
+                               "FP64 outer / FP32 inner to 1e-10 FP64 true residual",
+                   "lattice": WORKLOAD["L"], "sap_block": WORKLOAD["B"], **SOLVE},
+        "true_residual_max": worst,
+        "iterations": {"outer": stats[-1]["outer_iters"], "inner": stats[-1]["inner_iters"],
+                       "restarts": stats[-1]["restarts"]},
+        "dslash_GBs": achieved,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": "k_hop<float,12,HOPS_ALL> (FP32 D, 672 B/site algorithmic)",
+                     "ms_per_launch": ms_d},
+        "cpu_baseline": None,
+    }
+    if not a.no_cpu_baseline:
+        with tempfile.TemporaryDirectory() as td:
+            line["cpu_baseline"] = cpu_baseline(Q, lat, stats, td)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

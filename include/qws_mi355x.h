@@ -142,6 +142,9 @@ int qwsg_device_count(int* n);
 int qwsg_create(const int* L, const int* B, int tbc, int links32, qwsg_lattice** out);
 int qwsg_destroy(qwsg_lattice* lat);
 int qwsg_set_kappa(qwsg_lattice* lat, double kappa);
+/* SAP kernels: 1 (default) = one fused launch per half-sweep where the geometry
+ * allows it (proper colouring, 256-site blocks), 0 = one launch per operator step */
+int qwsg_set_sap_fused(qwsg_lattice* lat, int on);
 int qwsg_volume(qwsg_lattice* lat, long long* V);
 /* host plain layout: links [4][V][3][3][2] (QWS c[i][j] of the link x -> x+mu),
  * clover inverse [V][2][36] (QWS packing), site x + Lx*(y + Ly*(z + Lz*t)) */
@@ -150,6 +153,21 @@ int qwsg_load_clover(qwsg_lattice* lat, const double* C);
 /* FP32 fields rounded from separate FP32 host arrays (QWS's glus/clvs) */
 int qwsg_load_gauge_f32(qwsg_lattice* lat, const float* U);
 int qwsg_load_clover_f32(qwsg_lattice* lat, const float* C);
+
+/* No counterpart in the reference (QWS receives its fields from the caller):
+ * generated inputs.  Random numbers are Philox4x32-10 with counter {site, (mu<<8)|k,
+ * purpose, 0} and key = seed, so a field is the same on any launch shape.
+ *   gauge : U = exp(i eps H), H = (A + A^+)/2 - tr/3, A complex Gaussian (FP64)
+ *   clover: T = 1 + i kappa csw sum_{mu<nu} sigma_{mu nu} F_{mu nu} (clover leaves),
+ *           each chirality block inverted in FP64, stored as QWS's 1/2 T^-1
+ *           (uses the lattice's current kappa)
+ *   spinor: Gaussian (re, im ~ N(0,1)) or a point source at site (x,y,z,t) */
+int qwsg_generate_gauge(qwsg_lattice* lat, double eps, unsigned long long seed);
+int qwsg_compute_clover(qwsg_lattice* lat, double csw);
+int qwsg_gauge_download(qwsg_lattice* lat, double* U);
+int qwsg_clover_download(qwsg_lattice* lat, double* C);
+int qwsg_spinor_gaussian(qwsg_lattice* lat, int prec, void* dev, unsigned long long seed);
+int qwsg_spinor_point(qwsg_lattice* lat, int prec, void* dev, const int* site, int spin, int colour);
 
 /* spinors: device buffers of V sites in the given precision */
 int qwsg_spinor_alloc(qwsg_lattice* lat, int prec, void** dev);

@@ -171,6 +171,15 @@ def run(inp):
             lib.bicgstab_precdd_s_(_ptr(out), _ptr(bs), _dbl(float(inp["tol_s"])), ctypes.byref(conv),
                                    _i(int(inp["maxiter_s"])), _i(nsap), _i(nm))
             res[op] = r.back_s(out)
+        elif op == "time_inner":  # wall time of bicgstab_precdd_s_ with maxiter_s fixed iterations
+            import time
+            bs = r.s(b / np.sqrt(np.sum(np.asarray(b, dtype=np.float64) ** 2)))
+            out = np.zeros_like(bs)
+            conv = ctypes.c_int(0)
+            t0 = time.perf_counter()
+            lib.bicgstab_precdd_s_(_ptr(out), _ptr(bs), _dbl(-1.0), ctypes.byref(conv),
+                                   _i(int(inp["maxiter_s"])), _i(nsap), _i(nm))
+            res[op] = np.array(time.perf_counter() - t0)
         elif op == "bicgstab_dd_mix":
             x = r.d(inp["x0"])
             conv = ctypes.c_int(0)

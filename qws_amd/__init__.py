@@ -12,7 +12,7 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIBPATH = os.path.join(HERE, "lib", "libqws_mi355x.so")
+LIBPATH = os.environ.get("QWS_MI355X_LIB", os.path.join(HERE, "lib", "libqws_mi355x.so"))
 HEADER = os.path.join(os.path.dirname(HERE), "include", "qws_mi355x.h")
 
 F64, F32, F16 = 0, 1, 2
@@ -48,6 +48,7 @@ def lib():
         L.qwsg_create.argtypes = [vp, vp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(vp)]
         L.qwsg_destroy.argtypes = [vp]
         L.qwsg_set_kappa.argtypes = [vp, ctypes.c_double]
+        L.qwsg_set_sap_fused.argtypes = [vp, ctypes.c_int]
         L.qwsg_volume.argtypes = [vp, ctypes.POINTER(ctypes.c_longlong)]
         for n in ("qwsg_load_gauge", "qwsg_load_clover", "qwsg_load_gauge_f32", "qwsg_load_clover_f32"):
             getattr(L, n).argtypes = [vp, vp]
@@ -69,6 +70,12 @@ def lib():
         L.qwsg_synchronize.argtypes = [vp]
         L.qwsg_time_dslash.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
         L.qwsg_device_count.argtypes = [ip]
+        L.qwsg_generate_gauge.argtypes = [vp, ctypes.c_double, ctypes.c_ulonglong]
+        L.qwsg_compute_clover.argtypes = [vp, ctypes.c_double]
+        L.qwsg_gauge_download.argtypes = [vp, vp]
+        L.qwsg_clover_download.argtypes = [vp, vp]
+        L.qwsg_spinor_gaussian.argtypes = [vp, ctypes.c_int, vp, ctypes.c_ulonglong]
+        L.qwsg_spinor_point.argtypes = [vp, ctypes.c_int, vp, vp, ctypes.c_int, ctypes.c_int]
         _lib = L
     return _lib
 
@@ -159,6 +166,9 @@ class Lattice:
         except Exception:
             pass
 
+    def set_sap_fused(self, on):
+        check(lib().qwsg_set_sap_fused(self.h, int(bool(on))))
+
     def set_kappa(self, k):
         self.kappa = float(k)
         check(lib().qwsg_set_kappa(self.h, self.kappa))
@@ -174,6 +184,33 @@ class Lattice:
 
     def load_clover_f32(self, C):
         check(lib().qwsg_load_clover_f32(self.h, _p(np.ascontiguousarray(C, dtype=np.float32))))
+
+    def generate_gauge(self, eps, seed):
+        check(lib().qwsg_generate_gauge(self.h, float(eps), int(seed)))
+
+    def compute_clover(self, csw):
+        check(lib().qwsg_compute_clover(self.h, float(csw)))
+
+    def gauge(self):
+        U = np.empty((4, self.V, 3, 3, 2))
+        check(lib().qwsg_gauge_download(self.h, _p(U)))
+        return U
+
+    def clover_field(self):
+        C = np.empty((self.V, 2, 36))
+        check(lib().qwsg_clover_download(self.h, _p(C)))
+        return C
+
+    def gaussian(self, prec, seed):
+        s = Spinor(self, prec)
+        check(lib().qwsg_spinor_gaussian(self.h, prec, s.ptr, int(seed)))
+        return s
+
+    def point(self, prec, site, spin, colour):
+        s = Spinor(self, prec)
+        site = (ctypes.c_int * 4)(*site)
+        check(lib().qwsg_spinor_point(self.h, prec, s.ptr, site, spin, colour))
+        return s
 
     def spinor(self, prec, host=None):
         s = Spinor(self, prec)

@@ -60,6 +60,9 @@ int qwsg_destroy(qwsg_lattice* h) {
     }
   });
 }
+int qwsg_set_sap_fused(qwsg_lattice* h, int on) {
+  return guarded([&] { h->lat->set_fused_sap(on != 0); });
+}
 int qwsg_set_kappa(qwsg_lattice* h, double k) {
   return guarded([&] { h->lat->set_kappa(k); });
 }
@@ -77,6 +80,24 @@ int qwsg_load_gauge_f32(qwsg_lattice* h, const float* U) {
 }
 int qwsg_load_clover_f32(qwsg_lattice* h, const float* C) {
   return guarded([&] { h->lat->load_clover(C, 0, 0, 2); });
+}
+int qwsg_generate_gauge(qwsg_lattice* h, double eps, unsigned long long seed) {
+  return guarded([&] { h->lat->generate_gauge(eps, seed); });
+}
+int qwsg_compute_clover(qwsg_lattice* h, double csw) {
+  return guarded([&] { h->lat->compute_clover(csw); });
+}
+int qwsg_gauge_download(qwsg_lattice* h, double* U) {
+  return guarded([&] { h->lat->export_field(1, U); });
+}
+int qwsg_clover_download(qwsg_lattice* h, double* C) {
+  return guarded([&] { h->lat->export_field(2, C); });
+}
+int qwsg_spinor_gaussian(qwsg_lattice* h, int prec, void* dev, unsigned long long seed) {
+  return guarded([&] { h->lat->gaussian_spinor(prec, dev, seed); });
+}
+int qwsg_spinor_point(qwsg_lattice* h, int prec, void* dev, const int* site, int spin, int colour) {
+  return guarded([&] { h->lat->point_spinor(prec, dev, site, spin, colour); });
 }
 int qwsg_spinor_alloc(qwsg_lattice* h, int prec, void** dev) {
   return guarded([&] { *dev = h->lat->alloc_spinor(prec); });

@@ -18,9 +18,10 @@ k_hop(Geo g, const int* __restrict__ list, int wg_per_blk, typename Store<S>::R 
       const S* b, const S* c, const S* __restrict__ in, const S* __restrict__ U, const S* __restrict__ C,
       int skip_interior) {
   using R = typename Store<S>::R;
-  const int bi = blockIdx.x / wg_per_blk;
+  const int wg = xcd_remap(blockIdx.x, gridDim.x);
+  const int bi = wg / wg_per_blk;
   const int blk = list ? list[bi] : bi;
-  const int loc = (blockIdx.x - bi * wg_per_blk) * HOP_THREADS + threadIdx.x;
+  const int loc = (wg - bi * wg_per_blk) * HOP_THREADS + threadIdx.x;
   if (loc >= g.Vb) return;
   const long long s = (long long)blk * g.Vb + loc;
   SiteC sc;

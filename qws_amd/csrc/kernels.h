@@ -76,5 +76,7 @@ void launch_import(const Geo& g, int kind, int layout, int vlen, int nl, const v
 template <typename S>
 void launch_export(const Geo& g, int kind, int layout, int vlen, const S* src, void* dst, int dst_is_double,
                    hipStream_t st);
+// FP64 links (kind 1) or clover (kind 2) to the plain host layout (dst in device memory)
+void launch_export_field(const Geo& g, int kind, const double* src, double* dst, hipStream_t st);
 
 }  // namespace qws

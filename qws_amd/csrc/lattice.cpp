@@ -3,6 +3,10 @@
 #include "lattice.h"
 
 #include <chrono>
+#include <type_traits>
+
+#include "fieldgen.h"
+#include "sap.h"
 #include <cmath>
 #include <cstddef>
 #include <cstring>
@@ -47,6 +51,7 @@ Lattice::Lattice(const int L[4], const int B[4], int tbc, int links32) {
   g_.nblk = nblk;
   g_.V = V;
   g_.tbc = tbc;
+  g_.hopmask = 0xFF;
   nl32_ = links32;
   QWS_HIP(hipStreamCreate(&st_));
   // block lists per colour (colour = (bx+by+bz+bt)%2, qws.cc:275-276)
@@ -147,6 +152,60 @@ void Lattice::load_clover(const void* C, int layout, int is_double, int target) 
   QWS_HIP(hipStreamSynchronize(st_));
 }
 
+void Lattice::generate_gauge(double eps, unsigned long long seed) {
+  if (!Ud_) Ud_ = dalloc<double>((size_t)4 * 18 * g_.V);
+  if (!Us_) Us_ = dalloc<float>((size_t)4 * nl32_ * g_.V);
+  launch_gen_links(g_, eps, (uint64_t)seed, Ud_, st_);
+  launch_links_d2s(g_, nl32_, Ud_, Us_, st_);
+  half_valid_ = false;
+  QWS_HIP(hipStreamSynchronize(st_));
+}
+
+void Lattice::compute_clover(double csw) {
+  if (!Ud_) throw std::runtime_error("compute_clover: no FP64 links loaded");
+  if (!Cd_) Cd_ = dalloc<double>((size_t)72 * g_.V);
+  if (!Cs_) Cs_ = dalloc<float>((size_t)72 * g_.V);
+  launch_clover(g_, Ud_, kappa_ * csw, sigma_chiral_blocks(), Cd_, st_);
+  launch_clover_d2s(g_, Cd_, Cs_, st_);
+  half_valid_ = false;
+  QWS_HIP(hipStreamSynchronize(st_));
+}
+
+void Lattice::export_field(int kind, double* host) {
+  const double* src = kind == 1 ? Ud_ : Cd_;
+  if (!src) throw std::runtime_error("export_field: field not present");
+  const size_t n = (size_t)g_.V * (kind == 1 ? 72 : 72);
+  double* stage = (double*)work("stage", n * sizeof(double));
+  launch_export_field(g_, kind, src, stage, st_);
+  QWS_HIP(hipMemcpyAsync(host, stage, n * sizeof(double), hipMemcpyDeviceToHost, st_));
+  QWS_HIP(hipStreamSynchronize(st_));
+}
+
+void Lattice::gaussian_spinor(int prec, void* dev, unsigned long long seed) {
+  if (prec == P64) {
+    launch_gaussian_spinor(g_, (uint64_t)seed, (double*)dev, st_);
+  } else {
+    double* tmp = (double*)work("gen_tmp", spinor_bytes(P64));
+    launch_gaussian_spinor(g_, (uint64_t)seed, tmp, st_);
+    if (prec == P32) launch_convert<float, double>(g_.V, (float*)dev, tmp, st_);
+    else throw std::invalid_argument("gaussian_spinor: FP64 or FP32");
+  }
+}
+
+void Lattice::point_spinor(int prec, void* dev, const int x[4], int spin, int colour) {
+  for (int m = 0; m < 4; ++m)
+    if (x[m] < 0 || x[m] >= g_.L[m]) throw std::invalid_argument("point_spinor: site outside the lattice");
+  if (spin < 0 || spin > 3 || colour < 0 || colour > 2) throw std::invalid_argument("point_spinor: spin/colour");
+  if (prec == P64) {
+    launch_point_spinor(g_, x, spin, colour, (double*)dev, st_);
+  } else {
+    double* tmp = (double*)work("gen_tmp", spinor_bytes(P64));
+    launch_point_spinor(g_, x, spin, colour, tmp, st_);
+    if (prec == P32) launch_convert<float, double>(g_.V, (float*)dev, tmp, st_);
+    else throw std::invalid_argument("point_spinor: FP64 or FP32");
+  }
+}
+
 void Lattice::ensure_half_fields() {
   if (half_valid_) return;
   throw std::runtime_error("FP16 field storage is not enabled in this build");
@@ -230,6 +289,7 @@ void Lattice::clover(int prec, void* out, const void* in) {
 template <typename S>
 void Lattice::jinv_t(int colour, S* x, const S* b, S* tmp, int nm) {
   using R = typename Store<S>::R;
+  if (nm < 1) nm = 1;  // the first application is unconditional (static_solver.cc:77-101)
   const S* prev = b;
   for (int k = 1; k <= nm; ++k) {
     S* dst = ((nm - k) % 2 == 0) ? x : tmp;
@@ -262,6 +322,25 @@ void Lattice::sap_sweeps(const S* in, S* s, S* q, S* tmp, int nsap, int nm) {
     }
 }
 
+// Fused path: one launch per half-sweep (sap.hip), for FP32 storage, proper
+// colouring and 256-site blocks.
+template <typename S> bool Lattice::fused_ok() const {
+  return std::is_same<S, float>::value && use_fused_ && !improper_ && sap_fusable(g_);
+}
+
+// the nsap sweeps of qws.cc:809-841 with the face update of each half-sweep
+// done at the start of the next one
+void Lattice::sap_sweeps_fused(const float* in, float* s, float* q, int nsap, int nm) {
+  const FieldSet<float> f = fields<float>();
+  for (int isap = 0; isap < nsap; ++isap)
+    for (int c = 0; c < 2; ++c) {
+      int flags = SAP_EPI_SUPDATE;
+      if (isap == 0) flags |= SAP_SRC_IN;
+      if (isap > 0 || c == 1) flags |= SAP_FACE_PRO;
+      launch_sap_block(g_, sel(c), f.nl, (float)kappa_, nm, flags, s, in, q, q, nullptr, f.U, f.C, st_);
+    }
+}
+
 // prec_ddd_s_ (qws.cc:776-926): out = D M_SAP in, with the final sequence of the
 // improper (npe[mu]==1) and of the proper colouring.
 template <typename S>
@@ -272,6 +351,21 @@ void Lattice::prec_ddd_t(S* out, const S* in, int nsap, int nm) {
   S* q = (S*)work("sap_q", spinor_bytes(pr));
   S* tmp = (S*)work("sap_tmp", spinor_bytes(pr));
   const R k = (R)kappa_;
+  if constexpr (std::is_same<S, float>::value) {
+    if (fused_ok<S>()) {
+      const FieldSet<float> f = fields<float>();
+      const int src = nsap == 0 ? SAP_SRC_IN : 0;
+      sap_sweeps_fused(in, s, q, nsap, nm);
+      // final e: q_e = A_e s_e, out_e = D_in q_e
+      launch_sap_block(g_, sel(0), f.nl, (float)kappa_, nm, SAP_EPI_OUT | src | (nsap > 0 ? SAP_FACE_PRO : 0), s, in,
+                       q, q, out, f.U, f.C, st_);
+      // final o: s_o += kappa T^-1 H_face q, q_o = A_o s_o, out_o = D_oe q_e + D_in q_o
+      launch_sap_block(g_, sel(1), f.nl, (float)kappa_, nm, SAP_EPI_OUT_FACE | src | SAP_FACE_PRO, s, in, q, q, out,
+                       f.U, f.C, st_);
+      hop<S>(sel(0), HOPS_FACE, -k, out, out, nullptr, nullptr, q);  // out_e += D_eo q_o
+      return;
+    }
+  }
   sap_sweeps<S>(in, s, q, tmp, nsap, nm);
   if (improper_) launch_zero_blocks<S>(g_, sel(1), q, st_);
   jinv_t<S>(0, q, s, tmp, nm);
@@ -299,6 +393,18 @@ void Lattice::prec_s_t(S* out, const S* in, int nsap, int nm) {
   S* s = (S*)work("sap_s", spinor_bytes(pr));
   S* q = (S*)work("sap_q", spinor_bytes(pr));
   S* tmp = (S*)work("sap_tmp", spinor_bytes(pr));
+  if constexpr (std::is_same<S, float>::value) {
+    if (fused_ok<S>()) {
+      const FieldSet<float> f = fields<float>();
+      const int src = nsap == 0 ? SAP_SRC_IN : 0;
+      sap_sweeps_fused(in, s, q, nsap, nm);
+      launch_sap_block(g_, sel(0), f.nl, (float)kappa_, nm, SAP_EPI_NONE | src | (nsap > 0 ? SAP_FACE_PRO : 0), s, in,
+                       q, out, nullptr, f.U, f.C, st_);
+      launch_sap_block(g_, sel(1), f.nl, (float)kappa_, nm, SAP_EPI_NONE | src | SAP_FACE_PRO, s, in, out, out,
+                       nullptr, f.U, f.C, st_);
+      return;
+    }
+  }
   sap_sweeps<S>(in, s, q, tmp, nsap, nm);
   if (improper_) launch_zero_blocks<S>(g_, sel(1), out, st_);
   jinv_t<S>(0, out, s, tmp, nm);

@@ -56,11 +56,19 @@ class Lattice {
   void set_kappa(double k) { kappa_ = k; }
   hipStream_t stream() const { return st_; }
   int links32() const { return nl32_; }
+  // fused SAP half-sweep kernels (default on where applicable)
+  void set_fused_sap(bool on) { use_fused_ = on; }
 
   // fields from host arrays (layout 0 plain, 1 QWS AoSoA)
   void load_gauge(const void* U, int layout, int is_double, int target);  // target: bit0 FP64, bit1 FP32
   void load_clover(const void* C, int layout, int is_double, int target);
   void ensure_half_fields();
+  // generated inputs: SU(3) links exp(i eps H), clover inverse from the links
+  void generate_gauge(double eps, unsigned long long seed);
+  void compute_clover(double csw);
+  void export_field(int kind, double* host_plain);  // kind 1 links, 2 clover (FP64)
+  void gaussian_spinor(int prec, void* dev, unsigned long long seed);
+  void point_spinor(int prec, void* dev, const int x[4], int spin, int colour);
 
   // spinor I/O between host arrays and device buffers
   void* alloc_spinor(int prec);
@@ -92,6 +100,8 @@ class Lattice {
            const S* in);
   template <typename S> void jinv_t(int colour, S* x, const S* b, S* tmp, int nm);
   template <typename S> void sap_sweeps(const S* in, S* s, S* q, S* tmp, int nsap, int nm);
+  template <typename S> bool fused_ok() const;
+  void sap_sweeps_fused(const float* in, float* s, float* q, int nsap, int nm);
   template <typename S> void prec_ddd_t(S* out, const S* in, int nsap, int nm);
   template <typename S> void prec_s_t(S* out, const S* in, int nsap, int nm);
   template <typename S> int bicg_t(S* x, const S* b, double tol, int maxiter, int nsap, int nm);
@@ -111,6 +121,7 @@ class Lattice {
   float *Us_ = nullptr, *Cs_ = nullptr;
   __half *Uh_ = nullptr, *Ch_ = nullptr;
   bool half_valid_ = false;
+  bool use_fused_ = true;
   // reductions
   double* part_ = nullptr;
   Scalars* sc_in_ = nullptr;

@@ -131,3 +131,26 @@ template void launch_export<double>(const Geo&, int, int, int, const double*, vo
 template void launch_export<__half>(const Geo&, int, int, int, const __half*, void*, int, hipStream_t);
 
 }  // namespace qws
+
+namespace qws {
+
+// FP64 links (kind 1, 18 reals) or clover (kind 2) -> plain host layout (staged)
+__global__ void k_export_field(Geo g, HostMap h, int kind, const double* src, double* dst) {
+  const long long s = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (s >= g.V) return;
+  int x[4];
+  global_coords(g, s, x);
+  const int nmu = kind == 1 ? 4 : 1;
+  const int nrec = kind == 1 ? 18 : 72;
+  for (int mu = 0; mu < nmu; ++mu)
+    for (int k = 0; k < nrec; ++k)
+      dst[host_offset(g, h, x, kind, mu, k)] = src[dev_offset<double>(g, kind, 18, s, mu, k)];
+}
+
+void launch_export_field(const Geo& g, int kind, const double* src, double* dst, hipStream_t st) {
+  HostMap h = make_map(g, 0, 1);
+  const int T = 128;
+  hipLaunchKernelGGL(k_export_field, dim3((unsigned)((g.V + T - 1) / T)), dim3(T), 0, st, g, h, kind, src, dst);
+}
+
+}  // namespace qws

@@ -36,7 +36,20 @@ struct Geo {
   int nblk;       // number of blocks
   long long V;    // sites
   int tbc;        // +1 periodic / -1 antiperiodic in t
+  int hopmask;    // bit 2*mu + (dir < 0): hop taken by the full operator (0xFF); being a
+                  // runtime value it keeps each hop in its own basic block, so the
+                  // compiler cannot hoist all eight neighbour loads (256 -> ~146 VGPRs)
 };
+
+// MI355X dispatches the workgroups of a launch round-robin over its 8 XCDs, each
+// with its own L2.  Give XCD x a contiguous range of logical workgroup ids (blocks
+// that share faces then share an L2): a bijection of [0, nwg).
+constexpr int NUM_XCD = 8;
+__device__ __forceinline__ int xcd_remap(int wg, int nwg) {
+  const int q = nwg / NUM_XCD, r = nwg % NUM_XCD;
+  const int x = wg % NUM_XCD, i = wg / NUM_XCD;
+  return x * q + (x < r ? x : r) + i;
+}
 
 struct SiteC {
   int b[4];  // block coordinates
@@ -356,6 +369,7 @@ __device__ __forceinline__ void one_hop(const Geo& g, long long s, const SiteC& 
   bool inside;
   long long n = neighbour(g, s, c, MU, DIR, inside);
   if (!take_hop<SET>(inside)) return;
+  if (SET == HOPS_ALL && !(g.hopmask & (1 << (2 * MU + (DIR > 0 ? 0 : 1))))) return;
   Cx<R> v[12], a[6], b[6], u[9];
   SpinorIO<S>::load(in, n, g.V, v);
   project<R, MU, DIR>(v, a);

@@ -1,0 +1,24 @@
+// sap.h -- fused SAP half-sweep kernel (one workgroup per block of one colour).
+#pragma once
+
+#include "kernels.h"
+
+namespace qws {
+
+constexpr int SAP_THREADS = 256;  // sites per block handled by the fused kernel (e.g. 4^4)
+
+enum SapFlags {
+  SAP_EPI_SUPDATE = 0,   // s_c = s_c + in_c - x + kappa T^-1 H_in x ; q_c = x
+  SAP_EPI_OUT = 1,       // out_c = x - kappa T^-1 H_in x            ; q_c = x
+  SAP_EPI_OUT_FACE = 2,  // out_c = -kappa T^-1 H_face q + x - kappa T^-1 H_in x ; q_c = x
+  SAP_EPI_NONE = 3,      // q_c = x only
+  SAP_EPI_MASK = 3,
+  SAP_FACE_PRO = 4,      // prologue: s_c += kappa T^-1 H_face qface
+  SAP_SRC_IN = 8,        // take s_c from `in` (first sweep: s = in)
+};
+
+bool sap_fusable(const Geo& g);
+void launch_sap_block(const Geo& g, BlockSel sel, int nl, float kappa, int nm, int flags, float* s, const float* in,
+                      const float* qface, float* qout, float* out, const float* U, const float* C, hipStream_t st);
+
+}  // namespace qws

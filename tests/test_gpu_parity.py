@@ -129,6 +129,29 @@ def test_prec_ddd_and_prec_s(prob, nsap, nm):
     assert rel(out.download(), p.orc.prec_s(bs, nsap, nm)) < 5e-6
 
 
+@pytest.mark.parametrize("L,B", [([16, 8, 8, 16], [4, 4, 4, 4]), ([16, 8, 8, 8], [8, 4, 4, 2])])
+@pytest.mark.parametrize("nsap,nm", [(4, 2), (1, 3), (0, 2)])
+def test_fused_sap_matches_unfused_and_oracle(L, B, nsap, nm):
+    """The fused half-sweep kernel (256-site blocks, proper colouring) gives the
+    same prec_ddd_s_/prec_s_ as the step-by-step kernels and the oracle."""
+    p = Problem(L, B, kappa=0.125, seed=8)
+    bs = p.b.astype(np.float32)
+    inp = p.lat.spinor(Q.F32, bs)
+    res = {}
+    for fused in (True, False):
+        p.lat.set_sap_fused(fused)
+        o1, o2 = p.lat.spinor(Q.F32), p.lat.spinor(Q.F32)
+        p.lat.prec_ddd(o1, inp, nsap, nm)
+        p.lat.prec_s(o2, inp, nsap, nm)
+        res[fused] = (o1.download(), o2.download())
+    p.lat.set_sap_fused(True)
+    want_ddd, want_s = p.orc.prec_ddd_s(bs, nsap, nm), p.orc.prec_s(bs, nsap, nm)
+    for k in (True, False):
+        assert rel(res[k][0], want_ddd) < 5e-6
+        assert rel(res[k][1], want_s) < 5e-6
+    assert rel(res[True][0], res[False][0]) < 5e-6
+
+
 def test_inner_bicgstab(prob):
     p, lat = prob, prob.lat
     bs = (p.b / np.sqrt(O.norm2(p.b))).astype(np.float32)
