@@ -188,6 +188,14 @@ int qwsg_bicgstab_precdd(qwsg_lattice* lat, int prec, void* x, const void* b, do
                          int nm, int* iters);
 /* FP64 outer solve; x (FP64) holds the initial guess */
 int qwsg_solve(qwsg_lattice* lat, void* x, const void* b, const qwsg_solve_params* p, qwsg_solve_stats* st);
+/* No counterpart in the reference (it solves one source per call): n independent
+ * right-hand sides (1..64, FP64 device spinors; x[k] holds the initial guess)
+ * solved together.  Every kernel runs over all unconverged right-hand sides, so
+ * the links and clover are read once per batch; each right-hand side keeps its own
+ * recurrence and its own deterministic reductions, so its result is bit-identical
+ * to solving it alone.  st receives n entries. */
+int qwsg_solve_batch(qwsg_lattice* lat, int n, void* const* x, const void* const* b, const qwsg_solve_params* p,
+                     qwsg_solve_stats* st);
 int qwsg_norm2(qwsg_lattice* lat, int prec, const void* a, double* out);
 int qwsg_synchronize(qwsg_lattice* lat);
 /* average device time (ms) of `reps` full FP32/FP16/FP64 dslash launches on the

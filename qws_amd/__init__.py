@@ -67,6 +67,7 @@ def lib():
                                            ctypes.c_int, ip]
         L.qwsg_solve.argtypes = [vp, vp, vp, ctypes.POINTER(SolveParams), ctypes.POINTER(SolveStats)]
         L.qwsg_norm2.argtypes = [vp, ctypes.c_int, vp, ctypes.POINTER(ctypes.c_double)]
+        L.qwsg_solve_batch.argtypes = [vp, ctypes.c_int, vp, vp, ctypes.POINTER(SolveParams), ctypes.POINTER(SolveStats)]
         L.qwsg_synchronize.argtypes = [vp]
         L.qwsg_time_dslash.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
         L.qwsg_device_count.argtypes = [ip]
@@ -248,6 +249,17 @@ class Lattice:
         st = SolveStats()
         check(lib().qwsg_solve(self.h, x.ptr, b.ptr, ctypes.byref(p), ctypes.byref(st)))
         return {k: getattr(st, k) for k, _ in SolveStats._fields_}
+
+    def solve_batch(self, xs, bs, nsap=4, nm=2, tol=-1.0, maxiter=6, tol_s=-1.0, maxiter_s=50, true_tol=-1.0,
+                    max_restarts=0, sap_precision=F32):
+        n = len(xs)
+        assert len(bs) == n
+        p = SolveParams(nsap, nm, tol, maxiter, tol_s, maxiter_s, true_tol, max_restarts, sap_precision)
+        st = (SolveStats * n)()
+        xp = (ctypes.c_void_p * n)(*[x.ptr.value for x in xs])
+        bp = (ctypes.c_void_p * n)(*[b.ptr.value for b in bs])
+        check(lib().qwsg_solve_batch(self.h, n, xp, bp, ctypes.byref(p), st))
+        return [{k: getattr(s, k) for k, _ in SolveStats._fields_} for s in st]
 
     def sync(self):
         check(lib().qwsg_synchronize(self.h))

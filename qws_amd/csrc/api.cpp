@@ -164,6 +164,31 @@ int qwsg_solve(qwsg_lattice* h, void* x, const void* b, const qwsg_solve_params*
     }
   });
 }
+int qwsg_solve_batch(qwsg_lattice* h, int n, void* const* x, const void* const* b, const qwsg_solve_params* p,
+                     qwsg_solve_stats* st) {
+  return guarded([&] {
+    if (n < 1 || n > qws::MAX_BATCH) throw std::invalid_argument("batch size must be in 1..64");
+    qws::SolveParams sp;
+    sp.nsap = p->nsap;
+    sp.nm = p->nm;
+    sp.tol = p->tol;
+    sp.maxiter = p->maxiter;
+    sp.tol_s = p->tol_s;
+    sp.maxiter_s = p->maxiter_s;
+    sp.true_tol = p->true_tol;
+    sp.max_restarts = p->max_restarts;
+    sp.sap_prec = p->sap_precision;
+    std::vector<qws::SolveStats> s =
+        h->lat->solve_batch((double* const*)x, (const double* const*)b, n, sp);
+    for (int k = 0; k < n && st; ++k) {
+      st[k].outer_iters = s[k].outer_iters;
+      st[k].inner_iters = s[k].inner_iters;
+      st[k].restarts = s[k].restarts;
+      st[k].true_residual = s[k].true_residual;
+      st[k].seconds = s[k].seconds;
+    }
+  });
+}
 int qwsg_norm2(qwsg_lattice* h, int prec, const void* a, double* out) {
   return guarded([&] { *out = h->lat->norm2(prec, a); });
 }

@@ -1,12 +1,13 @@
-// blas.hip -- fused BiCGStab vector updates and reductions.
+// blas.hip -- fused BiCGStab vector updates and reductions, batched over RHS.
 //
-// Vectors are flat arrays of interleaved (re,im) pairs; the same kernels serve
-// the FP32 inner solver (bicgstab_precdd_s.cc:85-559) and the FP64 outer solver
-// (bicgstab_dd_mix.cc:105-388).  Every reduction accumulates in FP64 and writes
-// one partial per workgroup; launch_finalize sums the partials in a fixed order
-// (bit-reproducible) and updates the solver scalars in device memory, so a
-// BiCGStab iteration needs no host round trip unless the caller asks for the
-// residual.
+// Vectors are flat arrays of interleaved (re,im) pairs; a batch of nrhs vectors
+// sits at base + r * stride.  The same kernels serve the FP32 inner solver
+// (bicgstab_precdd_s.cc:85-559) and the FP64 outer solver (bicgstab_dd_mix.cc:105-388).
+// blockIdx.y selects the right-hand side; RHS whose active flag is 0 are skipped.
+// Every reduction accumulates in FP64 and writes one partial per workgroup;
+// k_finalize sums the partials in a fixed order (bit-reproducible, and independent
+// of what else is in the batch) and updates that RHS's scalars in device memory,
+// so an iteration needs no host round trip unless the caller asks for residuals.
 #include <math.h>
 
 #include "kernels.h"
@@ -17,7 +18,9 @@ template <typename R> struct V2;
 template <> struct V2<float> { using T = float2; };
 template <> struct V2<double> { using T = double2; };
 
-// Block-wide sum of NV doubles; thread 0 writes part[v * RED_BLOCKS + blockIdx.x].
+__device__ __forceinline__ bool rhs_off(const int* active) { return active && !active[blockIdx.y]; }
+
+// Block-wide sum of NV doubles; thread 0 writes part[(r*4 + v)*RED_BLOCKS + blockIdx.x].
 template <int NV>
 __device__ __forceinline__ void block_reduce_store(double (&v)[NV], double* part) {
   __shared__ double sh[NV][RED_THREADS / 64];
@@ -35,42 +38,52 @@ __device__ __forceinline__ void block_reduce_store(double (&v)[NV], double* part
     for (int k = 0; k < NV; ++k) {
       double x = 0;
       for (int w = 0; w < RED_THREADS / 64; ++w) x += sh[k][w];
-      part[k * RED_BLOCKS + blockIdx.x] = x;
+      part[((long long)blockIdx.y * 4 + k) * RED_BLOCKS + blockIdx.x] = x;
     }
   }
 }
 
-#define GRID_STRIDE(i, n) for (long long i = blockIdx.x * (long long)RED_THREADS + threadIdx.x; i < (n); i += (long long)RED_BLOCKS * RED_THREADS)
+#define GRID_STRIDE(i, n) \
+  for (long long i = blockIdx.x * (long long)RED_THREADS + threadIdx.x; i < (n); i += (long long)RED_BLOCKS * RED_THREADS)
+#define RHS(ptr, T) (reinterpret_cast<T*>(ptr) + (long long)blockIdx.y * (stride / 2))
 
 template <typename R>
-__global__ void __launch_bounds__(RED_THREADS) k_init_inner(long long n, R* x, R* r, R* p, R* r0, const R* b, double* part) {
+__global__ void __launch_bounds__(RED_THREADS)
+k_init_inner(long long n, long long stride, const int* active, R* x, R* r, R* p, R* r0, const R* b, double* part) {
   using T = typename V2<R>::T;
+  if (rhs_off(active)) return;
+  T *X = RHS(x, T), *Rr = RHS(r, T), *P = RHS(p, T), *R0 = RHS(r0, T);
+  const T* B = RHS(b, const T);
   double acc[1] = {0};
   GRID_STRIDE(i, n) {
-    T bv = reinterpret_cast<const T*>(b)[i];
-    T z; z.x = 0; z.y = 0;
-    reinterpret_cast<T*>(x)[i] = z;
-    reinterpret_cast<T*>(r)[i] = bv;
-    reinterpret_cast<T*>(p)[i] = bv;
-    reinterpret_cast<T*>(r0)[i] = bv;
+    T bv = B[i];
+    T z;
+    z.x = 0;
+    z.y = 0;
+    X[i] = z;
+    Rr[i] = bv;
+    P[i] = bv;
+    R0[i] = bv;
     acc[0] += (double)bv.x * bv.x + (double)bv.y * bv.y;
   }
   block_reduce_store<1>(acc, part);
 }
 
 template <typename R>
-__global__ void __launch_bounds__(RED_THREADS) k_init_outer(long long n, R* r, R* p, R* r0, const R* b, const R* q, double* part) {
+__global__ void __launch_bounds__(RED_THREADS)
+k_init_outer(long long n, long long stride, const int* active, R* r, R* p, R* r0, const R* b, const R* q, double* part) {
   using T = typename V2<R>::T;
+  if (rhs_off(active)) return;
+  T *Rr = RHS(r, T), *P = RHS(p, T), *R0 = RHS(r0, T);
+  const T *B = RHS(b, const T), *Qv = RHS(q, const T);
   double acc[2] = {0, 0};
   GRID_STRIDE(i, n) {
-    T bv = reinterpret_cast<const T*>(b)[i];
-    T qv = reinterpret_cast<const T*>(q)[i];
-    T rv;
+    T bv = B[i], qv = Qv[i], rv;
     rv.x = bv.x - qv.x;
     rv.y = bv.y - qv.y;
-    reinterpret_cast<T*>(r)[i] = rv;
-    reinterpret_cast<T*>(p)[i] = rv;
-    reinterpret_cast<T*>(r0)[i] = rv;
+    Rr[i] = rv;
+    P[i] = rv;
+    R0[i] = rv;
     acc[0] += (double)bv.x * bv.x + (double)bv.y * bv.y;
     acc[1] += (double)rv.x * rv.x + (double)rv.y * rv.y;
   }
@@ -79,12 +92,14 @@ __global__ void __launch_bounds__(RED_THREADS) k_init_outer(long long n, R* r, R
 
 // <a,b> = sum conj(a) b  (bicgstab_dd_mix.cc:180-181)
 template <typename R>
-__global__ void __launch_bounds__(RED_THREADS) k_dot(long long n, const R* a, const R* b, double* part) {
+__global__ void __launch_bounds__(RED_THREADS)
+k_dot(long long n, long long stride, const int* active, const R* a, const R* b, double* part) {
   using T = typename V2<R>::T;
+  if (rhs_off(active)) return;
+  const T *A = RHS(a, const T), *B = RHS(b, const T);
   double acc[2] = {0, 0};
   GRID_STRIDE(i, n) {
-    T av = reinterpret_cast<const T*>(a)[i];
-    T bv = reinterpret_cast<const T*>(b)[i];
+    T av = A[i], bv = B[i];
     acc[0] += (double)av.x * bv.x + (double)av.y * bv.y;
     acc[1] += (double)av.x * bv.y - (double)av.y * bv.x;
   }
@@ -92,12 +107,14 @@ __global__ void __launch_bounds__(RED_THREADS) k_dot(long long n, const R* a, co
 }
 
 template <typename R>
-__global__ void __launch_bounds__(RED_THREADS) k_norm_dot(long long n, const R* t, const R* r, double* part) {
+__global__ void __launch_bounds__(RED_THREADS)
+k_norm_dot(long long n, long long stride, const int* active, const R* t, const R* r, double* part) {
   using T = typename V2<R>::T;
+  if (rhs_off(active)) return;
+  const T *Tt = RHS(t, const T), *Rr = RHS(r, const T);
   double acc[3] = {0, 0, 0};
   GRID_STRIDE(i, n) {
-    T tv = reinterpret_cast<const T*>(t)[i];
-    T rv = reinterpret_cast<const T*>(r)[i];
+    T tv = Tt[i], rv = Rr[i];
     acc[0] += (double)tv.x * tv.x + (double)tv.y * tv.y;
     acc[1] += (double)tv.x * rv.x + (double)tv.y * rv.y;
     acc[2] += (double)tv.x * rv.y - (double)tv.y * rv.x;
@@ -106,39 +123,46 @@ __global__ void __launch_bounds__(RED_THREADS) k_norm_dot(long long n, const R* 
 }
 
 template <typename R>
-__global__ void __launch_bounds__(RED_THREADS) k_norm(long long n, const R* a, double* part) {
+__global__ void __launch_bounds__(RED_THREADS)
+k_norm(long long n, long long stride, const int* active, const R* a, double* part) {
   using T = typename V2<R>::T;
+  if (rhs_off(active)) return;
+  const T* A = RHS(a, const T);
   double acc[1] = {0};
   GRID_STRIDE(i, n) {
-    T av = reinterpret_cast<const T*>(a)[i];
+    T av = A[i];
     acc[0] += (double)av.x * av.x + (double)av.y * av.y;
   }
   block_reduce_store<1>(acc, part);
 }
 
 // x += c v1 ; r -= c v2 ; |r|^2 [, <r0,r>]   (bicgstab_precdd_s.cc:281-295,427-440)
+// c = alpha (coef 0) or omega (coef 1) of this RHS.
 template <typename R, bool WITH_R0>
 __global__ void __launch_bounds__(RED_THREADS)
-k_update_xr(long long n, R* x, const R* v1, R* r, const R* v2, const double* c, const R* r0, double* part) {
+k_update_xr(long long n, long long stride, const int* active, R* x, const R* v1, R* r, const R* v2, const Scalars* sc,
+            int coef, const R* r0, double* part) {
   using T = typename V2<R>::T;
+  if (rhs_off(active)) return;
+  const Scalars& S = sc[blockIdx.y];
+  const double* c = coef == 0 ? S.alpha : S.omega;
   const R cr = (R)c[0], ci = (R)c[1];
+  T *X = RHS(x, T), *Rr = RHS(r, T);
+  const T *A = RHS(v1, const T), *Qv = RHS(v2, const T), *Z = WITH_R0 ? RHS(r0, const T) : nullptr;
   double acc[WITH_R0 ? 3 : 1];
 #pragma unroll
   for (int k = 0; k < (WITH_R0 ? 3 : 1); ++k) acc[k] = 0;
   GRID_STRIDE(i, n) {
-    T xv = reinterpret_cast<T*>(x)[i];
-    T a = reinterpret_cast<const T*>(v1)[i];
-    T rv = reinterpret_cast<T*>(r)[i];
-    T q = reinterpret_cast<const T*>(v2)[i];
+    T xv = X[i], a = A[i], rv = Rr[i], q = Qv[i];
     xv.x = xv.x + cr * a.x - ci * a.y;
     xv.y = xv.y + cr * a.y + ci * a.x;
     rv.x = rv.x - cr * q.x + ci * q.y;
     rv.y = rv.y - cr * q.y - ci * q.x;
-    reinterpret_cast<T*>(x)[i] = xv;
-    reinterpret_cast<T*>(r)[i] = rv;
+    X[i] = xv;
+    Rr[i] = rv;
     acc[0] += (double)rv.x * rv.x + (double)rv.y * rv.y;
     if (WITH_R0) {
-      T z = reinterpret_cast<const T*>(r0)[i];
+      T z = Z[i];
       acc[1] += (double)z.x * rv.x + (double)z.y * rv.y;
       acc[2] += (double)z.x * rv.y - (double)z.y * rv.x;
     }
@@ -148,23 +172,26 @@ k_update_xr(long long n, R* x, const R* v1, R* r, const R* v2, const double* c, 
 
 // p = r + beta (p - omega q)   (bicgstab_precdd_s.cc:529-543)
 template <typename R>
-__global__ void __launch_bounds__(RED_THREADS) k_update_p(long long n, R* p, const R* r, const R* q, const Scalars* sc) {
+__global__ void __launch_bounds__(RED_THREADS)
+k_update_p(long long n, long long stride, const int* active, R* p, const R* r, const R* q, const Scalars* sc) {
   using T = typename V2<R>::T;
-  const R ar = (R)sc->omega[0], ai = (R)sc->omega[1], br = (R)sc->beta[0], bi = (R)sc->beta[1];
+  if (rhs_off(active)) return;
+  const Scalars& S = sc[blockIdx.y];
+  const R ar = (R)S.omega[0], ai = (R)S.omega[1], br = (R)S.beta[0], bi = (R)S.beta[1];
+  T* P = RHS(p, T);
+  const T *Qv = RHS(q, const T), *Rr = RHS(r, const T);
   GRID_STRIDE(i, n) {
-    T pv = reinterpret_cast<T*>(p)[i];
-    T qv = reinterpret_cast<const T*>(q)[i];
-    T rv = reinterpret_cast<const T*>(r)[i];
+    T pv = P[i], qv = Qv[i], rv = Rr[i];
     R cr = pv.x - ar * qv.x + ai * qv.y;
     R ci = pv.y - ar * qv.y - ai * qv.x;
     pv.x = br * cr - bi * ci + rv.x;
     pv.y = br * ci + bi * cr + rv.y;
-    reinterpret_cast<T*>(p)[i] = pv;
+    P[i] = pv;
   }
 }
 
 __device__ __forceinline__ void cdiv(double ar, double ai, double br, double bi, double* o) {
-  // (a / b) with scaling (std::complex semantics up to rounding)
+  // a / b with scaling (std::complex semantics up to rounding)
   if (fabs(br) >= fabs(bi)) {
     double t = bi / br, d = br + bi * t;
     o[0] = (ar + ai * t) / d;
@@ -176,13 +203,17 @@ __device__ __forceinline__ void cdiv(double ar, double ai, double br, double bi,
   }
 }
 
-// Deterministic sum of the partials, then the scalar recurrences of BiCGStab
-// with the reference's breakdown guards (bicgstab_dd_mix.cc:192,279,359).
-__global__ void __launch_bounds__(RED_THREADS) k_finalize(const double* part, int nval, Scalars* sc, int mode) {
+// Deterministic sum of the partials of RHS blockIdx.x, then the scalar
+// recurrences of BiCGStab with the reference's breakdown guards
+// (bicgstab_dd_mix.cc:192,279,359).
+__global__ void __launch_bounds__(RED_THREADS) k_finalize(const double* part, int nval, Scalars* scs, int mode,
+                                                          const int* active) {
+  const int r = blockIdx.x;
+  if (active && !active[r]) return;
   __shared__ double sh[4][RED_THREADS];
   for (int v = 0; v < nval; ++v) {
     double x = 0;
-    for (int i = threadIdx.x; i < RED_BLOCKS; i += RED_THREADS) x += part[v * RED_BLOCKS + i];
+    for (int i = threadIdx.x; i < RED_BLOCKS; i += RED_THREADS) x += part[((long long)r * 4 + v) * RED_BLOCKS + i];
     sh[v][threadIdx.x] = x;
   }
   __syncthreads();
@@ -192,6 +223,7 @@ __global__ void __launch_bounds__(RED_THREADS) k_finalize(const double* part, in
     __syncthreads();
   }
   if (threadIdx.x != 0) return;
+  Scalars* sc = scs + r;
   double red[4] = {0, 0, 0, 0};
   for (int v = 0; v < nval; ++v) red[v] = sh[v][0];
   for (int v = 0; v < 4; ++v) sc->red[v] = red[v];
@@ -248,31 +280,40 @@ __global__ void __launch_bounds__(RED_THREADS) k_finalize(const double* part, in
   }
 }
 
+// ---- per-site precision conversion (the two precisions have different layouts)
 template <typename S>
-__global__ void __launch_bounds__(RED_THREADS) k_d2s_scaled(long long V, S* out, const double* in, const Scalars* sc) {
+__global__ void __launch_bounds__(RED_THREADS)
+k_d2s_scaled(long long V, long long so, long long si, const int* active, S* out, const double* in, const Scalars* sc) {
   using R = typename Store<S>::R;
-  const double inv = 1.0 / sc->nrm;
+  if (rhs_off(active)) return;
+  const double inv = 1.0 / sc[blockIdx.y].nrm;
+  S* O = out + blockIdx.y * so;
+  const double* I = in + blockIdx.y * si;
   GRID_STRIDE(s, V) {
     Cx<double> v[12];
     Cx<R> w[12];
-    SpinorIO<double>::load(in, s, V, v);
+    SpinorIO<double>::load(I, s, V, v);
 #pragma unroll
     for (int k = 0; k < 12; ++k) w[k] = {(R)(v[k].re * inv), (R)(v[k].im * inv)};
-    SpinorIO<S>::store(out, s, V, w);
+    SpinorIO<S>::store(O, s, V, w);
   }
 }
 
 template <typename S>
-__global__ void __launch_bounds__(RED_THREADS) k_s2d_scaled(long long V, double* out, const S* in, const Scalars* sc) {
+__global__ void __launch_bounds__(RED_THREADS)
+k_s2d_scaled(long long V, long long so, long long si, const int* active, double* out, const S* in, const Scalars* sc) {
   using R = typename Store<S>::R;
-  const double nrm = sc->nrm;
+  if (rhs_off(active)) return;
+  const double nrm = sc[blockIdx.y].nrm;
+  double* O = out + blockIdx.y * so;
+  const S* I = in + blockIdx.y * si;
   GRID_STRIDE(s, V) {
     Cx<R> w[12];
     Cx<double> v[12];
-    SpinorIO<S>::load(in, s, V, w);
+    SpinorIO<S>::load(I, s, V, w);
 #pragma unroll
     for (int k = 0; k < 12; ++k) v[k] = {(double)w[k].re * nrm, (double)w[k].im * nrm};
-    SpinorIO<double>::store(out, s, V, v);
+    SpinorIO<double>::store(O, s, V, v);
   }
 }
 
@@ -296,67 +337,85 @@ __global__ void __launch_bounds__(RED_THREADS) k_sub(long long n, R* out, const 
 }
 
 // ------------------------------------------------------------------ launchers
-#define LAUNCH(kern, ...) hipLaunchKernelGGL(kern, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, st, __VA_ARGS__)
+#define LAUNCH_B(kern, nb, ...) \
+  hipLaunchKernelGGL(kern, dim3(RED_BLOCKS, (nb)), dim3(RED_THREADS), 0, st, __VA_ARGS__)
 
-template <typename R> void launch_bicg_init_inner(long long n, R* x, R* r, R* p, R* r0, const R* b, double* part, hipStream_t st) {
-  LAUNCH((k_init_inner<R>), n, x, r, p, r0, b, part);
-}
-template <typename R> void launch_bicg_init_outer(long long n, R* r, R* p, R* r0, const R* b, const R* q, double* part, hipStream_t st) {
-  LAUNCH((k_init_outer<R>), n, r, p, r0, b, q, part);
-}
-template <typename R> void launch_dot(long long n, const R* a, const R* b, double* part, hipStream_t st) {
-  LAUNCH((k_dot<R>), n, a, b, part);
-}
-template <typename R> void launch_norm_dot(long long n, const R* t, const R* r, double* part, hipStream_t st) {
-  LAUNCH((k_norm_dot<R>), n, t, r, part);
+template <typename R>
+void launch_bicg_init_inner(const Batch& B, long long n, R* x, R* r, R* p, R* r0, const R* b, double* part,
+                            hipStream_t st) {
+  LAUNCH_B((k_init_inner<R>), B.n, n, B.stride, B.active, x, r, p, r0, b, part);
 }
 template <typename R>
-void launch_update_xr(long long n, R* x, const R* v1, R* r, const R* v2, const double* c, const R* r0, double* part,
-                      hipStream_t st) {
-  if (r0) LAUNCH((k_update_xr<R, true>), n, x, v1, r, v2, c, r0, part);
-  else LAUNCH((k_update_xr<R, false>), n, x, v1, r, v2, c, r0, part);
+void launch_bicg_init_outer(const Batch& B, long long n, R* r, R* p, R* r0, const R* b, const R* q, double* part,
+                            hipStream_t st) {
+  LAUNCH_B((k_init_outer<R>), B.n, n, B.stride, B.active, r, p, r0, b, q, part);
 }
-template <typename R> void launch_update_p(long long n, R* p, const R* r, const R* q, const Scalars* sc, hipStream_t st) {
-  LAUNCH((k_update_p<R>), n, p, r, q, sc);
+template <typename R>
+void launch_dot(const Batch& B, long long n, const R* a, const R* b, double* part, hipStream_t st) {
+  LAUNCH_B((k_dot<R>), B.n, n, B.stride, B.active, a, b, part);
 }
-template <typename R> void launch_norm(long long n, const R* a, double* part, hipStream_t st) {
-  LAUNCH((k_norm<R>), n, a, part);
+template <typename R>
+void launch_norm_dot(const Batch& B, long long n, const R* t, const R* r, double* part, hipStream_t st) {
+  LAUNCH_B((k_norm_dot<R>), B.n, n, B.stride, B.active, t, r, part);
 }
-void launch_finalize(const double* part, int nval, Scalars* sc, int mode, hipStream_t st) {
-  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(RED_THREADS), 0, st, part, nval, sc, mode);
+template <typename R>
+void launch_update_xr(const Batch& B, long long n, R* x, const R* v1, R* r, const R* v2, const Scalars* sc, int coef,
+                      const R* r0, double* part, hipStream_t st) {
+  if (r0) LAUNCH_B((k_update_xr<R, true>), B.n, n, B.stride, B.active, x, v1, r, v2, sc, coef, r0, part);
+  else LAUNCH_B((k_update_xr<R, false>), B.n, n, B.stride, B.active, x, v1, r, v2, sc, coef, r0, part);
 }
-template <typename S> void launch_d2s_scaled(long long V, S* out, const double* in, const Scalars* sc, hipStream_t st) {
-  LAUNCH((k_d2s_scaled<S>), V, out, in, sc);
+template <typename R>
+void launch_update_p(const Batch& B, long long n, R* p, const R* r, const R* q, const Scalars* sc, hipStream_t st) {
+  LAUNCH_B((k_update_p<R>), B.n, n, B.stride, B.active, p, r, q, sc);
 }
-template <typename S> void launch_s2d_scaled(long long V, double* out, const S* in, const Scalars* sc, hipStream_t st) {
-  LAUNCH((k_s2d_scaled<S>), V, out, in, sc);
+template <typename R>
+void launch_norm(const Batch& B, long long n, const R* a, double* part, hipStream_t st) {
+  LAUNCH_B((k_norm<R>), B.n, n, B.stride, B.active, a, part);
+}
+void launch_finalize(const Batch& B, const double* part, int nval, Scalars* sc, int mode, hipStream_t st) {
+  hipLaunchKernelGGL(k_finalize, dim3(B.n), dim3(RED_THREADS), 0, st, part, nval, sc, mode, B.active);
+}
+template <typename S>
+void launch_d2s_scaled(const Batch& B, long long V, long long so, long long si, S* out, const double* in,
+                       const Scalars* sc, hipStream_t st) {
+  LAUNCH_B((k_d2s_scaled<S>), B.n, V, so, si, B.active, out, in, sc);
+}
+template <typename S>
+void launch_s2d_scaled(const Batch& B, long long V, long long so, long long si, double* out, const S* in,
+                       const Scalars* sc, hipStream_t st) {
+  LAUNCH_B((k_s2d_scaled<S>), B.n, V, so, si, B.active, out, in, sc);
 }
 template <typename SO, typename SI> void launch_convert(long long V, SO* out, const SI* in, hipStream_t st) {
-  LAUNCH((k_convert<SO, SI>), V, out, in);
+  LAUNCH_B((k_convert<SO, SI>), 1, V, out, in);
 }
-template void launch_d2s_scaled<float>(long long, float*, const double*, const Scalars*, hipStream_t);
-template void launch_d2s_scaled<__half>(long long, __half*, const double*, const Scalars*, hipStream_t);
-template void launch_s2d_scaled<float>(long long, double*, const float*, const Scalars*, hipStream_t);
-template void launch_s2d_scaled<__half>(long long, double*, const __half*, const Scalars*, hipStream_t);
+template <typename R> void launch_sub(long long n, R* out, const R* a, const R* b, hipStream_t st) {
+  LAUNCH_B((k_sub<R>), 1, n, out, a, b);
+}
+
+#define INST(R)                                                                                                       \
+  template void launch_bicg_init_inner<R>(const Batch&, long long, R*, R*, R*, R*, const R*, double*, hipStream_t);   \
+  template void launch_bicg_init_outer<R>(const Batch&, long long, R*, R*, R*, const R*, const R*, double*,           \
+                                          hipStream_t);                                                               \
+  template void launch_dot<R>(const Batch&, long long, const R*, const R*, double*, hipStream_t);                     \
+  template void launch_norm_dot<R>(const Batch&, long long, const R*, const R*, double*, hipStream_t);                \
+  template void launch_update_xr<R>(const Batch&, long long, R*, const R*, R*, const R*, const Scalars*, int,         \
+                                    const R*, double*, hipStream_t);                                                  \
+  template void launch_update_p<R>(const Batch&, long long, R*, const R*, const R*, const Scalars*, hipStream_t);     \
+  template void launch_norm<R>(const Batch&, long long, const R*, double*, hipStream_t);                              \
+  template void launch_sub<R>(long long, R*, const R*, const R*, hipStream_t);
+INST(float)
+INST(double)
+template void launch_d2s_scaled<float>(const Batch&, long long, long long, long long, float*, const double*,
+                                       const Scalars*, hipStream_t);
+template void launch_d2s_scaled<__half>(const Batch&, long long, long long, long long, __half*, const double*,
+                                        const Scalars*, hipStream_t);
+template void launch_s2d_scaled<float>(const Batch&, long long, long long, long long, double*, const float*,
+                                       const Scalars*, hipStream_t);
+template void launch_s2d_scaled<__half>(const Batch&, long long, long long, long long, double*, const __half*,
+                                        const Scalars*, hipStream_t);
 template void launch_convert<float, double>(long long, float*, const double*, hipStream_t);
 template void launch_convert<double, float>(long long, double*, const float*, hipStream_t);
 template void launch_convert<__half, float>(long long, __half*, const float*, hipStream_t);
 template void launch_convert<float, __half>(long long, float*, const __half*, hipStream_t);
-template <typename R> void launch_sub(long long n, R* out, const R* a, const R* b, hipStream_t st) {
-  LAUNCH((k_sub<R>), n, out, a, b);
-}
-
-#define INST(R)                                                                                                     \
-  template void launch_bicg_init_inner<R>(long long, R*, R*, R*, R*, const R*, double*, hipStream_t);              \
-  template void launch_bicg_init_outer<R>(long long, R*, R*, R*, const R*, const R*, double*, hipStream_t);        \
-  template void launch_dot<R>(long long, const R*, const R*, double*, hipStream_t);                                \
-  template void launch_norm_dot<R>(long long, const R*, const R*, double*, hipStream_t);                           \
-  template void launch_update_xr<R>(long long, R*, const R*, R*, const R*, const double*, const R*, double*,       \
-                                    hipStream_t);                                                                   \
-  template void launch_update_p<R>(long long, R*, const R*, const R*, const Scalars*, hipStream_t);                \
-  template void launch_norm<R>(long long, const R*, double*, hipStream_t);                                         \
-  template void launch_sub<R>(long long, R*, const R*, const R*, hipStream_t);
-INST(float)
-INST(double)
 
 }  // namespace qws

@@ -14,9 +14,9 @@ constexpr int HOP_THREADS = 256;
 
 template <typename S, int NL, int SET>
 __global__ void __launch_bounds__(HOP_THREADS)
-k_hop(Geo g, const int* __restrict__ list, int wg_per_blk, typename Store<S>::R gamma, S* out, const S* a,
-      const S* b, const S* c, const S* __restrict__ in, const S* __restrict__ U, const S* __restrict__ C,
-      int skip_interior) {
+k_hop(Geo g, const int* __restrict__ list, int wg_per_blk, int nrhs, long long stride, const int* __restrict__ active,
+      typename Store<S>::R gamma, S* out, const S* a, const S* b, const S* c, const S* __restrict__ in,
+      const S* __restrict__ U, const S* __restrict__ C, int skip_interior) {
   using R = typename Store<S>::R;
   const int wg = xcd_remap(blockIdx.x, gridDim.x);
   const int bi = wg / wg_per_blk;
@@ -32,35 +32,41 @@ k_hop(Geo g, const int* __restrict__ list, int wg_per_blk, typename Store<S>::R 
     for (int m = 0; m < 4; ++m) face |= (sc.l[m] == 0) | (sc.l[m] == g.B[m] - 1);
     if (!face) return;
   }
-  Cx<R> acc[12];
-  hop_sum<S, NL, SET>(g, s, sc, in, U, acc);
-  clover_apply<S>(C, s, g.V, acc);
-  Cx<R> v[12];
-  if (a) {
-    SpinorIO<S>::load(a, s, g.V, v);
-  } else {
+  // right-hand sides share this thread's links and clover (cache-resident after the first)
+  for (int r = 0; r < nrhs; ++r) {
+    if (active && !active[r]) continue;
+    const long long off = r * stride;
+    Cx<R> acc[12];
+    hop_sum<S, NL, SET>(g, s, sc, in + off, U, acc);
+    clover_apply<S>(C, s, g.V, acc);
+    Cx<R> v[12];
+    if (a) {
+      SpinorIO<S>::load(a + off, s, g.V, v);
+    } else {
 #pragma unroll
-    for (int k = 0; k < 12; ++k) v[k] = {0, 0};
+      for (int k = 0; k < 12; ++k) v[k] = {0, 0};
+    }
+    if (b) {
+      Cx<R> w[12];
+      SpinorIO<S>::load(b + off, s, g.V, w);
+#pragma unroll
+      for (int k = 0; k < 12; ++k) v[k] = cadd(v[k], w[k]);
+    }
+    if (c) {
+      Cx<R> w[12];
+      SpinorIO<S>::load(c + off, s, g.V, w);
+#pragma unroll
+      for (int k = 0; k < 12; ++k) v[k] = csub(v[k], w[k]);
+    }
+#pragma unroll
+    for (int k = 0; k < 12; ++k) v[k] = cadd(v[k], cscale(gamma, acc[k]));
+    SpinorIO<S>::store(out + off, s, g.V, v);
   }
-  if (b) {
-    Cx<R> w[12];
-    SpinorIO<S>::load(b, s, g.V, w);
-#pragma unroll
-    for (int k = 0; k < 12; ++k) v[k] = cadd(v[k], w[k]);
-  }
-  if (c) {
-    Cx<R> w[12];
-    SpinorIO<S>::load(c, s, g.V, w);
-#pragma unroll
-    for (int k = 0; k < 12; ++k) v[k] = csub(v[k], w[k]);
-  }
-#pragma unroll
-  for (int k = 0; k < 12; ++k) v[k] = cadd(v[k], cscale(gamma, acc[k]));
-  SpinorIO<S>::store(out, s, g.V, v);
 }
 
 template <typename S>
-__global__ void __launch_bounds__(HOP_THREADS) k_zero_blocks(Geo g, const int* list, int wg_per_blk, S* out) {
+__global__ void __launch_bounds__(HOP_THREADS)
+k_zero_blocks(Geo g, const int* list, int wg_per_blk, int nrhs, long long stride, const int* active, S* out) {
   using R = typename Store<S>::R;
   const int bi = blockIdx.x / wg_per_blk;
   const int blk = list ? list[bi] : bi;
@@ -70,48 +76,50 @@ __global__ void __launch_bounds__(HOP_THREADS) k_zero_blocks(Geo g, const int* l
   Cx<R> v[12];
 #pragma unroll
   for (int k = 0; k < 12; ++k) v[k] = {0, 0};
-  SpinorIO<S>::store(out, s, g.V, v);
+  for (int r = 0; r < nrhs; ++r)
+    if (!active || active[r]) SpinorIO<S>::store(out + r * stride, s, g.V, v);
 }
 
 template <typename S, int NL>
-void launch_hop(const Geo& g, BlockSel sel, int set, typename Store<S>::R gamma, S* out, const S* a, const S* b,
-                const S* c, const S* in, const S* U, const S* C, hipStream_t st) {
+void launch_hop(const Geo& g, BlockSel sel, const Batch& B, int set, typename Store<S>::R gamma, S* out, const S* a,
+                const S* b, const S* c, const S* in, const S* U, const S* C, hipStream_t st) {
   const int wpb = (g.Vb + HOP_THREADS - 1) / HOP_THREADS;
   dim3 grid(sel.n * wpb), block(HOP_THREADS);
   const int skip = (set == HOPS_FACE && out == a && b == nullptr && c == nullptr) ? 1 : 0;
+#define HOP_ARGS g, sel.list, wpb, B.n, B.stride, B.active, gamma, out, a, b, c, in, U, C
   switch (set) {
     case HOPS_ALL:
-      hipLaunchKernelGGL((k_hop<S, NL, HOPS_ALL>), grid, block, 0, st, g, sel.list, wpb, gamma, out, a, b, c, in, U, C, skip);
+      hipLaunchKernelGGL((k_hop<S, NL, HOPS_ALL>), grid, block, 0, st, HOP_ARGS, skip);
       break;
     case HOPS_INSIDE:
-      hipLaunchKernelGGL((k_hop<S, NL, HOPS_INSIDE>), grid, block, 0, st, g, sel.list, wpb, gamma, out, a, b, c, in, U, C, skip);
+      hipLaunchKernelGGL((k_hop<S, NL, HOPS_INSIDE>), grid, block, 0, st, HOP_ARGS, skip);
       break;
     case HOPS_FACE:
-      hipLaunchKernelGGL((k_hop<S, NL, HOPS_FACE>), grid, block, 0, st, g, sel.list, wpb, gamma, out, a, b, c, in, U, C, skip);
+      hipLaunchKernelGGL((k_hop<S, NL, HOPS_FACE>), grid, block, 0, st, HOP_ARGS, skip);
       break;
     default:
-      hipLaunchKernelGGL((k_hop<S, NL, HOPS_NONE>), grid, block, 0, st, g, sel.list, wpb, gamma, out, a, b, c, in, U, C, 0);
+      hipLaunchKernelGGL((k_hop<S, NL, HOPS_NONE>), grid, block, 0, st, HOP_ARGS, 0);
   }
+#undef HOP_ARGS
 }
 
 template <typename S>
-void launch_zero_blocks(const Geo& g, BlockSel sel, S* out, hipStream_t st) {
+void launch_zero_blocks(const Geo& g, BlockSel sel, const Batch& B, S* out, hipStream_t st) {
   const int wpb = (g.Vb + HOP_THREADS - 1) / HOP_THREADS;
-  hipLaunchKernelGGL((k_zero_blocks<S>), dim3(sel.n * wpb), dim3(HOP_THREADS), 0, st, g, sel.list, wpb, out);
+  hipLaunchKernelGGL((k_zero_blocks<S>), dim3(sel.n * wpb), dim3(HOP_THREADS), 0, st, g, sel.list, wpb, B.n, B.stride,
+                     B.active, out);
 }
 
-template void launch_hop<float, 18>(const Geo&, BlockSel, int, float, float*, const float*, const float*, const float*,
-                                    const float*, const float*, const float*, hipStream_t);
-template void launch_hop<float, 12>(const Geo&, BlockSel, int, float, float*, const float*, const float*, const float*,
-                                    const float*, const float*, const float*, hipStream_t);
-template void launch_hop<double, 18>(const Geo&, BlockSel, int, double, double*, const double*, const double*,
-                                     const double*, const double*, const double*, const double*, hipStream_t);
-template void launch_hop<__half, 12>(const Geo&, BlockSel, int, float, __half*, const __half*, const __half*,
-                                     const __half*, const __half*, const __half*, const __half*, hipStream_t);
-template void launch_hop<__half, 18>(const Geo&, BlockSel, int, float, __half*, const __half*, const __half*,
-                                     const __half*, const __half*, const __half*, const __half*, hipStream_t);
-template void launch_zero_blocks<float>(const Geo&, BlockSel, float*, hipStream_t);
-template void launch_zero_blocks<double>(const Geo&, BlockSel, double*, hipStream_t);
-template void launch_zero_blocks<__half>(const Geo&, BlockSel, __half*, hipStream_t);
+#define INST_HOP(S, NL)                                                                                           \
+  template void launch_hop<S, NL>(const Geo&, BlockSel, const Batch&, int, typename Store<S>::R, S*, const S*,     \
+                                  const S*, const S*, const S*, const S*, const S*, hipStream_t);
+INST_HOP(float, 18)
+INST_HOP(float, 12)
+INST_HOP(double, 18)
+INST_HOP(__half, 18)
+INST_HOP(__half, 12)
+template void launch_zero_blocks<float>(const Geo&, BlockSel, const Batch&, float*, hipStream_t);
+template void launch_zero_blocks<double>(const Geo&, BlockSel, const Batch&, double*, hipStream_t);
+template void launch_zero_blocks<__half>(const Geo&, BlockSel, const Batch&, __half*, hipStream_t);
 
 }  // namespace qws

@@ -18,13 +18,14 @@ struct BlockSel {
 // Any of a, b, c may be nullptr (term absent).  out may alias a, b or c, never in.
 // skip_interior: for set == HOPS_FACE with out == a, sites without a face hop
 // are left untouched.
+struct Batch;
 template <typename S, int NL>
-void launch_hop(const Geo& g, BlockSel sel, int set, typename Store<S>::R gamma, S* out, const S* a,
+void launch_hop(const Geo& g, BlockSel sel, const Batch& B, int set, typename Store<S>::R gamma, S* out, const S* a,
                 const S* b, const S* c, const S* in, const S* U, const S* C, hipStream_t st);
 
 // out = 0 on the selected blocks
 template <typename S>
-void launch_zero_blocks(const Geo& g, BlockSel sel, S* out, hipStream_t st);
+void launch_zero_blocks(const Geo& g, BlockSel sel, const Batch& B, S* out, hipStream_t st);
 
 // ---- BiCGStab vector kernels (full lattice, n = number of complex numbers)
 struct Scalars {
@@ -47,22 +48,42 @@ enum Finalize {
   FIN_RAW              // red[] only
 };
 
-template <typename R> void launch_bicg_init_inner(long long n, R* x, R* r, R* p, R* r0, const R* b, double* part, hipStream_t st);
-template <typename R> void launch_bicg_init_outer(long long n, R* r, R* p, R* r0, const R* b, const R* q, double* part, hipStream_t st);
-template <typename R> void launch_dot(long long n, const R* a, const R* b, double* part, hipStream_t st);      // <a,b>
-template <typename R> void launch_norm_dot(long long n, const R* t, const R* r, double* part, hipStream_t st);  // |t|^2, <t,r>
-// x += c v1 ; r -= c v2 ; red |r|^2 (and <r0,r> if r0)
+// A batch of right-hand sides: vector r of a buffer sits at base + r * stride
+// (stride in elements of the buffer's type); RHS with active[r] == 0 are skipped.
+struct Batch {
+  int n;
+  long long stride;
+  const int* active;  // device flags, or nullptr (all active)
+};
+
+// n = complex numbers per vector; part = partials [nrhs][4][RED_BLOCKS]; sc = Scalars[nrhs]
 template <typename R>
-void launch_update_xr(long long n, R* x, const R* v1, R* r, const R* v2, const double* c, const R* r0, double* part,
-                      hipStream_t st);
+void launch_bicg_init_inner(const Batch& B, long long n, R* x, R* r, R* p, R* r0, const R* b, double* part,
+                            hipStream_t st);
+template <typename R>
+void launch_bicg_init_outer(const Batch& B, long long n, R* r, R* p, R* r0, const R* b, const R* q, double* part,
+                            hipStream_t st);
+template <typename R> void launch_dot(const Batch& B, long long n, const R* a, const R* b, double* part, hipStream_t st);
+template <typename R>
+void launch_norm_dot(const Batch& B, long long n, const R* t, const R* r, double* part, hipStream_t st);
+// x += c v1 ; r -= c v2 ; red |r|^2 (and <r0,r> if r0); c = alpha (coef 0) or omega (coef 1)
+template <typename R>
+void launch_update_xr(const Batch& B, long long n, R* x, const R* v1, R* r, const R* v2, const Scalars* sc, int coef,
+                      const R* r0, double* part, hipStream_t st);
 // p = r + beta (p - omega q)
-template <typename R> void launch_update_p(long long n, R* p, const R* r, const R* q, const Scalars* sc, hipStream_t st);
-template <typename R> void launch_norm(long long n, const R* a, double* part, hipStream_t st);
-void launch_finalize(const double* part, int nval, Scalars* sc, int mode, hipStream_t st);
+template <typename R>
+void launch_update_p(const Batch& B, long long n, R* p, const R* r, const R* q, const Scalars* sc, hipStream_t st);
+template <typename R> void launch_norm(const Batch& B, long long n, const R* a, double* part, hipStream_t st);
+void launch_finalize(const Batch& B, const double* part, int nval, Scalars* sc, int mode, hipStream_t st);
 // Per-site precision conversion of spinors (V sites; the storage layouts of the
 // precisions differ): out = in / nrm (to lower precision), out = in * nrm (back).
-template <typename S> void launch_d2s_scaled(long long V, S* out, const double* in, const Scalars* sc, hipStream_t st);
-template <typename S> void launch_s2d_scaled(long long V, double* out, const S* in, const Scalars* sc, hipStream_t st);
+// so/si: RHS strides of out/in in elements.
+template <typename S>
+void launch_d2s_scaled(const Batch& B, long long V, long long so, long long si, S* out, const double* in,
+                       const Scalars* sc, hipStream_t st);
+template <typename S>
+void launch_s2d_scaled(const Batch& B, long long V, long long so, long long si, double* out, const S* in,
+                       const Scalars* sc, hipStream_t st);
 template <typename SO, typename SI> void launch_convert(long long V, SO* out, const SI* in, hipStream_t st);
 // out = a - b (real arrays of length n)
 template <typename R> void launch_sub(long long n, R* out, const R* a, const R* b, hipStream_t st);

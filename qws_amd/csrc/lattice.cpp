@@ -1,15 +1,16 @@
 // lattice.cpp -- host side of the device solver: field loading, the SAP step
-// sequences, the FP32 inner and FP64 outer BiCGStab, true residual.
+// sequences, the FP32 inner and FP64 outer BiCGStab (batched over right-hand
+// sides), true residual.
 #include "lattice.h"
 
 #include <chrono>
+#include <cmath>
+#include <cstddef>
+#include <cstring>
 #include <type_traits>
 
 #include "fieldgen.h"
 #include "sap.h"
-#include <cmath>
-#include <cstddef>
-#include <cstring>
 
 namespace qws {
 
@@ -19,6 +20,7 @@ template <typename T> T* dalloc(size_t n) {
   QWS_HIP(hipMalloc(&p, n * sizeof(T)));
   return (T*)p;
 }
+Batch single() { return Batch{1, 0, nullptr}; }
 }  // namespace
 
 Lattice::Lattice(const int L[4], const int B[4], int tbc, int links32) {
@@ -41,6 +43,7 @@ Lattice::Lattice(const int L[4], const int B[4], int tbc, int links32) {
       throw std::invalid_argument("y/z/t need an even number of blocks or a single block");
     if (m > 0 && g_.NB[m] == 1) improper_ = true;  // QWS's npe[mu]==1 branch (qws.cc:805)
   }
+  if (V >= (1LL << 31)) throw std::invalid_argument("lattice too large (site index must fit in 31 bits)");
   g_.sB[0] = 1;
   g_.sN[0] = 1;
   for (int m = 1; m < 4; ++m) {
@@ -69,22 +72,21 @@ Lattice::Lattice(const int L[4], const int B[4], int tbc, int links32) {
     d_blocks_[c] = dalloc<int>(lists[c].size() + 1);
     QWS_HIP(hipMemcpy(d_blocks_[c], lists[c].data(), lists[c].size() * sizeof(int), hipMemcpyHostToDevice));
   }
-  part_ = dalloc<double>(4 * RED_BLOCKS);
-  sc_in_ = dalloc<Scalars>(1);
-  sc_out_ = dalloc<Scalars>(1);
-  QWS_HIP(hipMemset(sc_in_, 0, sizeof(Scalars)));
-  QWS_HIP(hipMemset(sc_out_, 0, sizeof(Scalars)));
-  QWS_HIP(hipHostMalloc((void**)&sc_host_, sizeof(Scalars), hipHostMallocDefault));
+  sc_one_ = dalloc<Scalars>(1);
+  QWS_HIP(hipMemset(sc_one_, 0, sizeof(Scalars)));
+  d_inner_mask_ = dalloc<int>(MAX_BATCH);
+  QWS_HIP(hipHostMalloc((void**)&sc_host_, sizeof(Scalars) * MAX_BATCH, hipHostMallocDefault));
 }
 
 Lattice::~Lattice() {
-  hipStreamSynchronize(st_);
+  (void)hipStreamSynchronize(st_);
+  batch_.reset();
   for (auto* p : {(void*)Ud_, (void*)Cd_, (void*)Us_, (void*)Cs_, (void*)Uh_, (void*)Ch_, (void*)part_, (void*)sc_in_,
-                  (void*)sc_out_, (void*)d_blocks_[0], (void*)d_blocks_[1]})
-    if (p) hipFree(p);
-  for (auto& w : work_) hipFree(w.second.first);
-  if (sc_host_) hipHostFree(sc_host_);
-  if (st_) hipStreamDestroy(st_);
+                  (void*)sc_one_, (void*)d_inner_mask_, (void*)d_blocks_[0], (void*)d_blocks_[1]})
+    if (p) (void)hipFree(p);
+  for (auto& w : work_) (void)hipFree(w.second.first);
+  if (sc_host_) (void)hipHostFree(sc_host_);
+  if (st_) (void)hipStreamDestroy(st_);
 }
 
 void* Lattice::work(const std::string& name, size_t bytes) {
@@ -93,6 +95,7 @@ void* Lattice::work(const std::string& name, size_t bytes) {
       if (w.second.second >= bytes) return w.second.first;
       QWS_HIP(hipStreamSynchronize(st_));
       QWS_HIP(hipFree(w.second.first));
+      w.second.first = nullptr;
       QWS_HIP(hipMalloc(&w.second.first, bytes));
       w.second.second = bytes;
       return w.second.first;
@@ -103,9 +106,28 @@ void* Lattice::work(const std::string& name, size_t bytes) {
   return p;
 }
 
+double* Lattice::reduction_partials(int nrhs) {
+  if (nrhs > part_rhs_) {
+    QWS_HIP(hipStreamSynchronize(st_));
+    if (part_) QWS_HIP(hipFree(part_));
+    part_ = dalloc<double>((size_t)nrhs * 4 * RED_BLOCKS);
+    part_rhs_ = nrhs;
+  }
+  return part_;
+}
+
 size_t Lattice::spinor_bytes(int prec) const {
   const size_t es = prec == P64 ? 8 : (prec == P32 ? 4 : 2);
   return (size_t)g_.V * 24 * es;
+}
+
+size_t Lattice::field_bytes() const {
+  size_t b = 0;
+  if (Ud_) b += (size_t)g_.V * 72 * 8;
+  if (Cd_) b += (size_t)g_.V * 72 * 8;
+  if (Us_) b += (size_t)g_.V * 4 * nl32_ * 4;
+  if (Cs_) b += (size_t)g_.V * 72 * 4;
+  return b;
 }
 
 void* Lattice::alloc_spinor(int prec) {
@@ -174,7 +196,7 @@ void Lattice::compute_clover(double csw) {
 void Lattice::export_field(int kind, double* host) {
   const double* src = kind == 1 ? Ud_ : Cd_;
   if (!src) throw std::runtime_error("export_field: field not present");
-  const size_t n = (size_t)g_.V * (kind == 1 ? 72 : 72);
+  const size_t n = (size_t)g_.V * 72;  // links 4 x 18, clover 72 reals per site
   double* stage = (double*)work("stage", n * sizeof(double));
   launch_export_field(g_, kind, src, stage, st_);
   QWS_HIP(hipMemcpyAsync(host, stage, n * sizeof(double), hipMemcpyDeviceToHost, st_));
@@ -184,11 +206,12 @@ void Lattice::export_field(int kind, double* host) {
 void Lattice::gaussian_spinor(int prec, void* dev, unsigned long long seed) {
   if (prec == P64) {
     launch_gaussian_spinor(g_, (uint64_t)seed, (double*)dev, st_);
-  } else {
+  } else if (prec == P32) {
     double* tmp = (double*)work("gen_tmp", spinor_bytes(P64));
     launch_gaussian_spinor(g_, (uint64_t)seed, tmp, st_);
-    if (prec == P32) launch_convert<float, double>(g_.V, (float*)dev, tmp, st_);
-    else throw std::invalid_argument("gaussian_spinor: FP64 or FP32");
+    launch_convert<float, double>(g_.V, (float*)dev, tmp, st_);
+  } else {
+    throw std::invalid_argument("gaussian_spinor: FP64 or FP32");
   }
 }
 
@@ -198,11 +221,12 @@ void Lattice::point_spinor(int prec, void* dev, const int x[4], int spin, int co
   if (spin < 0 || spin > 3 || colour < 0 || colour > 2) throw std::invalid_argument("point_spinor: spin/colour");
   if (prec == P64) {
     launch_point_spinor(g_, x, spin, colour, (double*)dev, st_);
-  } else {
+  } else if (prec == P32) {
     double* tmp = (double*)work("gen_tmp", spinor_bytes(P64));
     launch_point_spinor(g_, x, spin, colour, tmp, st_);
-    if (prec == P32) launch_convert<float, double>(g_.V, (float*)dev, tmp, st_);
-    else throw std::invalid_argument("point_spinor: FP64 or FP32");
+    launch_convert<float, double>(g_.V, (float*)dev, tmp, st_);
+  } else {
+    throw std::invalid_argument("point_spinor: FP64 or FP32");
   }
 }
 
@@ -253,12 +277,13 @@ BlockSel Lattice::sel(int colour) const {
 }
 
 template <typename S>
-void Lattice::hop(BlockSel s, int set, typename Store<S>::R gamma, S* out, const S* a, const S* b, const S* c,
-                  const S* in) {
+void Lattice::hop(BlockSel s, const Batch& B, int set, typename Store<S>::R gamma, S* out, const S* a, const S* b,
+                  const S* c, const S* in) {
   if (s.n == 0) return;
   FieldSet<S> f = fields<S>();
-  if (f.nl == 18) launch_hop<S, 18>(g_, s, set, gamma, out, a, b, c, in, f.U, f.C, st_);
-  else if constexpr (!std::is_same<S, double>::value) launch_hop<S, 12>(g_, s, set, gamma, out, a, b, c, in, f.U, f.C, st_);
+  if (f.nl == 18) launch_hop<S, 18>(g_, s, B, set, gamma, out, a, b, c, in, f.U, f.C, st_);
+  else if constexpr (!std::is_same<S, double>::value)
+    launch_hop<S, 12>(g_, s, B, set, gamma, out, a, b, c, in, f.U, f.C, st_);
 }
 
 #define DISPATCH_PREC(prec, CALL)                                  \
@@ -269,31 +294,35 @@ void Lattice::hop(BlockSel s, int set, typename Store<S>::R gamma, S* out, const
   } while (0)
 
 void Lattice::dslash(int prec, void* out, const void* in, int hops, int colour) {
-  DISPATCH_PREC(prec, (hop<S>(sel(colour), hops, (typename Store<S>::R)(-kappa_), (S*)out, (const S*)in, nullptr,
-                              nullptr, (const S*)in)));
+  DISPATCH_PREC(prec, (hop<S>(sel(colour), single(), hops, (typename Store<S>::R)(-kappa_), (S*)out, (const S*)in,
+                              nullptr, nullptr, (const S*)in)));
+}
+
+void Lattice::dslash_batch(const Batch& B, double* out, const double* in) {
+  hop<double>(sel(-1), B, HOPS_ALL, -kappa_, out, in, nullptr, nullptr, in);
 }
 
 void Lattice::face(int prec, void* out, const void* in, double factor, int colour) {
-  DISPATCH_PREC(prec, (hop<S>(sel(colour), HOPS_FACE, (typename Store<S>::R)factor, (S*)out, (const S*)out, nullptr,
-                              nullptr, (const S*)in)));
+  DISPATCH_PREC(prec, (hop<S>(sel(colour), single(), HOPS_FACE, (typename Store<S>::R)factor, (S*)out, (const S*)out,
+                              nullptr, nullptr, (const S*)in)));
 }
 
 void Lattice::clover(int prec, void* out, const void* in) {
-  DISPATCH_PREC(prec, (hop<S>(sel(-1), HOPS_NONE, (typename Store<S>::R)1, (S*)out, nullptr, nullptr, nullptr,
-                              (const S*)in)));
+  DISPATCH_PREC(prec, (hop<S>(sel(-1), single(), HOPS_NONE, (typename Store<S>::R)1, (S*)out, nullptr, nullptr,
+                              nullptr, (const S*)in)));
 }
 
 // Jacobi block solve (static_solver.cc:65-140): x_1 = 2b - D_in b, then
 // x_{k+1} = x_k + b - D_in x_k, i.e. x_{k+1} = b + kappa T^-1 H_in x_k with x_0 = b.
 // Iterates alternate between x and tmp so that the last one lands in x.
 template <typename S>
-void Lattice::jinv_t(int colour, S* x, const S* b, S* tmp, int nm) {
+void Lattice::jinv_t(const Batch& B, int colour, S* x, const S* b, S* tmp, int nm) {
   using R = typename Store<S>::R;
   if (nm < 1) nm = 1;  // the first application is unconditional (static_solver.cc:77-101)
   const S* prev = b;
   for (int k = 1; k <= nm; ++k) {
     S* dst = ((nm - k) % 2 == 0) ? x : tmp;
-    hop<S>(sel(colour), HOPS_INSIDE, (R)kappa_, dst, b, nullptr, nullptr, prev);
+    hop<S>(sel(colour), B, HOPS_INSIDE, (R)kappa_, dst, b, nullptr, nullptr, prev);
     prev = dst;
   }
 }
@@ -301,24 +330,24 @@ void Lattice::jinv_t(int colour, S* x, const S* b, S* tmp, int nm) {
 void Lattice::jinv(int prec, void* x, const void* b, int colour, int nm) {
   DISPATCH_PREC(prec, ({
     S* tmp = (S*)work("jinv_tmp", spinor_bytes(prec));
-    jinv_t<S>(colour, (S*)x, (const S*)b, tmp, nm);
+    jinv_t<S>(single(), colour, (S*)x, (const S*)b, tmp, nm);
   }));
 }
 
 // SAP sweeps (qws.cc:809-841): for each colour c (0 = domain_e, 1 = domain_o):
 //   q_c = A_c s_c ;  s_c += in_c - D_in q_c ;  s_o += kappa T^-1 H_face q
 template <typename S>
-void Lattice::sap_sweeps(const S* in, S* s, S* q, S* tmp, int nsap, int nm) {
+void Lattice::sap_sweeps(const Batch& B, const S* in, S* s, S* q, S* tmp, int nsap, int nm) {
   using R = typename Store<S>::R;
   const R k = (R)kappa_;
-  QWS_HIP(hipMemcpyAsync(s, in, spinor_bytes(sizeof(S) == 8 ? P64 : sizeof(S) == 4 ? P32 : P16),
-                         hipMemcpyDeviceToDevice, st_));
-  if (improper_) launch_zero_blocks<S>(g_, sel(1), q, st_);
+  const int pr = sizeof(S) == 8 ? P64 : sizeof(S) == 4 ? P32 : P16;
+  QWS_HIP(hipMemcpyAsync(s, in, spinor_bytes(pr) * B.n, hipMemcpyDeviceToDevice, st_));
+  if (improper_) launch_zero_blocks<S>(g_, sel(1), B, q, st_);
   for (int isap = 0; isap < nsap; ++isap)
     for (int c = 0; c < 2; ++c) {
-      jinv_t<S>(c, q, s, tmp, nm);
-      hop<S>(sel(c), HOPS_INSIDE, k, s, s, in, q, q);  // s_c = s_c + in_c - q_c + kappa T^-1 H_in q_c
-      hop<S>(sel(1 - c), HOPS_FACE, k, s, s, nullptr, nullptr, q);
+      jinv_t<S>(B, c, q, s, tmp, nm);
+      hop<S>(sel(c), B, HOPS_INSIDE, k, s, s, in, q, q);  // s_c = s_c + in_c - q_c + kappa T^-1 H_in q_c
+      hop<S>(sel(1 - c), B, HOPS_FACE, k, s, s, nullptr, nullptr, q);
     }
 }
 
@@ -330,96 +359,96 @@ template <typename S> bool Lattice::fused_ok() const {
 
 // the nsap sweeps of qws.cc:809-841 with the face update of each half-sweep
 // done at the start of the next one
-void Lattice::sap_sweeps_fused(const float* in, float* s, float* q, int nsap, int nm) {
+void Lattice::sap_sweeps_fused(const Batch& B, const float* in, float* s, float* q, int nsap, int nm) {
   const FieldSet<float> f = fields<float>();
   for (int isap = 0; isap < nsap; ++isap)
     for (int c = 0; c < 2; ++c) {
       int flags = SAP_EPI_SUPDATE;
       if (isap == 0) flags |= SAP_SRC_IN;
       if (isap > 0 || c == 1) flags |= SAP_FACE_PRO;
-      launch_sap_block(g_, sel(c), f.nl, (float)kappa_, nm, flags, s, in, q, q, nullptr, f.U, f.C, st_);
+      launch_sap_block(g_, sel(c), B, f.nl, (float)kappa_, nm, flags, s, in, q, q, nullptr, f.U, f.C, st_);
     }
 }
 
 // prec_ddd_s_ (qws.cc:776-926): out = D M_SAP in, with the final sequence of the
 // improper (npe[mu]==1) and of the proper colouring.
 template <typename S>
-void Lattice::prec_ddd_t(S* out, const S* in, int nsap, int nm) {
+void Lattice::prec_ddd_t(const Batch& B, S* out, const S* in, int nsap, int nm) {
   using R = typename Store<S>::R;
   const int pr = sizeof(S) == 8 ? P64 : sizeof(S) == 4 ? P32 : P16;
-  S* s = (S*)work("sap_s", spinor_bytes(pr));
-  S* q = (S*)work("sap_q", spinor_bytes(pr));
-  S* tmp = (S*)work("sap_tmp", spinor_bytes(pr));
+  S* s = (S*)work("sap_s", spinor_bytes(pr) * B.n);
+  S* q = (S*)work("sap_q", spinor_bytes(pr) * B.n);
   const R k = (R)kappa_;
   if constexpr (std::is_same<S, float>::value) {
     if (fused_ok<S>()) {
       const FieldSet<float> f = fields<float>();
       const int src = nsap == 0 ? SAP_SRC_IN : 0;
-      sap_sweeps_fused(in, s, q, nsap, nm);
+      sap_sweeps_fused(B, in, s, q, nsap, nm);
       // final e: q_e = A_e s_e, out_e = D_in q_e
-      launch_sap_block(g_, sel(0), f.nl, (float)kappa_, nm, SAP_EPI_OUT | src | (nsap > 0 ? SAP_FACE_PRO : 0), s, in,
-                       q, q, out, f.U, f.C, st_);
+      launch_sap_block(g_, sel(0), B, f.nl, (float)kappa_, nm, SAP_EPI_OUT | src | (nsap > 0 ? SAP_FACE_PRO : 0), s,
+                       in, q, q, out, f.U, f.C, st_);
       // final o: s_o += kappa T^-1 H_face q, q_o = A_o s_o, out_o = D_oe q_e + D_in q_o
-      launch_sap_block(g_, sel(1), f.nl, (float)kappa_, nm, SAP_EPI_OUT_FACE | src | SAP_FACE_PRO, s, in, q, q, out,
-                       f.U, f.C, st_);
-      hop<S>(sel(0), HOPS_FACE, -k, out, out, nullptr, nullptr, q);  // out_e += D_eo q_o
+      launch_sap_block(g_, sel(1), B, f.nl, (float)kappa_, nm, SAP_EPI_OUT_FACE | src | SAP_FACE_PRO, s, in, q, q,
+                       out, f.U, f.C, st_);
+      hop<S>(sel(0), B, HOPS_FACE, -k, out, out, nullptr, nullptr, q);  // out_e += D_eo q_o
       return;
     }
   }
-  sap_sweeps<S>(in, s, q, tmp, nsap, nm);
-  if (improper_) launch_zero_blocks<S>(g_, sel(1), q, st_);
-  jinv_t<S>(0, q, s, tmp, nm);
-  hop<S>(sel(0), HOPS_INSIDE, -k, out, q, nullptr, nullptr, q);  // out_e = D_in q_e
+  S* tmp = (S*)work("sap_tmp", spinor_bytes(pr) * B.n);
+  sap_sweeps<S>(B, in, s, q, tmp, nsap, nm);
+  if (improper_) launch_zero_blocks<S>(g_, sel(1), B, q, st_);
+  jinv_t<S>(B, 0, q, s, tmp, nm);
+  hop<S>(sel(0), B, HOPS_INSIDE, -k, out, q, nullptr, nullptr, q);  // out_e = D_in q_e
   if (improper_) {
-    hop<S>(sel(1), HOPS_FACE, k, s, s, nullptr, nullptr, q);      // s_o -= D_face q (q_o = 0)
-    jinv_t<S>(1, q, s, tmp, nm);
-    hop<S>(sel(1), HOPS_INSIDE, -k, out, q, nullptr, nullptr, q);  // out_o = D_in q_o
-    hop<S>(sel(0), HOPS_FACE, -k, out, out, nullptr, nullptr, q);  // out_e += D_face q
-    hop<S>(sel(1), HOPS_FACE, -k, out, out, nullptr, nullptr, q);  // out_o += D_face q
+    hop<S>(sel(1), B, HOPS_FACE, k, s, s, nullptr, nullptr, q);      // s_o -= D_face q (q_o = 0)
+    jinv_t<S>(B, 1, q, s, tmp, nm);
+    hop<S>(sel(1), B, HOPS_INSIDE, -k, out, q, nullptr, nullptr, q);  // out_o = D_in q_o
+    hop<S>(sel(0), B, HOPS_FACE, -k, out, out, nullptr, nullptr, q);  // out_e += D_face q
+    hop<S>(sel(1), B, HOPS_FACE, -k, out, out, nullptr, nullptr, q);  // out_o += D_face q
   } else {
-    hop<S>(sel(1), HOPS_FACE, -k, out, nullptr, nullptr, nullptr, q);  // out_o = D_oe q_e
-    hop<S>(sel(1), HOPS_FACE, k, s, s, nullptr, nullptr, q);           // s_o -= out_o
-    jinv_t<S>(1, q, s, tmp, nm);
-    hop<S>(sel(1), HOPS_INSIDE, -k, out, out, q, nullptr, q);          // out_o += D_in q_o
-    hop<S>(sel(0), HOPS_FACE, -k, out, out, nullptr, nullptr, q);      // out_e += D_eo q_o
+    hop<S>(sel(1), B, HOPS_FACE, -k, out, nullptr, nullptr, nullptr, q);  // out_o = D_oe q_e
+    hop<S>(sel(1), B, HOPS_FACE, k, s, s, nullptr, nullptr, q);           // s_o -= out_o
+    jinv_t<S>(B, 1, q, s, tmp, nm);
+    hop<S>(sel(1), B, HOPS_INSIDE, -k, out, out, q, nullptr, q);          // out_o += D_in q_o
+    hop<S>(sel(0), B, HOPS_FACE, -k, out, out, nullptr, nullptr, q);      // out_e += D_eo q_o
   }
 }
 
 // prec_s_ (qws.cc:929-975): out = M_SAP in
 template <typename S>
-void Lattice::prec_s_t(S* out, const S* in, int nsap, int nm) {
+void Lattice::prec_s_t(const Batch& B, S* out, const S* in, int nsap, int nm) {
   using R = typename Store<S>::R;
   const int pr = sizeof(S) == 8 ? P64 : sizeof(S) == 4 ? P32 : P16;
-  S* s = (S*)work("sap_s", spinor_bytes(pr));
-  S* q = (S*)work("sap_q", spinor_bytes(pr));
-  S* tmp = (S*)work("sap_tmp", spinor_bytes(pr));
+  S* s = (S*)work("sap_s", spinor_bytes(pr) * B.n);
+  S* q = (S*)work("sap_q", spinor_bytes(pr) * B.n);
   if constexpr (std::is_same<S, float>::value) {
     if (fused_ok<S>()) {
       const FieldSet<float> f = fields<float>();
       const int src = nsap == 0 ? SAP_SRC_IN : 0;
-      sap_sweeps_fused(in, s, q, nsap, nm);
-      launch_sap_block(g_, sel(0), f.nl, (float)kappa_, nm, SAP_EPI_NONE | src | (nsap > 0 ? SAP_FACE_PRO : 0), s, in,
-                       q, out, nullptr, f.U, f.C, st_);
-      launch_sap_block(g_, sel(1), f.nl, (float)kappa_, nm, SAP_EPI_NONE | src | SAP_FACE_PRO, s, in, out, out,
+      sap_sweeps_fused(B, in, s, q, nsap, nm);
+      launch_sap_block(g_, sel(0), B, f.nl, (float)kappa_, nm, SAP_EPI_NONE | src | (nsap > 0 ? SAP_FACE_PRO : 0), s,
+                       in, q, out, nullptr, f.U, f.C, st_);
+      launch_sap_block(g_, sel(1), B, f.nl, (float)kappa_, nm, SAP_EPI_NONE | src | SAP_FACE_PRO, s, in, out, out,
                        nullptr, f.U, f.C, st_);
       return;
     }
   }
-  sap_sweeps<S>(in, s, q, tmp, nsap, nm);
-  if (improper_) launch_zero_blocks<S>(g_, sel(1), out, st_);
-  jinv_t<S>(0, out, s, tmp, nm);
-  hop<S>(sel(1), HOPS_FACE, (R)kappa_, s, s, nullptr, nullptr, out);
-  jinv_t<S>(1, out, s, tmp, nm);
+  S* tmp = (S*)work("sap_tmp", spinor_bytes(pr) * B.n);
+  sap_sweeps<S>(B, in, s, q, tmp, nsap, nm);
+  if (improper_) launch_zero_blocks<S>(g_, sel(1), B, out, st_);
+  jinv_t<S>(B, 0, out, s, tmp, nm);
+  hop<S>(sel(1), B, HOPS_FACE, (R)kappa_, s, s, nullptr, nullptr, out);
+  jinv_t<S>(B, 1, out, s, tmp, nm);
 }
 
 void Lattice::prec_ddd(int prec, void* out, const void* in, int nsap, int nm) {
   if (nsap > 10 || nm > 10) throw std::invalid_argument("prec_ddd_s: nsap and nm must be <= 10 (qws.cc:798-799)");
-  DISPATCH_PREC(prec, (prec_ddd_t<S>((S*)out, (const S*)in, nsap, nm)));
+  DISPATCH_PREC(prec, (prec_ddd_t<S>(single(), (S*)out, (const S*)in, nsap, nm)));
 }
 
 void Lattice::prec_s(int prec, void* out, const void* in, int nsap, int nm) {
   if (nsap > 10 || nm > 10) throw std::invalid_argument("prec_s: nsap and nm must be <= 10");
-  DISPATCH_PREC(prec, (prec_s_t<S>((S*)out, (const S*)in, nsap, nm)));
+  DISPATCH_PREC(prec, (prec_s_t<S>(single(), (S*)out, (const S*)in, nsap, nm)));
 }
 
 double Lattice::read_scalar(const Scalars* sc, size_t offset) {
@@ -430,108 +459,260 @@ double Lattice::read_scalar(const Scalars* sc, size_t offset) {
   return v;
 }
 
+void Lattice::read_scalars(const Scalars* dev, int n) {
+  QWS_HIP(hipMemcpyAsync(sc_host_, dev, sizeof(Scalars) * n, hipMemcpyDeviceToHost, st_));
+  QWS_HIP(hipStreamSynchronize(st_));
+}
+
 double Lattice::norm2(int prec, const void* a) {
   const long long n = g_.V * 12;
-  if (prec == P64) launch_norm<double>(n, (const double*)a, part_, st_);
-  else if (prec == P32) launch_norm<float>(n, (const float*)a, part_, st_);
+  double* part = reduction_partials(1);
+  if (prec == P64) launch_norm<double>(single(), n, (const double*)a, part, st_);
+  else if (prec == P32) launch_norm<float>(single(), n, (const float*)a, part, st_);
   else throw std::invalid_argument("norm2: FP16 vectors are not reduced directly");
-  launch_finalize(part_, 1, sc_out_, FIN_RAW, st_);
-  return read_scalar(sc_out_, offsetof(Scalars, red));
+  launch_finalize(single(), part, 1, sc_one_, FIN_RAW, st_);
+  return read_scalar(sc_one_, offsetof(Scalars, red));
 }
 
 // ------------------------------------------------------------------ inner BiCGStab
-// Right-preconditioned BiCGStab for (D M_SAP) x = b, x0 = 0 (bicgstab_precdd_s.cc:85-559).
+// Right-preconditioned BiCGStab for (D M_SAP) x = b, x0 = 0 (bicgstab_precdd_s.cc:85-559),
+// for every RHS k with live[k] != 0.  B.active must point at d_inner_mask_; an RHS
+// leaves the batch when its recursive residual meets tol (tol < 0: fixed count).
 template <typename S>
-int Lattice::bicg_t(S* x, const S* b, double tol, int maxiter, int nsap, int nm) {
+void Lattice::bicg_t(const Batch& Bin, S* x, const S* b, double tol, int maxiter, int nsap, int nm,
+                     std::vector<int>& iters) {
   static_assert(sizeof(S) == 4, "the inner solver runs on FP32 vectors");
+  const int nb = Bin.n;
   const long long n = g_.V * 12;
-  const size_t bytes = spinor_bytes(P32);
+  const size_t bytes = spinor_bytes(P32) * nb;
   S* q = (S*)work("bicg_q", bytes);
   S* r = (S*)work("bicg_r", bytes);
   S* p = (S*)work("bicg_p", bytes);
   S* t = (S*)work("bicg_t", bytes);
   S* r0 = (S*)work("bicg_r0", bytes);
-  Scalars* sc = sc_in_;
-  launch_bicg_init_inner<S>(n, x, r, p, r0, b, part_, st_);
-  launch_finalize(part_, 1, sc, FIN_INIT_INNER, st_);
-  double bnorm = -1;
-  if (tol >= 0) bnorm = read_scalar(sc, offsetof(Scalars, bnorm));
-  int iter;
-  for (iter = 0; iter < maxiter; ++iter) {
-    prec_ddd_t<S>(q, p, nsap, nm);
-    launch_dot<S>(n, r0, q, part_, st_);
-    launch_finalize(part_, 2, sc, FIN_ALPHA, st_);
-    launch_update_xr<S>(n, x, p, r, q, sc->alpha, nullptr, part_, st_);
-    launch_finalize(part_, 1, sc, FIN_RNORM, st_);
-    if (tol >= 0 && std::sqrt(read_scalar(sc, offsetof(Scalars, rnorm)) / bnorm) < tol) { ++iter; break; }
-    prec_ddd_t<S>(t, r, nsap, nm);
-    launch_norm_dot<S>(n, t, r, part_, st_);
-    launch_finalize(part_, 3, sc, FIN_OMEGA, st_);
-    launch_update_xr<S>(n, x, r, r, t, sc->omega, r0, part_, st_);
-    launch_finalize(part_, 3, sc, FIN_RHO, st_);
-    if (tol >= 0 && std::sqrt(read_scalar(sc, offsetof(Scalars, rnorm)) / bnorm) < tol) { ++iter; break; }
-    launch_update_p<S>(n, p, r, q, sc, st_);
+  if (nb > sc_in_n_) {
+    if (sc_in_) QWS_HIP(hipFree(sc_in_));
+    sc_in_ = dalloc<Scalars>(nb);
+    sc_in_n_ = nb;
   }
-  return iter;
+  Scalars* sc = sc_in_;
+  double* part = reduction_partials(nb);
+  std::vector<int> live(nb);
+  for (int k = 0; k < nb; ++k) live[k] = iters[k] >= 0 ? 1 : 0;  // iters[k] < 0 marks RHS outside the batch
+  for (int k = 0; k < nb; ++k) iters[k] = 0;
+  auto upload = [&] {
+    QWS_HIP(hipMemcpyAsync(d_inner_mask_, live.data(), sizeof(int) * nb, hipMemcpyHostToDevice, st_));
+  };
+  upload();
+  const Batch B{nb, g_.V * 24, d_inner_mask_};
+  launch_bicg_init_inner<S>(B, n, x, r, p, r0, b, part, st_);
+  launch_finalize(B, part, 1, sc, FIN_INIT_INNER, st_);
+  std::vector<double> bnorm(nb, 1.0);
+  if (tol >= 0) {
+    read_scalars(sc, nb);
+    for (int k = 0; k < nb; ++k) bnorm[k] = sc_host_[k].bnorm;
+  }
+  auto converge = [&]() {  // drop converged RHS from the batch; true if any remain
+    if (tol < 0) return true;
+    read_scalars(sc, nb);
+    bool changed = false, any = false;
+    for (int k = 0; k < nb; ++k) {
+      if (!live[k]) continue;
+      if (std::sqrt(sc_host_[k].rnorm / bnorm[k]) < tol) {
+        live[k] = 0;
+        changed = true;
+      } else {
+        any = true;
+      }
+    }
+    if (changed) upload();
+    return any;
+  };
+  auto any_live = [&] {
+    for (int v : live)
+      if (v) return true;
+    return false;
+  };
+  for (int iter = 0; iter < maxiter && any_live(); ++iter) {
+    for (int k = 0; k < nb; ++k) iters[k] += live[k];
+    prec_ddd_t<S>(B, q, p, nsap, nm);
+    launch_dot<S>(B, n, r0, q, part, st_);
+    launch_finalize(B, part, 2, sc, FIN_ALPHA, st_);
+    launch_update_xr<S>(B, n, x, p, r, q, sc, 0, nullptr, part, st_);
+    launch_finalize(B, part, 1, sc, FIN_RNORM, st_);
+    if (!converge()) break;
+    prec_ddd_t<S>(B, t, r, nsap, nm);
+    launch_norm_dot<S>(B, n, t, r, part, st_);
+    launch_finalize(B, part, 3, sc, FIN_OMEGA, st_);
+    launch_update_xr<S>(B, n, x, r, r, t, sc, 1, r0, part, st_);
+    launch_finalize(B, part, 3, sc, FIN_RHO, st_);
+    if (!converge()) break;
+    launch_update_p<S>(B, n, p, r, q, sc, st_);
+  }
 }
 
 int Lattice::bicgstab_precdd(int prec, void* x, const void* b, double tol, int maxiter, int nsap, int nm) {
   if (prec != P32) throw std::invalid_argument("bicgstab_precdd: FP32 vectors only");
-  return bicg_t<float>((float*)x, (const float*)b, tol, maxiter, nsap, nm);
+  std::vector<int> it(1, 0);
+  bicg_t<float>(Batch{1, g_.V * 24, d_inner_mask_}, (float*)x, (const float*)b, tol, maxiter, nsap, nm, it);
+  return it[0];
 }
 
 // ------------------------------------------------------------------ outer mixed BiCGStab
 // u = M p: normalise, FP32 inner solve, prec_s_, rescale (bicgstab_dd_mix.cc:76-102,
-// reordering.cc:108-204).
-void Lattice::approx_inv(double* u, const double* p, const SolveParams& prm, int* inner) {
+// reordering.cc:108-204), for the RHS of B with live[k] != 0 (B.active holds the
+// same flags on the device); sc = the outer scalars (nrm is set here); the inner
+// iterations are added to inner[k].
+void Lattice::approx_inv(const Batch& B, Scalars* sc, const std::vector<int>& live, double* u, const double* p,
+                         const SolveParams& prm, std::vector<int>& inner) {
   const long long n = g_.V * 12;
-  float* bs = (float*)work("ai_b", spinor_bytes(P32));
-  float* ts = (float*)work("ai_t", spinor_bytes(P32));
-  float* xs = (float*)work("ai_x", spinor_bytes(P32));
-  launch_norm<double>(n, p, part_, st_);
-  launch_finalize(part_, 1, sc_out_, FIN_NRM, st_);
-  launch_d2s_scaled<float>(g_.V, bs, p, sc_out_, st_);
-  int it = bicg_t<float>(ts, bs, prm.tol_s, prm.maxiter_s, prm.nsap, prm.nm);
-  prec_s_t<float>(xs, ts, prm.nsap, prm.nm);
-  launch_s2d_scaled<float>(g_.V, u, xs, sc_out_, st_);
-  if (inner) *inner += it;
+  const int nb = B.n;
+  const size_t bytes = spinor_bytes(P32) * nb;
+  float* bs = (float*)work("ai_b", bytes);
+  float* ts = (float*)work("ai_t", bytes);
+  float* xs = (float*)work("ai_x", bytes);
+  double* part = reduction_partials(nb);
+  launch_norm<double>(B, n, p, part, st_);
+  launch_finalize(B, part, 1, sc, FIN_NRM, st_);
+  launch_d2s_scaled<float>(B, g_.V, B.stride, B.stride, bs, p, sc, st_);
+  std::vector<int> it(nb);
+  for (int k = 0; k < nb; ++k) it[k] = live[k] ? 0 : -1;
+  bicg_t<float>(Batch{nb, g_.V * 24, d_inner_mask_}, ts, bs, prm.tol_s, prm.maxiter_s, prm.nsap, prm.nm, it);
+  prec_s_t<float>(B, xs, ts, prm.nsap, prm.nm);
+  launch_s2d_scaled<float>(B, g_.V, B.stride, B.stride, u, xs, sc, st_);
+  for (int k = 0; k < nb; ++k)
+    if (live[k]) inner[k] += it[k];
 }
 
-// FP64 outer BiCGStab on D, x = initial guess (bicgstab_dd_mix.cc:105-388).
-int Lattice::bicgstab_dd_mix(double* x, const double* b, const SolveParams& prm, int* inner_total) {
-  const long long n = g_.V * 12;
-  const size_t bytes = spinor_bytes(P64);
-  double* q = (double*)work("mix_q", bytes);
-  double* r = (double*)work("mix_r", bytes);
-  double* p = (double*)work("mix_p", bytes);
-  double* t = (double*)work("mix_t", bytes);
-  double* u = (double*)work("mix_u", bytes);
-  double* r0 = (double*)work("mix_r0", bytes);
-  Scalars* sc = sc_out_;
-  dslash(P64, q, x, HOPS_ALL, -1);
-  launch_bicg_init_outer<double>(n, r, p, r0, b, q, part_, st_);
-  launch_finalize(part_, 2, sc, FIN_INIT_OUTER, st_);
-  double bnorm = -1;
-  if (prm.tol >= 0) bnorm = read_scalar(sc, offsetof(Scalars, bnorm));
-  int iter;
-  for (iter = 1; iter < prm.maxiter; ++iter) {
-    approx_inv(u, p, prm, inner_total);
-    dslash(P64, q, u, HOPS_ALL, -1);
-    launch_dot<double>(n, r0, q, part_, st_);
-    launch_finalize(part_, 2, sc, FIN_ALPHA, st_);
-    launch_update_xr<double>(n, x, u, r, q, sc->alpha, nullptr, part_, st_);
-    launch_finalize(part_, 1, sc, FIN_RNORM, st_);
-    if (prm.tol >= 0 && std::sqrt(read_scalar(sc, offsetof(Scalars, rnorm)) / bnorm) < prm.tol) break;
-    approx_inv(u, r, prm, inner_total);
-    dslash(P64, t, u, HOPS_ALL, -1);
-    launch_norm_dot<double>(n, t, r, part_, st_);
-    launch_finalize(part_, 3, sc, FIN_OMEGA, st_);
-    launch_update_xr<double>(n, x, u, r, t, sc->omega, r0, part_, st_);
-    launch_finalize(part_, 3, sc, FIN_RHO, st_);
-    if (prm.tol >= 0 && std::sqrt(read_scalar(sc, offsetof(Scalars, rnorm)) / bnorm) < prm.tol) break;
-    launch_update_p<double>(n, p, r, q, sc, st_);
+OuterBatch::OuterBatch(Lattice& L, int capacity, const SolveParams& p) : L_(L), cap_(capacity), prm_(p) {
+  if (cap_ < 1 || cap_ > MAX_BATCH) throw std::invalid_argument("batch capacity must be in 1..64");
+  if (p.nsap > 10 || p.nm > 10) throw std::invalid_argument("nsap and nm must be <= 10 (qws.cc:798-799)");
+  stride_ = L.volume() * 24;
+  const size_t bytes = sizeof(double) * (size_t)stride_ * cap_;
+  double** bufs[] = {&x_, &b_, &q_, &r_, &p_, &t_, &u_, &r0_};
+  for (double** b : bufs) {
+    *b = nullptr;
+    QWS_HIP(hipMalloc((void**)b, bytes));
   }
-  return iter;
+  QWS_HIP(hipMalloc((void**)&sc_, sizeof(Scalars) * cap_));
+  QWS_HIP(hipMemset(sc_, 0, sizeof(Scalars) * cap_));
+  QWS_HIP(hipMalloc((void**)&d_mask_, sizeof(int) * cap_));
+  state_.assign(cap_, EMPTY);
+  iters_.assign(cap_, 0);
+  inner_.assign(cap_, 0);
+}
+
+OuterBatch::~OuterBatch() {
+  (void)hipStreamSynchronize(L_.stream());
+  for (double* b : {x_, b_, q_, r_, p_, t_, u_, r0_})
+    if (b) (void)hipFree(b);
+  if (sc_) (void)hipFree(sc_);
+  if (d_mask_) (void)hipFree(d_mask_);
+}
+
+int OuterBatch::active_count() const {
+  int n = 0;
+  for (int s : state_) n += s == RUNNING;
+  return n;
+}
+
+void OuterBatch::upload_mask(const std::vector<int>& m) {
+  QWS_HIP(hipMemcpyAsync(d_mask_, m.data(), sizeof(int) * cap_, hipMemcpyHostToDevice, L_.stream()));
+}
+
+// start slot k: x = x0 (or 0), r = b - D x, p = r0 = r (bicgstab_dd_mix.cc:136-158)
+void OuterBatch::admit(int k, const double* b, const double* x0) {
+  if (k < 0 || k >= cap_) throw std::invalid_argument("admit: slot out of range");
+  hipStream_t st = L_.stream();
+  const size_t bytes = sizeof(double) * stride_;
+  if (b != this->b(k)) QWS_HIP(hipMemcpyAsync(this->b(k), b, bytes, hipMemcpyDeviceToDevice, st));
+  if (!x0) QWS_HIP(hipMemsetAsync(x(k), 0, bytes, st));
+  else if (x0 != x(k)) QWS_HIP(hipMemcpyAsync(x(k), x0, bytes, hipMemcpyDeviceToDevice, st));
+  std::vector<int> only(cap_, 0);
+  only[k] = 1;
+  upload_mask(only);
+  const Batch B{cap_, stride_, d_mask_};
+  double* part = L_.reduction_partials(cap_);
+  L_.dslash_batch(B, q_, x_);
+  launch_bicg_init_outer<double>(B, L_.volume() * 12, r_, p_, r0_, b_, q_, part, st);
+  launch_finalize(B, part, 2, sc_, FIN_INIT_OUTER, st);
+  state_[k] = RUNNING;
+  iters_[k] = 0;
+  inner_[k] = 0;
+}
+
+// Slots whose recursive residual sqrt(|r|^2/|b|^2) met tol finish
+// (bicgstab_dd_mix.cc:249,336).
+void OuterBatch::check(std::vector<int>& live) {
+  if (prm_.tol < 0) return;
+  L_.read_scalars(sc_, cap_);
+  const Scalars* h = L_.host_scalars();
+  bool changed = false;
+  for (int k = 0; k < cap_; ++k)
+    if (live[k] && std::sqrt(h[k].rnorm / h[k].bnorm) < prm_.tol) {
+      live[k] = 0;
+      state_[k] = FINISHED;
+      changed = true;
+    }
+  if (changed) upload_mask(live);
+}
+
+int OuterBatch::step() {
+  std::vector<int> live(cap_);
+  int nlive = 0;
+  for (int k = 0; k < cap_; ++k) nlive += (live[k] = state_[k] == RUNNING);
+  if (!nlive) return 0;
+  hipStream_t st = L_.stream();
+  const long long n = L_.volume() * 12;
+  double* part = L_.reduction_partials(cap_);
+  upload_mask(live);
+  const Batch B{cap_, stride_, d_mask_};
+  for (int k = 0; k < cap_; ++k) iters_[k] += live[k];
+  // u = M p ; q = D u ; alpha ; x += alpha u ; r -= alpha q
+  L_.approx_inv(B, sc_, live, u_, p_, prm_, inner_);
+  L_.dslash_batch(B, q_, u_);
+  launch_dot<double>(B, n, r0_, q_, part, st);
+  launch_finalize(B, part, 2, sc_, FIN_ALPHA, st);
+  launch_update_xr<double>(B, n, x_, u_, r_, q_, sc_, 0, nullptr, part, st);
+  launch_finalize(B, part, 1, sc_, FIN_RNORM, st);
+  check(live);
+  bool any = false;
+  for (int v : live) any |= v != 0;
+  if (any) {
+    // u = M r ; t = D u ; omega ; x += omega u ; r -= omega t ; rho, beta
+    L_.approx_inv(B, sc_, live, u_, r_, prm_, inner_);
+    L_.dslash_batch(B, t_, u_);
+    launch_norm_dot<double>(B, n, t_, r_, part, st);
+    launch_finalize(B, part, 3, sc_, FIN_OMEGA, st);
+    launch_update_xr<double>(B, n, x_, u_, r_, t_, sc_, 1, r0_, part, st);
+    launch_finalize(B, part, 3, sc_, FIN_RHO, st);
+    check(live);
+    any = false;
+    for (int v : live) any |= v != 0;
+    if (any) launch_update_p<double>(B, n, p_, r_, q_, sc_, st);
+  }
+  // the reference loops iter = 1 .. maxiter-1 and reports maxiter when it ran out
+  for (int k = 0; k < cap_; ++k)
+    if (state_[k] == RUNNING && iters_[k] >= prm_.maxiter - 1) {
+      state_[k] = FINISHED;
+      iters_[k] = prm_.maxiter;
+    }
+  return active_count();
+}
+
+int Lattice::bicgstab_dd_mix(double* x, const double* b, const SolveParams& prm, int* inner_total) {
+  if (prm.maxiter <= 1) {
+    if (inner_total) *inner_total = 0;
+    return prm.maxiter;
+  }
+  OuterBatch ob(*this, 1, prm);
+  ob.admit(0, b, x);
+  while (ob.step() > 0) {
+  }
+  QWS_HIP(hipMemcpyAsync(x, ob.x(0), spinor_bytes(P64), hipMemcpyDeviceToDevice, st_));
+  sync();
+  if (inner_total) *inner_total = ob.inner_iterations(0);
+  return ob.iterations(0);
 }
 
 // |b - D x| / |b| in FP64 on the GPU
@@ -545,21 +726,51 @@ double Lattice::true_residual(const double* x, const double* b) {
   return std::sqrt(rr / bb);
 }
 
-SolveStats Lattice::solve(double* x, const double* b, const SolveParams& prm) {
-  SolveStats st;
+std::vector<SolveStats> Lattice::solve_batch(double* const* x, const double* const* b, int nrhs,
+                                             const SolveParams& prm) {
+  std::vector<SolveStats> out(nrhs);
+  if (nrhs <= 0) return out;
   auto t0 = std::chrono::steady_clock::now();
-  for (int attempt = 0;; ++attempt) {
-    int inner = 0;
-    int it = bicgstab_dd_mix(x, b, prm, &inner);
-    st.outer_iters += it;
-    st.inner_iters += inner;
-    st.true_residual = true_residual(x, b);
-    if (prm.true_tol <= 0 || st.true_residual <= prm.true_tol || attempt >= prm.max_restarts) break;
-    st.restarts++;
+  if (!batch_ || batch_->capacity() != nrhs) {
+    batch_.reset();
+    batch_.reset(new OuterBatch(*this, nrhs, prm));
   }
+  OuterBatch& ob = *batch_;
+  ob.set_params(prm);
+  for (int k = 0; k < nrhs; ++k) ob.admit(k, b[k], x[k]);
+  std::vector<int> open(nrhs, 1);
+  for (;;) {
+    while (ob.step() > 0) {
+    }
+    // FP64 true residual of every slot that just finished; restart those above true_tol
+    bool again = false;
+    for (int k = 0; k < nrhs; ++k) {
+      if (!open[k]) continue;
+      out[k].outer_iters += ob.iterations(k);
+      out[k].inner_iters += ob.inner_iterations(k);
+      out[k].true_residual = true_residual(ob.x(k), ob.b(k));
+      if (prm.true_tol > 0 && out[k].true_residual > prm.true_tol && out[k].restarts < prm.max_restarts) {
+        out[k].restarts++;
+        ob.admit(k, ob.b(k), ob.x(k));
+        again = true;
+      } else {
+        open[k] = 0;
+      }
+    }
+    if (!again) break;
+  }
+  for (int k = 0; k < nrhs; ++k)
+    QWS_HIP(hipMemcpyAsync(x[k], ob.x(k), spinor_bytes(P64), hipMemcpyDeviceToDevice, st_));
   sync();
-  st.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-  return st;
+  const double sec = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  for (auto& s : out) s.seconds = sec;
+  return out;
+}
+
+SolveStats Lattice::solve(double* x, const double* b, const SolveParams& prm) {
+  double* xs[1] = {x};
+  const double* bs[1] = {b};
+  return solve_batch(xs, bs, 1, prm)[0];
 }
 
 double Lattice::time_dslash(int prec, int reps) {
@@ -576,8 +787,8 @@ double Lattice::time_dslash(int prec, int reps) {
   QWS_HIP(hipEventSynchronize(e1));
   float ms = 0;
   QWS_HIP(hipEventElapsedTime(&ms, e0, e1));
-  hipEventDestroy(e0);
-  hipEventDestroy(e1);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
   return ms / reps;
 }
 

@@ -4,6 +4,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <memory>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -42,6 +43,46 @@ struct SolveStats {
   double true_residual = -1, seconds = 0;
 };
 
+class Lattice;
+
+// A batch of FP64 outer BiCGStab solves (bicgstab_dd_mix.cc:105-388) advanced one
+// outer iteration at a time.  Each slot holds one right-hand side with its own
+// recurrence; a slot can be (re)filled between iterations while the others run
+// on (continuous batching), and every kernel processes all running slots in one
+// launch, so the links and clover are read once per batch.
+class OuterBatch {
+ public:
+  OuterBatch(Lattice& L, int capacity, const SolveParams& p);
+  ~OuterBatch();
+  int capacity() const { return cap_; }
+  void set_params(const SolveParams& p) { prm_ = p; }
+  // start slot k on source b (device FP64 spinor) with initial guess x0 (nullptr: zero)
+  void admit(int k, const double* b, const double* x0);
+  // one outer iteration of every running slot; returns the number still running
+  int step();
+  bool running(int k) const { return state_[k] == RUNNING; }
+  bool finished(int k) const { return state_[k] == FINISHED; }
+  int iterations(int k) const { return iters_[k]; }
+  int inner_iterations(int k) const { return inner_[k]; }
+  double* x(int k) const { return x_ + (size_t)k * stride_; }
+  double* b(int k) const { return b_ + (size_t)k * stride_; }
+  void release(int k) { state_[k] = EMPTY; }
+  int active_count() const;
+
+ private:
+  enum { EMPTY = 0, RUNNING = 1, FINISHED = 2 };
+  void upload_mask(const std::vector<int>& m);
+  void check(std::vector<int>& live);  // converged slots leave `live`
+  Lattice& L_;
+  int cap_;
+  SolveParams prm_;
+  long long stride_;
+  double *x_, *b_, *q_, *r_, *p_, *t_, *u_, *r0_;
+  Scalars* sc_;
+  int* d_mask_;
+  std::vector<int> state_, iters_, inner_;
+};
+
 class Lattice {
  public:
   Lattice(const int L[4], const int B[4], int tbc, int links32);
@@ -62,21 +103,22 @@ class Lattice {
   // fields from host arrays (layout 0 plain, 1 QWS AoSoA)
   void load_gauge(const void* U, int layout, int is_double, int target);  // target: bit0 FP64, bit1 FP32
   void load_clover(const void* C, int layout, int is_double, int target);
-  void ensure_half_fields();
   // generated inputs: SU(3) links exp(i eps H), clover inverse from the links
   void generate_gauge(double eps, unsigned long long seed);
   void compute_clover(double csw);
   void export_field(int kind, double* host_plain);  // kind 1 links, 2 clover (FP64)
   void gaussian_spinor(int prec, void* dev, unsigned long long seed);
   void point_spinor(int prec, void* dev, const int x[4], int spin, int colour);
+  size_t field_bytes() const;
 
   // spinor I/O between host arrays and device buffers
   void* alloc_spinor(int prec);
   void upload_spinor(int prec, void* dev, const void* host, int layout, int host_is_double);
   void download_spinor(int prec, const void* dev, void* host, int layout, int host_is_double);
   size_t spinor_bytes(int prec) const;
+  long long spinor_elems() const { return g_.V * 24; }
 
-  // operators (prec selects the field set)
+  // operators on one spinor (prec selects the field set)
   void dslash(int prec, void* out, const void* in, int hops, int colour);
   void face(int prec, void* out, const void* in, double factor, int colour);
   void clover(int prec, void* out, const void* in);
@@ -86,28 +128,38 @@ class Lattice {
   int bicgstab_precdd(int prec, void* x, const void* b, double tol, int maxiter, int nsap, int nm);
   int bicgstab_dd_mix(double* x, const double* b, const SolveParams& p, int* inner_total);
   SolveStats solve(double* x, const double* b, const SolveParams& p);
+  // n independent right-hand sides solved together (x[k] holds the initial guess)
+  std::vector<SolveStats> solve_batch(double* const* x, const double* const* b, int n, const SolveParams& p);
   double norm2(int prec, const void* a);
   double true_residual(const double* x, const double* b);
   double time_dslash(int prec, int reps);
   void sync() { QWS_HIP(hipStreamSynchronize(st_)); }
+
+  // ---- batched building blocks (used by OuterBatch)
+  void dslash_batch(const Batch& B, double* out, const double* in);
+  void approx_inv(const Batch& B, Scalars* sc, const std::vector<int>& live, double* u, const double* p,
+                  const SolveParams& prm, std::vector<int>& inner);
+  Scalars* host_scalars() { return sc_host_; }
+  void read_scalars(const Scalars* dev, int n);
+  void* work(const std::string& name, size_t bytes);
+  double* reduction_partials(int nrhs);
 
  private:
   template <typename S> struct FieldSet { const S* U; const S* C; int nl; };
   template <typename S> FieldSet<S> fields() const;
   BlockSel sel(int colour) const;
   template <typename S>
-  void hop(BlockSel s, int set, typename Store<S>::R gamma, S* out, const S* a, const S* b, const S* c,
-           const S* in);
-  template <typename S> void jinv_t(int colour, S* x, const S* b, S* tmp, int nm);
-  template <typename S> void sap_sweeps(const S* in, S* s, S* q, S* tmp, int nsap, int nm);
+  void hop(BlockSel s, const Batch& B, int set, typename Store<S>::R gamma, S* out, const S* a, const S* b,
+           const S* c, const S* in);
+  template <typename S> void jinv_t(const Batch& B, int colour, S* x, const S* b, S* tmp, int nm);
+  template <typename S> void sap_sweeps(const Batch& B, const S* in, S* s, S* q, S* tmp, int nsap, int nm);
   template <typename S> bool fused_ok() const;
-  void sap_sweeps_fused(const float* in, float* s, float* q, int nsap, int nm);
-  template <typename S> void prec_ddd_t(S* out, const S* in, int nsap, int nm);
-  template <typename S> void prec_s_t(S* out, const S* in, int nsap, int nm);
-  template <typename S> int bicg_t(S* x, const S* b, double tol, int maxiter, int nsap, int nm);
-  void approx_inv(double* u, const double* p, const SolveParams& prm, int* inner);
+  void sap_sweeps_fused(const Batch& B, const float* in, float* s, float* q, int nsap, int nm);
+  template <typename S> void prec_ddd_t(const Batch& B, S* out, const S* in, int nsap, int nm);
+  template <typename S> void prec_s_t(const Batch& B, S* out, const S* in, int nsap, int nm);
+  template <typename S>
+  void bicg_t(const Batch& B, S* x, const S* b, double tol, int maxiter, int nsap, int nm, std::vector<int>& iters);
   double read_scalar(const Scalars* sc, size_t offset);
-  void* work(const std::string& name, size_t bytes);
 
   Geo g_;
   int nl32_;
@@ -124,10 +176,20 @@ class Lattice {
   bool use_fused_ = true;
   // reductions
   double* part_ = nullptr;
+  int part_rhs_ = 0;
   Scalars* sc_in_ = nullptr;
-  Scalars* sc_out_ = nullptr;
-  Scalars* sc_host_ = nullptr;  // pinned
+  int sc_in_n_ = 0;
+  Scalars* sc_one_ = nullptr;
+  Scalars* sc_host_ = nullptr;  // pinned, room for MAX_BATCH
+  int* d_inner_mask_ = nullptr;
   std::vector<std::pair<std::string, std::pair<void*, size_t>>> work_;
+
+  std::unique_ptr<OuterBatch> batch_;  // reused by solve/solve_batch (same capacity)
+
+  friend class OuterBatch;
+  void ensure_half_fields();
 };
+
+constexpr int MAX_BATCH = 64;
 
 }  // namespace qws

@@ -18,7 +18,8 @@ enum SapFlags {
 };
 
 bool sap_fusable(const Geo& g);
-void launch_sap_block(const Geo& g, BlockSel sel, int nl, float kappa, int nm, int flags, float* s, const float* in,
-                      const float* qface, float* qout, float* out, const float* U, const float* C, hipStream_t st);
+void launch_sap_block(const Geo& g, BlockSel sel, const Batch& B, int nl, float kappa, int nm, int flags, float* s,
+                      const float* in, const float* qface, float* qout, float* out, const float* U, const float* C,
+                      hipStream_t st);
 
 }  // namespace qws

@@ -93,8 +93,9 @@ __device__ __forceinline__ void clover_regs(const float cl[72], Cx<float> v[12])
 
 template <int NL>
 __global__ void __launch_bounds__(SAP_THREADS, 1)
-k_sap_block(Geo g, const int* __restrict__ list, float kappa, int nm, int flags, float* s, const float* __restrict__ in,
-            const float* qface, float* qout, float* out, const float* __restrict__ U, const float* __restrict__ C) {
+k_sap_block(Geo g, const int* __restrict__ list, int nrhs, long long stride, const int* __restrict__ active, float kappa,
+            int nm, int flags, float* s, const float* __restrict__ in, const float* qface, float* qout, float* out,
+            const float* __restrict__ U, const float* __restrict__ C) {
   constexpr int NC = NL / 2;
   __shared__ float lU[4 * NC * 2 * SAP_THREADS];
   __shared__ float lx[24 * SAP_THREADS];
@@ -104,7 +105,7 @@ k_sap_block(Geo g, const int* __restrict__ list, float kappa, int nm, int flags,
   SiteC c;
   site_coords(g, site, c);
 
-  // forward links of this site into LDS (coalesced [mu][k][V] reads)
+  // forward links of this site into LDS (coalesced [mu][k][V] reads), once for all RHS
   {
     const float2* Ug = reinterpret_cast<const float2*>(U);
 #pragma unroll
@@ -119,70 +120,77 @@ k_sap_block(Geo g, const int* __restrict__ list, float kappa, int nm, int flags,
   float cl[72];
   CloverIO<float>::load_chir(C, site, g.V, 0, cl);
   CloverIO<float>::load_chir(C, site, g.V, 1, cl + 36);
+  const int epi = flags & SAP_EPI_MASK;
 
-  Cx<float> sv[12], acc[12], f[12];
-  SpinorIO<float>::load((flags & SAP_SRC_IN) ? in : s, site, g.V, sv);
+  for (int r = 0; r < nrhs; ++r) {
+    if (active && !active[r]) continue;  // uniform over the workgroup
+    const long long off = r * stride;
+    Cx<float> sv[12], acc[12], f[12];
+    SpinorIO<float>::load(((flags & SAP_SRC_IN) ? in : s) + off, site, g.V, sv);
 
-  if (flags & SAP_FACE_PRO) {
-    hop_sum<float, NL, HOPS_FACE>(g, site, c, qface, U, f);
-    clover_regs(cl, f);
+    if (flags & SAP_FACE_PRO) {
+      hop_sum<float, NL, HOPS_FACE>(g, site, c, qface + off, U, f);
+      clover_regs(cl, f);
 #pragma unroll
-    for (int k = 0; k < 12; ++k) sv[k] = cadd(sv[k], cscale(kappa, f[k]));
-  }
+      for (int k = 0; k < 12; ++k) sv[k] = cadd(sv[k], cscale(kappa, f[k]));
+    }
 
-  // block solve: x_0 = s_c ; x_{k+1} = s_c + kappa T^-1 H_in x_k
-  Cx<float> x[12];
+    // block solve: x_0 = s_c ; x_{k+1} = s_c + kappa T^-1 H_in x_k
+    Cx<float> x[12];
 #pragma unroll
-  for (int k = 0; k < 12; ++k) x[k] = sv[k];
-  lds_store_spinor(lx, t, x);
-  __syncthreads();
-  for (int it = 0; it < nm; ++it) {
-    lds_hop_sum<NL>(g, c, t, lx, lU, acc);
-    clover_regs(cl, acc);
-#pragma unroll
-    for (int k = 0; k < 12; ++k) x[k] = cadd(sv[k], cscale(kappa, acc[k]));
-    __syncthreads();
+    for (int k = 0; k < 12; ++k) x[k] = sv[k];
+    __syncthreads();  // the previous RHS's readers of lx are done (and lU is complete)
     lds_store_spinor(lx, t, x);
     __syncthreads();
-  }
-  SpinorIO<float>::store(qout, site, g.V, x);
-
-  const int epi = flags & SAP_EPI_MASK;
-  if (epi == SAP_EPI_NONE) return;
-  // acc = T^-1 H_in x
-  lds_hop_sum<NL>(g, c, t, lx, lU, acc);
-  clover_regs(cl, acc);
-  Cx<float> o[12];
-  if (epi == SAP_EPI_SUPDATE) {
-    // s_c = s_c + in_c - x + kappa T^-1 H_in x
-    Cx<float> iv[12];
-    SpinorIO<float>::load(in, site, g.V, iv);
+    for (int it = 0; it < nm; ++it) {
+      lds_hop_sum<NL>(g, c, t, lx, lU, acc);
+      clover_regs(cl, acc);
 #pragma unroll
-    for (int k = 0; k < 12; ++k) o[k] = cadd(csub(cadd(sv[k], iv[k]), x[k]), cscale(kappa, acc[k]));
-    SpinorIO<float>::store(s, site, g.V, o);
-  } else {
-    // out_c = [- kappa f] + x - kappa T^-1 H_in x
-#pragma unroll
-    for (int k = 0; k < 12; ++k) o[k] = csub(x[k], cscale(kappa, acc[k]));
-    if (epi == SAP_EPI_OUT_FACE) {
-#pragma unroll
-      for (int k = 0; k < 12; ++k) o[k] = csub(o[k], cscale(kappa, f[k]));
+      for (int k = 0; k < 12; ++k) x[k] = cadd(sv[k], cscale(kappa, acc[k]));
+      __syncthreads();
+      lds_store_spinor(lx, t, x);
+      __syncthreads();
     }
-    SpinorIO<float>::store(out, site, g.V, o);
+    SpinorIO<float>::store(qout + off, site, g.V, x);
+
+    if (epi == SAP_EPI_NONE) continue;
+    // acc = T^-1 H_in x
+    lds_hop_sum<NL>(g, c, t, lx, lU, acc);
+    clover_regs(cl, acc);
+    Cx<float> o[12];
+    if (epi == SAP_EPI_SUPDATE) {
+      // s_c = s_c + in_c - x + kappa T^-1 H_in x
+      Cx<float> iv[12];
+      SpinorIO<float>::load(in + off, site, g.V, iv);
+#pragma unroll
+      for (int k = 0; k < 12; ++k) o[k] = cadd(csub(cadd(sv[k], iv[k]), x[k]), cscale(kappa, acc[k]));
+      SpinorIO<float>::store(s + off, site, g.V, o);
+    } else {
+      // out_c = [- kappa f] + x - kappa T^-1 H_in x
+#pragma unroll
+      for (int k = 0; k < 12; ++k) o[k] = csub(x[k], cscale(kappa, acc[k]));
+      if (epi == SAP_EPI_OUT_FACE) {
+#pragma unroll
+        for (int k = 0; k < 12; ++k) o[k] = csub(o[k], cscale(kappa, f[k]));
+      }
+      SpinorIO<float>::store(out + off, site, g.V, o);
+    }
   }
 }
 
 bool sap_fusable(const Geo& g) { return g.Vb == SAP_THREADS; }
 
-void launch_sap_block(const Geo& g, BlockSel sel, int nl, float kappa, int nm, int flags, float* s, const float* in,
-                      const float* qface, float* qout, float* out, const float* U, const float* C, hipStream_t st) {
+void launch_sap_block(const Geo& g, BlockSel sel, const Batch& B, int nl, float kappa, int nm, int flags, float* s,
+                      const float* in, const float* qface, float* qout, float* out, const float* U, const float* C,
+                      hipStream_t st) {
   if (sel.n == 0) return;
+  if (nm < 1) nm = 1;  // the first application is unconditional (static_solver.cc:77-101)
+#define SAP_ARGS g, sel.list, B.n, B.stride, B.active, kappa, nm, flags, s, in, qface, qout, out, U, C
   if (nl == 12)
-    hipLaunchKernelGGL((k_sap_block<12>), dim3(sel.n), dim3(SAP_THREADS), 0, st, g, sel.list, kappa, nm, flags, s, in,
-                       qface, qout, out, U, C);
+    hipLaunchKernelGGL((k_sap_block<12>), dim3(sel.n), dim3(SAP_THREADS), 0, st, SAP_ARGS);
   else
-    hipLaunchKernelGGL((k_sap_block<18>), dim3(sel.n), dim3(SAP_THREADS), 0, st, g, sel.list, kappa, nm, flags, s, in,
-                       qface, qout, out, U, C);
+    hipLaunchKernelGGL((k_sap_block<18>), dim3(sel.n), dim3(SAP_THREADS), 0, st, SAP_ARGS);
+#undef SAP_ARGS
 }
 
 }  // namespace qws
