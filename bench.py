@@ -28,7 +28,7 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 WORKLOAD = dict(L=[32, 32, 32, 64], B=[4, 4, 4, 4], eps=0.3, seed=2, kappa=0.1365, csw=1.0, tbc=-1, links32=12)
-# Solver settings of the measured workload (tuned on this hardware)
+# Solver settings of the measured workload (chosen from a sweep of nsap, nm and tol_s on MI355X)
 SOLVE = dict(nsap=4, nm=2, tol=1e-11, maxiter=40, tol_s=1e-3, maxiter_s=40, true_tol=1e-10, max_restarts=4)
 # algorithmic bytes per site of one FP32 D application with 12-real links:
 # 4 links x 12 x 4 B + clover 72 x 4 B + input 24 x 4 B + output 24 x 4 B
@@ -101,11 +101,26 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--dslash-only", type=int, default=0, help="only time N dslash launches (for counter runs)")
     ap.add_argument("--sweep", action="store_true", help="print solve statistics for a grid of solver settings")
+    ap.add_argument("--copy-calib", type=int, default=0,
+                    help="only run N device-to-device copies of 1 GiB (HBM counter calibration)")
     a = ap.parse_args()
     if a.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is design-only (DESIGN.md)",
+        print(json.dumps({"error": "this solver runs on one GPU; sharding over several GPUs is not built",
                           "n_gpus": a.gpus}))
         sys.exit(2)
+    if a.copy_calib:
+        # a plain copy of known size: its FETCH_SIZE / WRITE_SIZE counters give the
+        # factor between counter units and bytes
+        import torch
+
+        n = 1 << 28  # 2^28 floats = 1 GiB
+        src = torch.ones(n, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(a.copy_calib):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_each": 4 * n, "copies": a.copy_calib}))
+        return
     import qws_amd as Q
 
     lat, b = setup(Q)

@@ -78,6 +78,19 @@ void bicgstab_precdd_s_(scs_t* x, scs_t* b, double* tol, int* conviter, int* max
 void prec_ddd_s_(scs_t* out, scs_t* in, int* nsap, int* nm);
 /* qws.cc:929 -- out = M_SAP in */
 void prec_s_(scs_t* out, scs_t* in, int* nsap, int* nm);
+/* Half-precision variant (qws_h.h:60-62, bicgstab_dd_mix2_hf.cc:110, bicgstab_precdd_s_h.cc:83).
+ * qws_h_init_ makes the FP16 copies of the FP32 links and clover inverse;
+ * assign_mult_mSAP_s_h_ is M_SAP with FP16-stored iterates and fields (nsap - 1
+ * sweeps plus the final block solves, FP32 arithmetic); assign_mult_wd_s_ is the
+ * FP32 D; bicgstab_precdd_s_h_ is the inner BiCGStab whose x accumulates M_SAP p
+ * directly; bicgstab_dd_mix2_hf_ is the FP64 outer solve on top of it. */
+void qws_h_init_(const glus_t* u, const clvs_t* c);
+void assign_mult_mSAP_s_h_(scs_t* xs, const scs_t* bs, const int* nsap, const int* nm);
+void assign_mult_wd_s_(scs_t* x, const scs_t* b);
+void bicgstab_precdd_s_h_(scs_t* x, const scs_t* b, const double* tol, int* conviter, const int* maxiter,
+                          const int* nsap, const int* nm);
+void bicgstab_dd_mix2_hf_(scd_t* x, const scd_t* b, const double* tol, int* conviter, const int* maxiter,
+                          const double* tol_s, const int* maxiter_s, const int* nsap, const int* nm);
 /* static_solver.cc:141 -- Jacobi block solve on one domain (x, b point at that domain) */
 void jinv_ddd_in_s_(scs_t* x, scs_t* b, int* domain, int* maxiter);
 /* qws.cc:625,641 -- full operator D = 1 - kappa T^-1 H */
@@ -121,7 +134,8 @@ typedef struct {
   int maxiter_s;           /* inner iterations */
   double true_tol;         /* > 0: restart until the FP64 true residual |b-Dx|/|b| <= true_tol */
   int max_restarts;        /* restarts allowed for true_tol */
-  int sap_precision;       /* QWSG_F32 or QWSG_F16 (FP16 storage, FP32 arithmetic) */
+  int sap_precision;       /* QWSG_F32, or QWSG_F16: the method of bicgstab_dd_mix2_hf_ (FP16-stored
+                              SAP with nsap - 1 sweeps, FP32 D, inner x accumulates M_SAP p) */
 } qwsg_solve_params;
 
 typedef struct {
@@ -186,6 +200,11 @@ int qwsg_prec_ddd(qwsg_lattice* lat, int prec, void* out, const void* in, int ns
 int qwsg_prec_s(qwsg_lattice* lat, int prec, void* out, const void* in, int nsap, int nm);
 int qwsg_bicgstab_precdd(qwsg_lattice* lat, int prec, void* x, const void* b, double tol, int maxiter, int nsap,
                          int nm, int* iters);
+/* prec_ddd_s_h.h:63 -- out = M_SAP in with the FP16-stored SAP (FP32 vectors in and out) */
+int qwsg_msap_h(qwsg_lattice* lat, void* out, const void* in, int nsap, int nm);
+/* bicgstab_precdd_s_h.cc:83 -- inner solve D x = b with the FP16-stored SAP, x0 = 0 (FP32 vectors) */
+int qwsg_bicgstab_precdd_h(qwsg_lattice* lat, void* x, const void* b, double tol, int maxiter, int nsap, int nm,
+                           int* iters);
 /* FP64 outer solve; x (FP64) holds the initial guess */
 int qwsg_solve(qwsg_lattice* lat, void* x, const void* b, const qwsg_solve_params* p, qwsg_solve_stats* st);
 /* No counterpart in the reference (it solves one source per call): n independent

@@ -44,6 +44,8 @@ def lib():
         L.orc_jinv_s.argtypes = [vp, fp, fp, ctypes.c_int, ctypes.c_int]
         L.orc_prec_ddd_s.argtypes = [vp, fp, fp, ctypes.c_int, ctypes.c_int]
         L.orc_prec_s.argtypes = [vp, fp, fp, ctypes.c_int, ctypes.c_int]
+        L.orc_msap_h.argtypes = [vp, fp, fp, ctypes.c_int, ctypes.c_int]
+        L.orc_to_half.argtypes = [ctypes.c_long, fp, fp]
         L.orc_prec_ddd_d.argtypes = [vp, dp, dp, ctypes.c_int, ctypes.c_int]
         L.orc_bicgstab_precdd_s.argtypes = [vp, fp, fp, ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_int]
         L.orc_bicgstab_dd_mix.argtypes = [vp, dp, dp, ctypes.c_double, ctypes.c_int, ctypes.c_double,
@@ -153,6 +155,12 @@ class Oracle:
         lib().orc_prec_s(self.h, _p(out), _p(np.ascontiguousarray(psi, dtype=np.float32)), nsap, nm)
         return out
 
+    def msap_h(self, psi, nsap, nm):
+        """M_SAP with FP16 storage (assign_mult_mSAP_s_h_: nsap - 1 sweeps)."""
+        out = self._new(np.float32)
+        lib().orc_msap_h(self.h, _p(out), _p(np.ascontiguousarray(psi, dtype=np.float32)), nsap, nm)
+        return out
+
     def bicgstab_precdd_s(self, b, tol, maxiter, nsap, nm):
         x = self._new(np.float32)
         it = lib().orc_bicgstab_precdd_s(self.h, _p(x), _p(np.ascontiguousarray(b, dtype=np.float32)),
@@ -166,6 +174,14 @@ class Oracle:
                                       float(tol), maxiter, float(tol_s), maxiter_s, nsap, nm,
                                       ctypes.byref(inner))
         return x, it, inner.value
+
+
+def to_half(a):
+    """binary16 round-to-nearest-even of float32 values, returned as float32."""
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    out = np.empty_like(a)
+    lib().orc_to_half(a.size, _p(a), _p(out))
+    return out
 
 
 def norm2(a):

@@ -15,9 +15,10 @@
 // colour is (bx+by+bz+bt)%2, which is QWS's domain colour
 // domain_e=(py+pz+pt)%2 with bx = 2*px + d (qws.cc:275-276, main.cc:426).
 //
-// Parity status of each function (how it is pinned) is listed in DESIGN.md and in
-// tests/test_oracle.py; every function here is checked against data/CASE* golden
-// norms and/or the reference library built into oracle/_ref.
+// How each function is pinned: tests/test_oracle.py checks every function here
+// against the data/CASE* golden norms, the reference library built into
+// oracle/_ref, and closed forms (free field, Neumann series, Schwarz identities,
+// IEEE binary16 rounding).
 
 #include <cmath>
 #include <complex>
@@ -335,6 +336,92 @@ void prec_s(const Geom& g, const Fields<R>& f, R kappa, int nsap, int nm, R* out
   jinv(g, f, kappa, O, nm, out, s.data(), tmp.data());
 }
 
+// ---------------------------------------------------------------- FP16 storage
+// IEEE 754 binary16 round-to-nearest-even of a float (the value stored by an FP16
+// field), returned as a float.  Normal range: 11 significant bits, spacing
+// 2^(e-11) for |x| in [2^(e-1), 2^e); below 2^-14 the spacing is 2^-24
+// (subnormals); |x| >= 65520 rounds to infinity.
+float to_half(float x) {
+  if (!std::isfinite(x)) return x;
+  const float a = std::fabs(x);
+  if (a >= 65520.0f) return std::copysign(INFINITY, x);
+  float q;
+  if (a < 6.103515625e-05f) {
+    q = std::nearbyint(a * 16777216.0f) / 16777216.0f;
+  } else {
+    int e;
+    std::frexp(a, &e);
+    const float quantum = std::ldexp(1.0f, e - 11);
+    q = std::nearbyint(a / quantum) * quantum;
+  }
+  return std::copysign(q, x);
+}
+
+// out = a + b - c + factor * T^-1 H_set in on the sites of colour col, each
+// stored value rounded to binary16 (b, c may be null).
+void hop_h(const Geom& g, const Fields<float>& f, HopSet set, int col, float factor, float* out, const float* a,
+           const float* b, const float* c, const float* in) {
+#pragma omp parallel for schedule(static)
+  for (long s = 0; s < g.V; ++s) {
+    int cc4[4];
+    coords(g, s, cc4);
+    if (col >= 0 && colour(g, cc4) != col) continue;
+    cplx<float> acc[3][4];
+    clover_hops(g, f, in, s, set, acc);
+    for (int cc = 0; cc < 3; ++cc)
+      for (int sp = 0; sp < 4; ++sp)
+        for (int ri = 0; ri < 2; ++ri) {
+          const long k = s * 24 + (cc * 4 + sp) * 2 + ri;
+          float v = a[k];
+          if (b) v += b[k];
+          if (c) v -= c[k];
+          v += factor * (ri ? acc[cc][sp].imag() : acc[cc][sp].real());
+          out[k] = to_half(v);
+        }
+  }
+}
+
+// Jacobi block solve with FP16 storage: x_k = b + kappa T^-1 H_in x_{k-1}, x_0 = b
+// (the same sum as static_solver.cc:65-140), every x_k stored in binary16.
+void jinv_h(const Geom& g, const Fields<float>& f, float kappa, int col, int nm, float* x, const float* b,
+            float* tmp) {
+  if (nm < 1) nm = 1;
+  const float* prev = b;
+  for (int k = 1; k <= nm; ++k) {
+    float* dst = ((nm - k) % 2 == 0) ? x : tmp;
+    hop_h(g, f, HOPS_INSIDE, col, kappa, dst, b, nullptr, nullptr, prev);
+    prev = dst;
+  }
+}
+
+// assign_mult_mSAP_s_h_ (prec_ddd_s_h.h:63-231) in the FP16-storage / FP32-
+// arithmetic scheme: in is scaled to unit RMS component and rounded to binary16,
+// nsap - 1 colour sweeps and the final block solves of prec_s_ run with every
+// stored iterate rounded to binary16 and with binary16 links and clover inverse
+// (fh), and the result is scaled back.
+void msap_h(const Geom& g, const Fields<float>& fh, float kappa, int nsap, int nm, float* out, const float* in) {
+  const int E = 0, O = 1;
+  const long n = g.V * 24;
+  std::vector<float> ih(n), s(n), q(n, 0.0f), tmp(n), oh(n, 0.0f);
+  long double n2 = 0;
+  for (long i = 0; i < n; ++i) n2 += (long double)in[i] * in[i];
+  const double sigma = n2 > 0 ? std::sqrt(24.0 * (double)g.V / (double)n2) : 0.0;
+  const float fs = (float)sigma, fi = sigma > 0 ? (float)(1.0 / sigma) : 0.0f;
+  for (long i = 0; i < n; ++i) ih[i] = to_half(in[i] * fs);
+  s = ih;
+  const int ns = nsap > 1 ? nsap - 1 : 0;
+  for (int isap = 0; isap < ns; ++isap)
+    for (int c = 0; c < 2; ++c) {
+      jinv_h(g, fh, kappa, c, nm, q.data(), s.data(), tmp.data());
+      hop_h(g, fh, HOPS_INSIDE, c, kappa, s.data(), s.data(), ih.data(), q.data(), q.data());
+      hop_h(g, fh, HOPS_FACE, 1 - c, kappa, s.data(), s.data(), nullptr, nullptr, q.data());
+    }
+  jinv_h(g, fh, kappa, E, nm, oh.data(), s.data(), tmp.data());
+  hop_h(g, fh, HOPS_FACE, O, kappa, s.data(), s.data(), nullptr, nullptr, oh.data());
+  jinv_h(g, fh, kappa, O, nm, oh.data(), s.data(), tmp.data());
+  for (long i = 0; i < n; ++i) out[i] = oh[i] * fi;
+}
+
 // Conjugated inner product <a,b> = sum conj(a) b (bicgstab_dd_mix.cc:180-181).
 // Reductions accumulate in long double so that the oracle's norms are
 // insensitive to summation order at the 1e-14 level check.sh asks for.
@@ -418,6 +505,7 @@ struct Ctx {
   Fields<double> fd;
   Fields<float> fs;
   std::vector<float> Us, Cs;
+  std::vector<float> Uh, Ch;  // binary16 roundings of the FP32 fields
 };
 
 // u = M p in FP64: normalise, FP32 inner solve, prec_s_, rescale
@@ -573,6 +661,22 @@ void orc_prec_ddd_s(void* h, float* out, const float* in, int nsap, int nm) {
 void orc_prec_s(void* h, float* out, const float* in, int nsap, int nm) {
   Ctx* cx = (Ctx*)h;
   prec_s(cx->g, cx->fs, (float)cx->kappa, nsap, nm, out, in);
+}
+// M_SAP with FP16 storage (assign_mult_mSAP_s_h_ semantics: nsap - 1 sweeps);
+// the binary16 fields are the roundings of the current FP32 fields.
+void orc_msap_h(void* h, float* out, const float* in, int nsap, int nm) {
+  Ctx* cx = (Ctx*)h;
+  const long V = cx->g.V;
+  cx->Uh.resize(4 * V * 18);
+  cx->Ch.resize(V * 72);
+  for (long i = 0; i < 4 * V * 18; ++i) cx->Uh[i] = to_half(cx->fs.U[i]);
+  for (long i = 0; i < V * 72; ++i) cx->Ch[i] = to_half(cx->fs.C[i]);
+  Fields<float> fh = {cx->Uh.data(), cx->Ch.data()};
+  msap_h(cx->g, fh, (float)cx->kappa, nsap, nm, out, in);
+}
+// binary16 rounding of n floats (exposed so that it can be checked on its own)
+void orc_to_half(long n, const float* in, float* out) {
+  for (long i = 0; i < n; ++i) out[i] = to_half(in[i]);
 }
 // Same algorithms in FP64 (used to build the all-FP64 comparison solve).
 void orc_prec_ddd_d(void* h, double* out, const double* in, int nsap, int nm) {

@@ -65,6 +65,8 @@ def lib():
         L.qwsg_prec_s.argtypes = [vp, ctypes.c_int, vp, vp, ctypes.c_int, ctypes.c_int]
         L.qwsg_bicgstab_precdd.argtypes = [vp, ctypes.c_int, vp, vp, ctypes.c_double, ctypes.c_int, ctypes.c_int,
                                            ctypes.c_int, ip]
+        L.qwsg_msap_h.argtypes = [vp, vp, vp, ctypes.c_int, ctypes.c_int]
+        L.qwsg_bicgstab_precdd_h.argtypes = [vp, vp, vp, ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_int, ip]
         L.qwsg_solve.argtypes = [vp, vp, vp, ctypes.POINTER(SolveParams), ctypes.POINTER(SolveStats)]
         L.qwsg_norm2.argtypes = [vp, ctypes.c_int, vp, ctypes.POINTER(ctypes.c_double)]
         L.qwsg_solve_batch.argtypes = [vp, ctypes.c_int, vp, vp, ctypes.POINTER(SolveParams), ctypes.POINTER(SolveStats)]
@@ -241,6 +243,17 @@ class Lattice:
         it = ctypes.c_int()
         check(lib().qwsg_bicgstab_precdd(self.h, b.prec, x.ptr, b.ptr, float(tol), maxiter, nsap, nm,
                                          ctypes.byref(it)))
+        return it.value
+
+    def msap_h(self, out, inp, nsap, nm):
+        """out = M_SAP inp with FP16-stored SAP iterates and fields (FP32 vectors)."""
+        assert out.prec == F32 and inp.prec == F32
+        check(lib().qwsg_msap_h(self.h, out.ptr, inp.ptr, nsap, nm))
+
+    def bicgstab_precdd_h(self, x, b, tol, maxiter, nsap, nm):
+        assert x.prec == F32 and b.prec == F32
+        it = ctypes.c_int()
+        check(lib().qwsg_bicgstab_precdd_h(self.h, x.ptr, b.ptr, float(tol), maxiter, nsap, nm, ctypes.byref(it)))
         return it.value
 
     def solve(self, x, b, nsap=4, nm=2, tol=-1.0, maxiter=6, tol_s=-1.0, maxiter_s=50, true_tol=-1.0,

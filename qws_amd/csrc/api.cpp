@@ -141,6 +141,16 @@ int qwsg_bicgstab_precdd(qwsg_lattice* h, int prec, void* x, const void* b, doub
     if (iters) *iters = it;
   });
 }
+int qwsg_msap_h(qwsg_lattice* h, void* out, const void* in, int nsap, int nm) {
+  return guarded([&] { h->lat->msap_h((float*)out, (const float*)in, nsap, nm); });
+}
+int qwsg_bicgstab_precdd_h(qwsg_lattice* h, void* x, const void* b, double tol, int maxiter, int nsap, int nm,
+                           int* iters) {
+  return guarded([&] {
+    int it = h->lat->bicgstab_precdd_h((float*)x, (const float*)b, tol, maxiter, nsap, nm);
+    if (iters) *iters = it;
+  });
+}
 int qwsg_solve(qwsg_lattice* h, void* x, const void* b, const qwsg_solve_params* p, qwsg_solve_stats* st) {
   return guarded([&] {
     qws::SolveParams sp;
@@ -506,10 +516,11 @@ void bicgstab_precdd_s_(scs_t* x, scs_t* b, double* tol, int* conviter, int* max
   });
 }
 
-void bicgstab_dd_mix_(scd_t* x, scd_t* b, double* tol, int* conviter, int* maxiter, double* tol_s, int* maxiter_s,
-                      int* nsap, int* nm) {
-  drop_in(__func__, [&] {
-    Lattice& L = lat_sync("bicgstab_dd_mix_");
+static void dd_mix_generic(const char* where, scd_t* x, const scd_t* b, const double* tol, int* conviter,
+                           const int* maxiter, const double* tol_s, const int* maxiter_s, const int* nsap, const int* nm,
+                           int sap_prec) {
+  drop_in(where, [&] {
+    Lattice& L = lat_sync(where);
     double* xx = (double*)dbuf(0, qws::P64);
     double* bb = (double*)dbuf(1, qws::P64);
     L.upload_spinor(qws::P64, xx, x, 1, 1);
@@ -521,11 +532,65 @@ void bicgstab_dd_mix_(scd_t* x, scd_t* b, double* tol, int* conviter, int* maxit
     p.maxiter_s = *maxiter_s;
     p.nsap = *nsap;
     p.nm = *nm;
+    p.sap_prec = sap_prec;
+    // bicgstab_dd_mix.cc:165 runs iter = 1..maxiter-1 and reports the last iter;
+    // bicgstab_dd_mix2_hf.cc:169 runs iter = 0..maxiter-1 and reports iter only on convergence
+    if (sap_prec == qws::P16) p.maxiter = *maxiter + 1;
     int inner = 0;
     int it = L.bicgstab_dd_mix(xx, bb, p, &inner);
     L.download_spinor(qws::P64, xx, x, 1, 1);
+    if (sap_prec != qws::P16) {
+      if (conviter) *conviter = it;
+    } else if (it < p.maxiter && conviter) {
+      *conviter = it - 1;
+    }
+  });
+}
+
+void bicgstab_dd_mix_(scd_t* x, scd_t* b, double* tol, int* conviter, int* maxiter, double* tol_s, int* maxiter_s,
+                      int* nsap, int* nm) {
+  dd_mix_generic("bicgstab_dd_mix_", x, b, tol, conviter, maxiter, tol_s, maxiter_s, nsap, nm, qws::P32);
+}
+
+// ---- half-precision variant (qws_h.cc, prec_ddd_s_h.h, bicgstab_precdd_s_h.cc, bicgstab_dd_mix2_hf.cc)
+void qws_h_init_(const glus_t* u, const clvs_t* c) {
+  // The FP16 copies are made on the device from the FP32 fields the library holds;
+  // the reference takes the FP32 arrays it copies here (qws_h.cc:150-166).
+  if (u != glus || c != clvs) fatal("qws_h_init_", "expects the library's glus and clvs arrays");
+  drop_in(__func__, [&] { lat_sync("qws_h_init_"); });
+}
+
+void assign_mult_mSAP_s_h_(scs_t* xs, const scs_t* bs, const int* nsap, const int* nm) {
+  if (*nsap > 10) { std::printf("%s : %s nsap > 10\n", __FILE__, __func__); std::exit(1); }
+  if (*nm > 10) { std::printf("%s : %s nn   > 10\n", __FILE__, __func__); std::exit(1); }
+  drop_in(__func__, [&] {
+    Lattice& L = lat_sync("assign_mult_mSAP_s_h_");
+    float* a = (float*)dbuf(2, qws::P32);
+    float* b = (float*)dbuf(3, qws::P32);
+    L.upload_spinor(qws::P32, a, bs, 1, 0);
+    L.msap_h(b, a, *nsap, *nm);
+    L.download_spinor(qws::P32, b, xs, 1, 0);
+  });
+}
+
+void assign_mult_wd_s_(scs_t* x, const scs_t* b) { ddd_s_(x, const_cast<scs_t*>(b)); }
+
+void bicgstab_precdd_s_h_(scs_t* x, const scs_t* b, const double* tol, int* conviter, const int* maxiter,
+                          const int* nsap, const int* nm) {
+  drop_in(__func__, [&] {
+    Lattice& L = lat_sync("bicgstab_precdd_s_h_");
+    float* bb = (float*)dbuf(2, qws::P32);
+    float* xx = (float*)dbuf(3, qws::P32);
+    L.upload_spinor(qws::P32, bb, b, 1, 0);
+    int it = L.bicgstab_precdd_h(xx, bb, *tol, *maxiter, *nsap, *nm);
+    L.download_spinor(qws::P32, xx, x, 1, 0);
     if (conviter) *conviter = it;
   });
+}
+
+void bicgstab_dd_mix2_hf_(scd_t* x, const scd_t* b, const double* tol, int* conviter, const int* maxiter,
+                          const double* tol_s, const int* maxiter_s, const int* nsap, const int* nm) {
+  dd_mix_generic("bicgstab_dd_mix2_hf_", x, b, tol, conviter, maxiter, tol_s, maxiter_s, nsap, nm, qws::P16);
 }
 
 // clover inverse on the first vold elements with the clover of block `pe`

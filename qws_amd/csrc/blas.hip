@@ -331,6 +331,34 @@ __global__ void __launch_bounds__(RED_THREADS) k_convert(long long V, SO* out, c
   }
 }
 
+// elementwise real conversion of a field (the FP32 and FP16 field layouts share their order)
+template <typename SO, typename SI>
+__global__ void __launch_bounds__(RED_THREADS) k_convert_reals(long long n, SO* out, const SI* in) {
+  GRID_STRIDE(i, n) out[i] = (SO)(float)in[i];
+}
+
+// Scaling between an FP32 vector and its FP16 copy with unit RMS component:
+// sigma = sqrt(24 V / |in|^2) from sc[r].red[0] = |in_r|^2 (sigma = 0 for a zero
+// vector).  dir > 0: out = sigma * in ; dir < 0: out = in / sigma.
+template <typename SO, typename SI>
+__global__ void __launch_bounds__(RED_THREADS)
+k_scale_rms(long long V, long long so, long long si, const int* active, SO* out, const SI* in, const Scalars* sc,
+            int dir) {
+  if (rhs_off(active)) return;
+  const double n2 = sc[blockIdx.y].red[0];
+  const double sigma = n2 > 0 ? sqrt(24.0 * (double)V / n2) : 0.0;
+  const float f = dir > 0 ? (float)sigma : (sigma > 0 ? (float)(1.0 / sigma) : 0.0f);
+  SO* O = out + blockIdx.y * so;
+  const SI* I = in + blockIdx.y * si;
+  GRID_STRIDE(s, V) {
+    Cx<float> v[12];
+    SpinorIO<SI>::load(I, s, V, v);
+#pragma unroll
+    for (int k = 0; k < 12; ++k) v[k] = {v[k].re * f, v[k].im * f};
+    SpinorIO<SO>::store(O, s, V, v);
+  }
+}
+
 template <typename R>
 __global__ void __launch_bounds__(RED_THREADS) k_sub(long long n, R* out, const R* a, const R* b) {
   GRID_STRIDE(i, n) out[i] = a[i] - b[i];
@@ -388,6 +416,14 @@ void launch_s2d_scaled(const Batch& B, long long V, long long so, long long si, 
 template <typename SO, typename SI> void launch_convert(long long V, SO* out, const SI* in, hipStream_t st) {
   LAUNCH_B((k_convert<SO, SI>), 1, V, out, in);
 }
+template <typename SO, typename SI> void launch_convert_reals(long long n, SO* out, const SI* in, hipStream_t st) {
+  LAUNCH_B((k_convert_reals<SO, SI>), 1, n, out, in);
+}
+template <typename SO, typename SI>
+void launch_scale_rms(const Batch& B, long long V, long long so, long long si, SO* out, const SI* in, const Scalars* sc,
+                      int dir, hipStream_t st) {
+  LAUNCH_B((k_scale_rms<SO, SI>), B.n, V, so, si, B.active, out, in, sc, dir);
+}
 template <typename R> void launch_sub(long long n, R* out, const R* a, const R* b, hipStream_t st) {
   LAUNCH_B((k_sub<R>), 1, n, out, a, b);
 }
@@ -417,5 +453,10 @@ template void launch_convert<float, double>(long long, float*, const double*, hi
 template void launch_convert<double, float>(long long, double*, const float*, hipStream_t);
 template void launch_convert<__half, float>(long long, __half*, const float*, hipStream_t);
 template void launch_convert<float, __half>(long long, float*, const __half*, hipStream_t);
+template void launch_convert_reals<__half, float>(long long, __half*, const float*, hipStream_t);
+template void launch_scale_rms<__half, float>(const Batch&, long long, long long, long long, __half*, const float*,
+                                              const Scalars*, int, hipStream_t);
+template void launch_scale_rms<float, __half>(const Batch&, long long, long long, long long, float*, const __half*,
+                                              const Scalars*, int, hipStream_t);
 
 }  // namespace qws

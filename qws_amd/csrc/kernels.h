@@ -85,6 +85,12 @@ template <typename S>
 void launch_s2d_scaled(const Batch& B, long long V, long long so, long long si, double* out, const S* in,
                        const Scalars* sc, hipStream_t st);
 template <typename SO, typename SI> void launch_convert(long long V, SO* out, const SI* in, hipStream_t st);
+// elementwise conversion of n reals
+template <typename SO, typename SI> void launch_convert_reals(long long n, SO* out, const SI* in, hipStream_t st);
+// FP16 copy with unit RMS component (dir > 0) and back (dir < 0); sc[r].red[0] = |in_r|^2
+template <typename SO, typename SI>
+void launch_scale_rms(const Batch& B, long long V, long long so, long long si, SO* out, const SI* in, const Scalars* sc,
+                      int dir, hipStream_t st);
 // out = a - b (real arrays of length n)
 template <typename R> void launch_sub(long long n, R* out, const R* a, const R* b, hipStream_t st);
 

@@ -82,7 +82,7 @@ Lattice::~Lattice() {
   (void)hipStreamSynchronize(st_);
   batch_.reset();
   for (auto* p : {(void*)Ud_, (void*)Cd_, (void*)Us_, (void*)Cs_, (void*)Uh_, (void*)Ch_, (void*)part_, (void*)sc_in_,
-                  (void*)sc_one_, (void*)d_inner_mask_, (void*)d_blocks_[0], (void*)d_blocks_[1]})
+                  (void*)sc_one_, (void*)sc_h_, (void*)d_inner_mask_, (void*)d_blocks_[0], (void*)d_blocks_[1]})
     if (p) (void)hipFree(p);
   for (auto& w : work_) (void)hipFree(w.second.first);
   if (sc_host_) (void)hipHostFree(sc_host_);
@@ -127,6 +127,8 @@ size_t Lattice::field_bytes() const {
   if (Cd_) b += (size_t)g_.V * 72 * 8;
   if (Us_) b += (size_t)g_.V * 4 * nl32_ * 4;
   if (Cs_) b += (size_t)g_.V * 72 * 4;
+  if (Uh_) b += (size_t)g_.V * 4 * nl32_ * 2;
+  if (Ch_) b += (size_t)g_.V * 72 * 2;
   return b;
 }
 
@@ -230,9 +232,17 @@ void Lattice::point_spinor(int prec, void* dev, const int x[4], int spin, int co
   }
 }
 
+// FP16 copies of the FP32 links and clover inverse (assign_u_s2h, assign_clv_s2h,
+// qws_h.cc:230-269), made on first use after the FP32 fields change.
 void Lattice::ensure_half_fields() {
   if (half_valid_) return;
-  throw std::runtime_error("FP16 field storage is not enabled in this build");
+  if (!Us_ || !Cs_) throw std::runtime_error("FP16 fields are made from the FP32 fields, which are not loaded");
+  const long long nu = 4LL * nl32_ * g_.V, nc = 72LL * g_.V;
+  if (!Uh_) Uh_ = dalloc<__half>((size_t)nu);
+  if (!Ch_) Ch_ = dalloc<__half>((size_t)nc);
+  launch_convert_reals<__half, float>(nu, Uh_, Us_, st_);
+  launch_convert_reals<__half, float>(nc, Ch_, Cs_, st_);
+  half_valid_ = true;
 }
 
 void Lattice::upload_spinor(int prec, void* dev, const void* host, int layout, int host_is_double) {
@@ -441,6 +451,29 @@ void Lattice::prec_s_t(const Batch& B, S* out, const S* in, int nsap, int nm) {
   jinv_t<S>(B, 1, out, s, tmp, nm);
 }
 
+// assign_mult_mSAP_s_h_ (prec_ddd_s_h.h:63-231): out = M_SAP in with the SAP's
+// iterates, links and clover inverse stored in FP16 and FP32 arithmetic.  It runs
+// nsap - 1 colour sweeps and the final block solves (the sweep structure of
+// prec_s_) on an FP16 copy of `in` scaled to unit RMS component; the reference
+// scales to unit norm instead, which leaves large lattices' components in FP16's
+// subnormal range.
+void Lattice::msap_h_t(const Batch& B, float* out, const float* in, int nsap, int nm) {
+  if (nsap > 10 || nm > 10) throw std::invalid_argument("nsap and nm must be <= 10 (prec_ddd_s_h.h:130-131)");
+  ensure_half_fields();
+  const int nb = B.n;
+  if (!sc_h_) sc_h_ = dalloc<Scalars>(MAX_BATCH);
+  __half* ih = (__half*)work("hp_in", spinor_bytes(P16) * nb);
+  __half* oh = (__half*)work("hp_out", spinor_bytes(P16) * nb);
+  double* part = reduction_partials(nb);
+  launch_norm<float>(B, g_.V * 12, in, part, st_);
+  launch_finalize(B, part, 1, sc_h_, FIN_RAW, st_);
+  launch_scale_rms<__half, float>(B, g_.V, B.stride, B.stride, ih, in, sc_h_, +1, st_);
+  prec_s_t<__half>(B, oh, ih, nsap > 1 ? nsap - 1 : 0, nm);
+  launch_scale_rms<float, __half>(B, g_.V, B.stride, B.stride, out, oh, sc_h_, -1, st_);
+}
+
+void Lattice::msap_h(float* out, const float* in, int nsap, int nm) { msap_h_t(single(), out, in, nsap, nm); }
+
 void Lattice::prec_ddd(int prec, void* out, const void* in, int nsap, int nm) {
   if (nsap > 10 || nm > 10) throw std::invalid_argument("prec_ddd_s: nsap and nm must be <= 10 (qws.cc:798-799)");
   DISPATCH_PREC(prec, (prec_ddd_t<S>(single(), (S*)out, (const S*)in, nsap, nm)));
@@ -478,9 +511,13 @@ double Lattice::norm2(int prec, const void* a) {
 // Right-preconditioned BiCGStab for (D M_SAP) x = b, x0 = 0 (bicgstab_precdd_s.cc:85-559),
 // for every RHS k with live[k] != 0.  B.active must point at d_inner_mask_; an RHS
 // leaves the batch when its recursive residual meets tol (tol < 0: fixed count).
+//
+// hp: the method of bicgstab_precdd_s_h.cc:83-370 instead -- w = M_SAP p with the
+// FP16 SAP, q = D w with the FP32 operator, and x accumulates w directly (x is the
+// solution of D x = b, no final M_SAP).
 template <typename S>
 void Lattice::bicg_t(const Batch& Bin, S* x, const S* b, double tol, int maxiter, int nsap, int nm,
-                     std::vector<int>& iters) {
+                     std::vector<int>& iters, bool hp) {
   static_assert(sizeof(S) == 4, "the inner solver runs on FP32 vectors");
   const int nb = Bin.n;
   const long long n = g_.V * 12;
@@ -490,6 +527,8 @@ void Lattice::bicg_t(const Batch& Bin, S* x, const S* b, double tol, int maxiter
   S* p = (S*)work("bicg_p", bytes);
   S* t = (S*)work("bicg_t", bytes);
   S* r0 = (S*)work("bicg_r0", bytes);
+  S* w = hp ? (S*)work("bicg_w", bytes) : nullptr;
+  const float mk = (float)(-kappa_);
   if (nb > sc_in_n_) {
     if (sc_in_) QWS_HIP(hipFree(sc_in_));
     sc_in_ = dalloc<Scalars>(nb);
@@ -535,16 +574,26 @@ void Lattice::bicg_t(const Batch& Bin, S* x, const S* b, double tol, int maxiter
   };
   for (int iter = 0; iter < maxiter && any_live(); ++iter) {
     for (int k = 0; k < nb; ++k) iters[k] += live[k];
-    prec_ddd_t<S>(B, q, p, nsap, nm);
+    if (hp) {
+      msap_h_t(B, w, p, nsap, nm);
+      hop<S>(sel(-1), B, HOPS_ALL, mk, q, w, nullptr, nullptr, w);
+    } else {
+      prec_ddd_t<S>(B, q, p, nsap, nm);
+    }
     launch_dot<S>(B, n, r0, q, part, st_);
     launch_finalize(B, part, 2, sc, FIN_ALPHA, st_);
-    launch_update_xr<S>(B, n, x, p, r, q, sc, 0, nullptr, part, st_);
+    launch_update_xr<S>(B, n, x, hp ? w : p, r, q, sc, 0, nullptr, part, st_);
     launch_finalize(B, part, 1, sc, FIN_RNORM, st_);
     if (!converge()) break;
-    prec_ddd_t<S>(B, t, r, nsap, nm);
+    if (hp) {
+      msap_h_t(B, w, r, nsap, nm);
+      hop<S>(sel(-1), B, HOPS_ALL, mk, t, w, nullptr, nullptr, w);
+    } else {
+      prec_ddd_t<S>(B, t, r, nsap, nm);
+    }
     launch_norm_dot<S>(B, n, t, r, part, st_);
     launch_finalize(B, part, 3, sc, FIN_OMEGA, st_);
-    launch_update_xr<S>(B, n, x, r, r, t, sc, 1, r0, part, st_);
+    launch_update_xr<S>(B, n, x, hp ? w : r, r, t, sc, 1, r0, part, st_);
     launch_finalize(B, part, 3, sc, FIN_RHO, st_);
     if (!converge()) break;
     launch_update_p<S>(B, n, p, r, q, sc, st_);
@@ -554,7 +603,13 @@ void Lattice::bicg_t(const Batch& Bin, S* x, const S* b, double tol, int maxiter
 int Lattice::bicgstab_precdd(int prec, void* x, const void* b, double tol, int maxiter, int nsap, int nm) {
   if (prec != P32) throw std::invalid_argument("bicgstab_precdd: FP32 vectors only");
   std::vector<int> it(1, 0);
-  bicg_t<float>(Batch{1, g_.V * 24, d_inner_mask_}, (float*)x, (const float*)b, tol, maxiter, nsap, nm, it);
+  bicg_t<float>(Batch{1, g_.V * 24, d_inner_mask_}, (float*)x, (const float*)b, tol, maxiter, nsap, nm, it, false);
+  return it[0];
+}
+
+int Lattice::bicgstab_precdd_h(float* x, const float* b, double tol, int maxiter, int nsap, int nm) {
+  std::vector<int> it(1, 0);
+  bicg_t<float>(Batch{1, g_.V * 24, d_inner_mask_}, x, b, tol, maxiter, nsap, nm, it, true);
   return it[0];
 }
 
@@ -577,8 +632,13 @@ void Lattice::approx_inv(const Batch& B, Scalars* sc, const std::vector<int>& li
   launch_d2s_scaled<float>(B, g_.V, B.stride, B.stride, bs, p, sc, st_);
   std::vector<int> it(nb);
   for (int k = 0; k < nb; ++k) it[k] = live[k] ? 0 : -1;
-  bicg_t<float>(Batch{nb, g_.V * 24, d_inner_mask_}, ts, bs, prm.tol_s, prm.maxiter_s, prm.nsap, prm.nm, it);
-  prec_s_t<float>(B, xs, ts, prm.nsap, prm.nm);
+  if (prm.sap_prec == P16) {
+    // approx_inv_dirac_op_hf (bicgstab_dd_mix2_hf.cc:88-106): the inner solution is x itself
+    bicg_t<float>(Batch{nb, g_.V * 24, d_inner_mask_}, xs, bs, prm.tol_s, prm.maxiter_s, prm.nsap, prm.nm, it, true);
+  } else {
+    bicg_t<float>(Batch{nb, g_.V * 24, d_inner_mask_}, ts, bs, prm.tol_s, prm.maxiter_s, prm.nsap, prm.nm, it, false);
+    prec_s_t<float>(B, xs, ts, prm.nsap, prm.nm);
+  }
   launch_s2d_scaled<float>(B, g_.V, B.stride, B.stride, u, xs, sc, st_);
   for (int k = 0; k < nb; ++k)
     if (live[k]) inner[k] += it[k];

@@ -126,6 +126,10 @@ class Lattice {
   void prec_ddd(int prec, void* out, const void* in, int nsap, int nm);
   void prec_s(int prec, void* out, const void* in, int nsap, int nm);
   int bicgstab_precdd(int prec, void* x, const void* b, double tol, int maxiter, int nsap, int nm);
+  // FP16-storage SAP (assign_mult_mSAP_s_h_) and the inner solver built on it
+  // (bicgstab_precdd_s_h_); FP32 vectors in and out
+  void msap_h(float* out, const float* in, int nsap, int nm);
+  int bicgstab_precdd_h(float* x, const float* b, double tol, int maxiter, int nsap, int nm);
   int bicgstab_dd_mix(double* x, const double* b, const SolveParams& p, int* inner_total);
   SolveStats solve(double* x, const double* b, const SolveParams& p);
   // n independent right-hand sides solved together (x[k] holds the initial guess)
@@ -157,8 +161,10 @@ class Lattice {
   void sap_sweeps_fused(const Batch& B, const float* in, float* s, float* q, int nsap, int nm);
   template <typename S> void prec_ddd_t(const Batch& B, S* out, const S* in, int nsap, int nm);
   template <typename S> void prec_s_t(const Batch& B, S* out, const S* in, int nsap, int nm);
+  void msap_h_t(const Batch& B, float* out, const float* in, int nsap, int nm);
   template <typename S>
-  void bicg_t(const Batch& B, S* x, const S* b, double tol, int maxiter, int nsap, int nm, std::vector<int>& iters);
+  void bicg_t(const Batch& B, S* x, const S* b, double tol, int maxiter, int nsap, int nm, std::vector<int>& iters,
+              bool hp);
   double read_scalar(const Scalars* sc, size_t offset);
 
   Geo g_;
@@ -181,6 +187,7 @@ class Lattice {
   int sc_in_n_ = 0;
   Scalars* sc_one_ = nullptr;
   Scalars* sc_host_ = nullptr;  // pinned, room for MAX_BATCH
+  Scalars* sc_h_ = nullptr;     // FP16 SAP input norms, room for MAX_BATCH
   int* d_inner_mask_ = nullptr;
   std::vector<std::pair<std::string, std::pair<void*, size_t>>> work_;
 

@@ -56,12 +56,12 @@ template <> __device__ __forceinline__ double load_real<__half>(const __half* p,
 template <typename S>
 __device__ __forceinline__ long long dev_offset(const Geo& g, int kind, int nl, long long s, int mu, int k) {
   if (kind == 0) {
-    if (sizeof(S) == 4) return ((long long)(k >> 2) * g.V + s) * 4 + (k & 3);  // float4 [6][V]
-    return ((long long)(k >> 1) * g.V + s) * 2 + (k & 1);                     // Vec2 [12][V]
+    if (sizeof(S) != 8) return ((long long)(k >> 2) * g.V + s) * 4 + (k & 3);  // float4 / Half4 [6][V]
+    return ((long long)(k >> 1) * g.V + s) * 2 + (k & 1);                     // double2 [12][V]
   }
   if (kind == 1) return (((long long)mu * (nl / 2) + (k >> 1)) * g.V + s) * 2 + (k & 1);
-  if (sizeof(S) == 4) return ((long long)(k >> 2) * g.V + s) * 4 + (k & 3);  // float4 [18][V]
-  return ((long long)(k >> 1) * g.V + s) * 2 + (k & 1);                       // Vec2 [36][V]
+  if (sizeof(S) != 8) return ((long long)(k >> 2) * g.V + s) * 4 + (k & 3);  // float4 / Half4 [18][V]
+  return ((long long)(k >> 1) * g.V + s) * 2 + (k & 1);                       // double2 [36][V]
 }
 
 template <typename S>

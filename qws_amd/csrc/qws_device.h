@@ -9,12 +9,13 @@
 //   spinor FP32  : float4  [6][V]   float4 j = complex components 2j, 2j+1,
 //                                    component k = colour*4 + spin (qws.h:170-177 order)
 //   spinor FP64  : double2 [12][V]
-//   spinor FP16  : __half2 [12][V] (storage only; arithmetic in FP32)
+//   spinor FP16  : Half4   [6][V]   same component order as FP32 (storage only;
+//                                    arithmetic in FP32)
 //   link         : Vec2    [4][NL/2][V]  complex c[i][j] at k = i*3 + j, link from s to
 //                                    s+mu; NL = 18 stores 3 rows, NL = 12 stores rows 0,1
 //                                    and rebuilds row 2 = conj(row0 x row1)
 //   clover (T^-1): FP32 float4 [18][V] (72 reals), FP64 double2 [36][V],
-//                  FP16 __half2 [36][V]; per site chirality i at reals 36i..36i+35 in
+//                  FP16 Half4 [18][V]; per site chirality i at reals 36i..36i+35 in
 //                  QWS's packing (clover_def.h:88-284)
 // Every kernel is written for 64-wide wavefronts; one thread owns one site.
 #pragma once
@@ -158,19 +159,33 @@ template <> struct SpinorIO<double> {
   }
 };
 
+// four FP16 values in 8 bytes, ordered like the components of a float4
+struct alignas(8) Half4 { __half2 a, b; };
+__device__ __forceinline__ float4 half4_load(const __half* base, long long i) {
+  const Half4 h = reinterpret_cast<const Half4*>(base)[i];
+  const float2 x = __half22float2(h.a), y = __half22float2(h.b);
+  return make_float4(x.x, x.y, y.x, y.y);
+}
+__device__ __forceinline__ void half4_store(__half* base, long long i, float4 f) {
+  Half4 h;
+  h.a = __floats2half2_rn(f.x, f.y);
+  h.b = __floats2half2_rn(f.z, f.w);
+  reinterpret_cast<Half4*>(base)[i] = h;
+}
+
 template <> struct SpinorIO<__half> {
   static __device__ __forceinline__ void load(const __half* base, long long s, long long V, Cx<float> v[12]) {
-    const __half2* p = reinterpret_cast<const __half2*>(base);
 #pragma unroll
-    for (int k = 0; k < 12; ++k) {
-      float2 q = __half22float2(p[k * V + s]);
-      v[k] = {q.x, q.y};
+    for (int j = 0; j < 6; ++j) {
+      float4 q = half4_load(base, j * V + s);
+      v[2 * j] = {q.x, q.y};
+      v[2 * j + 1] = {q.z, q.w};
     }
   }
   static __device__ __forceinline__ void store(__half* base, long long s, long long V, const Cx<float> v[12]) {
-    __half2* p = reinterpret_cast<__half2*>(base);
 #pragma unroll
-    for (int k = 0; k < 12; ++k) p[k * V + s] = __floats2half2_rn(v[k].re, v[k].im);
+    for (int j = 0; j < 6; ++j)
+      half4_store(base, j * V + s, make_float4(v[2 * j].re, v[2 * j].im, v[2 * j + 1].re, v[2 * j + 1].im));
   }
 };
 
@@ -304,11 +319,10 @@ template <> struct CloverIO<double> {
 };
 template <> struct CloverIO<__half> {
   static __device__ __forceinline__ void load_chir(const __half* C, long long s, long long V, int i, float a[36]) {
-    const __half2* p = reinterpret_cast<const __half2*>(C);
 #pragma unroll
-    for (int f = 0; f < 18; ++f) {
-      float2 q = __half22float2(p[(long long)(18 * i + f) * V + s]);
-      a[2 * f] = q.x; a[2 * f + 1] = q.y;
+    for (int f = 0; f < 9; ++f) {
+      float4 q = half4_load(C, (long long)(9 * i + f) * V + s);
+      a[4 * f] = q.x; a[4 * f + 1] = q.y; a[4 * f + 2] = q.z; a[4 * f + 3] = q.w;
     }
   }
 };

@@ -279,3 +279,37 @@ def test_prec_ddd_equals_d_prec(L, B):
     a = orc.prec_ddd_s(ps, 4, 2)
     bb = orc.dslash(orc.prec_s(ps, 4, 2))
     assert _rel(a, bb) < 2e-5
+
+
+# ---------------------------------------------------------------- FP16 storage (prec_ddd_s_h.h)
+def test_to_half_is_ieee_binary16():
+    """The oracle's binary16 rounding equals numpy's IEEE float16 conversion
+    (round to nearest even, subnormals, overflow to infinity)."""
+    rng = np.random.default_rng(1)
+    x = rng.standard_normal(50000) * 10.0 ** rng.uniform(-9, 5, 50000)
+    edges = [65504, 65519.9, 65520, 1e-8, 2.0 ** -24, 3 * 2.0 ** -25, 2.0 ** -25, 0.0, -0.0, 1 + 2.0 ** -11,
+             1 + 3 * 2.0 ** -11, 6.1e-5, -6.2e-5]
+    x = np.concatenate([x, edges]).astype(np.float32)
+    with np.errstate(over="ignore"):
+        want = x.astype(np.float16).astype(np.float32)
+    assert np.array_equal(O.to_half(x), want)
+
+
+@pytest.mark.parametrize("L,B", [([8, 4, 4, 8], [2, 2, 2, 2]), ([8, 2, 2, 4], [4, 2, 2, 2])])
+def test_msap_h_is_fp16_prec_s_with_one_sweep_less(L, B):
+    """assign_mult_mSAP_s_h_(nsap) is prec_s_ with nsap-1 sweeps (prec_ddd_s_h.h:133 vs
+    qws.cc:955): with FP16 storage it stays within FP16 rounding of the FP32
+    prec_s_(nsap-1) and well away from prec_s_(nsap); the scaling to unit RMS makes
+    it homogeneous (M(a b) = a M(b) up to rounding)."""
+    V = int(np.prod(L))
+    rng = np.random.default_rng(3)
+    U, C = K.random_su3_like(V, rng)
+    orc = O.Oracle(L, B, -1, 0.12, U, C)
+    b = rng.standard_normal((V, 3, 4, 2)).astype(np.float32)
+    rel = lambda a, c: float(np.linalg.norm(a - c) / np.linalg.norm(c))
+    for nsap, nm in [(4, 2), (2, 1), (1, 3)]:
+        h = orc.msap_h(b, nsap, nm)
+        assert 1e-5 < rel(h, orc.prec_s(b, nsap - 1, nm)) < 3e-3
+        assert rel(h, orc.prec_s(b, nsap, nm)) > 1e-2
+    h1 = orc.msap_h(b, 3, 2)
+    assert np.array_equal(orc.msap_h(b * np.float32(1024.0), 3, 2), h1 * np.float32(1024.0))
