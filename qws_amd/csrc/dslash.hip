@@ -18,7 +18,7 @@ k_hop(Geo g, const int* __restrict__ list, int wg_per_blk, int nrhs, long long s
       typename Store<S>::R gamma, S* out, const S* a, const S* b, const S* c, const S* __restrict__ in,
       const S* __restrict__ U, const S* __restrict__ C, int skip_interior) {
   using R = typename Store<S>::R;
-  const int wg = xcd_remap(blockIdx.x, gridDim.x);
+  const int wg = xcd_remap(g, blockIdx.x, gridDim.x);
   const int bi = wg / wg_per_blk;
   const int blk = list ? list[bi] : bi;
   const int loc = (wg - bi * wg_per_blk) * HOP_THREADS + threadIdx.x;
@@ -36,9 +36,12 @@ k_hop(Geo g, const int* __restrict__ list, int wg_per_blk, int nrhs, long long s
   for (int r = 0; r < nrhs; ++r) {
     if (active && !active[r]) continue;
     const long long off = r * stride;
+    const long long sr = opaque_v(s);
+    SiteC scr;
+    site_coords(g, sr, scr);
     Cx<R> acc[12];
-    hop_sum<S, NL, SET>(g, s, sc, in + off, U, acc);
-    clover_apply<S>(C, s, g.V, acc);
+    hop_sum<S, NL, SET>(g, sr, scr, in + off, opaque(U), acc);
+    clover_apply<S>(opaque(C), sr, g.V, acc);
     Cx<R> v[12];
     if (a) {
       SpinorIO<S>::load(a + off, s, g.V, v);

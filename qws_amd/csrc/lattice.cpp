@@ -6,6 +6,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstddef>
+#include <cstdlib>
 #include <cstring>
 #include <type_traits>
 
@@ -55,6 +56,9 @@ Lattice::Lattice(const int L[4], const int B[4], int tbc, int links32) {
   g_.V = V;
   g_.tbc = tbc;
   g_.hopmask = 0xFF;
+  // workgroup placement: XCD-contiguous block ranges unless QWS_MI355X_XCD_REMAP=0
+  const char* xr = std::getenv("QWS_MI355X_XCD_REMAP");
+  g_.xcd_remap = (xr && xr[0] == '0') ? 0 : 1;
   nl32_ = links32;
   QWS_HIP(hipStreamCreate(&st_));
   // block lists per colour (colour = (bx+by+bz+bt)%2, qws.cc:275-276)

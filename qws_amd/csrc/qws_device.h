@@ -40,16 +40,32 @@ struct Geo {
   int hopmask;    // bit 2*mu + (dir < 0): hop taken by the full operator (0xFF); being a
                   // runtime value it keeps each hop in its own basic block, so the
                   // compiler cannot hoist all eight neighbour loads (256 -> ~146 VGPRs)
+  int xcd_remap;  // 1: map workgroups to blocks XCD by XCD (xcd_remap), 0: identity
 };
 
 // MI355X dispatches the workgroups of a launch round-robin over its 8 XCDs, each
 // with its own L2.  Give XCD x a contiguous range of logical workgroup ids (blocks
 // that share faces then share an L2): a bijection of [0, nwg).
 constexpr int NUM_XCD = 8;
-__device__ __forceinline__ int xcd_remap(int wg, int nwg) {
+__device__ __forceinline__ int xcd_remap(const Geo& g, int wg, int nwg) {
+  if (!g.xcd_remap) return wg;
   const int q = nwg / NUM_XCD, r = nwg % NUM_XCD;
   const int x = wg % NUM_XCD, i = wg / NUM_XCD;
   return x * q + (x < r ? x : r) + i;
+}
+
+// p, hidden from the optimiser.  Loads through the result taken inside a loop over
+// right-hand sides are not loop-invariant to the compiler, so it does not hoist a
+// site's links and clover out of the loop and keep them all in registers (which
+// costs occupancy; the re-reads hit the cache).
+template <typename T> __device__ __forceinline__ const T* opaque(const T* p) {
+  asm volatile("" : "+s"(p));
+  return p;
+}
+// the same for a per-thread value (everything derived from it is recomputed)
+__device__ __forceinline__ long long opaque_v(long long x) {
+  asm volatile("" : "+v"(x));
+  return x;
 }
 
 struct SiteC {

@@ -100,7 +100,7 @@ k_sap_block(Geo g, const int* __restrict__ list, int nrhs, long long stride, con
   __shared__ float lU[4 * NC * 2 * SAP_THREADS];
   __shared__ float lx[24 * SAP_THREADS];
   const int t = threadIdx.x;
-  const int blk = list[xcd_remap(blockIdx.x, gridDim.x)];
+  const int blk = list[xcd_remap(g, blockIdx.x, gridDim.x)];
   const long long site = (long long)blk * SAP_THREADS + t;
   SiteC c;
   site_coords(g, site, c);
@@ -129,7 +129,7 @@ k_sap_block(Geo g, const int* __restrict__ list, int nrhs, long long stride, con
     SpinorIO<float>::load(((flags & SAP_SRC_IN) ? in : s) + off, site, g.V, sv);
 
     if (flags & SAP_FACE_PRO) {
-      hop_sum<float, NL, HOPS_FACE>(g, site, c, qface + off, U, f);
+      hop_sum<float, NL, HOPS_FACE>(g, site, c, qface + off, opaque(U), f);
       clover_regs(cl, f);
 #pragma unroll
       for (int k = 0; k < 12; ++k) sv[k] = cadd(sv[k], cscale(kappa, f[k]));
