@@ -109,6 +109,48 @@ void ddd_out_pos_s_(scs_t* out, scs_t* in, int* domain, float factor);
 /* qws.cc:390,429 -- clover inverse multiply on one parity/domain block of vold sites */
 void clvd_vm_(int* pe, scd_t* inout);
 void clvd_vm2_(int* pe, scd_t* out, scd_t* in);
+/* Even/odd family (main.cc:554-721 lists them).  The host arrays glud/glus, clvd/clvs
+ * are read as two parity halves of x-compressed sites (qws_bqcd.cc:82-85): a field
+ * of parity p holds vold (FP64) or vols (FP32) vectors, vector i = xh/VLEN +
+ * nxd*(y + ny*(z + nz*t)), lane xh % VLEN.  out has parity *pe, in parity *po.
+ * "_in_": the hops that stay inside the lattice; "_out_pre_/_pos_": the hops that
+ * wrap round it (pre records in, pos adds factor times them to out); "dee_": T^-1
+ * applied; "dag": H^+; "one_minus_": out = in0 + factor (...).
+ * deo_in_d.cc:65,191,319; deo_out_d.cc:88,156,220; deo_dag_in.cc:64,190,317;
+ * deo_dag_out.cc:88,156,220; qws.cc:518-576 (composites, mtilde with pce/pco). */
+void deo_in_vm_(int* pe, int* po, scd_t* out, scd_t* in);
+void dee_deo_in_vm_(int* pe, int* po, scd_t* out, scd_t* in);
+void deo_dag_in_vm_(int* pe, int* po, scd_t* out, scd_t* in);
+void dee_deo_dag_in_vm_(int* pe, int* po, scd_t* out, scd_t* in);
+void one_minus_dee_deo_in_vm_(int* pe, int* po, scd_t* out, scd_t* in, scd_t* in0, double factor);
+void one_minus_deo_dag_in_vm_(int* pe, int* po, scd_t* out, scd_t* in, scd_t* in0, double factor);
+void deo_out_pre_vm_(int* pe, int* po, scd_t* out, scd_t* in);
+void deo_dag_out_pre_vm_(int* pe, int* po, scd_t* out, scd_t* in);
+void deo_out_pos_vm_(int* pe, int* po, scd_t* out, scd_t* in, double factor);
+void deo_dag_out_pos_vm_(int* pe, int* po, scd_t* out, scd_t* in, double factor);
+void dee_deo_out_pos_vm_(int* pe, int* po, scd_t* out, scd_t* in, double factor);
+void dee_deo_dag_out_pos_vm_(int* pe, int* po, scd_t* out, scd_t* in, double factor);
+void deo_vm_(int* pe, int* po, scd_t* out, scd_t* in);
+void dee_deo_vm_(int* pe, int* po, scd_t* out, scd_t* in);
+void one_minus_dee_deo_vm_(int* pe, int* po, scd_t* out, scd_t* in, scd_t* in0);
+void deo_dag_vm_(int* pe, int* po, scd_t* out, scd_t* in);
+void dee_deo_dag_vm_(int* pe, int* po, scd_t* out, scd_t* in);
+void one_minus_deo_dag_vm_(int* pe, int* po, scd_t* out, scd_t* in, scd_t* in0);
+void mtilde_vm_(scd_t* out, scd_t* in);
+void mtilde_dag_vm_(scd_t* out, scd_t* in);
+void mtdagmt_vm_(scd_t* out, scd_t* in);
+/* FP32: deo_in_s.cc:65,191,317; deo_out_s.cc:93,155,229; qws.cc:576-605; clover_s.cc (clv_s_) */
+void deo_in_s_(int* pe, int* po, scs_t* out, scs_t* in);
+void dee_deo_in_s_(int* pe, int* po, scs_t* out, scs_t* in);
+void one_minus_dee_deo_in_s_(int* pe, int* po, scs_t* out, scs_t* in, scs_t* in0, float factor);
+void deo_out_pre_s_(const int* pe, const int* po, const scs_t* out, const scs_t* in);
+void deo_out_pos_s_(const int* pe, const int* po, const scs_t* out, const scs_t* in, float factor);
+void dee_deo_out_pos_s_(int* pe, int* po, scs_t* out, scs_t* in, float factor);
+void clv_s_(int* pe, scs_t* inout);
+void deo_s_(int* pe, int* po, scs_t* out, scs_t* in);
+void dee_deo_s_(int* pe, int* po, scs_t* out, scs_t* in);
+void one_minus_dee_deo_s_(int* pe, int* po, scs_t* out, scs_t* in, scs_t* in0);
+void mtilde_s_(scs_t* out, scs_t* in);
 /* reordering.cc:141,108 -- FP64 <-> FP32 with normalisation */
 void fermi_reorder_d2s_dd_renorm_(scs_t* out, const scd_t* in, double* dnorm);
 void fermi_reorder_s2d_dd_renorm_(scd_t* out, const scs_t* in, const double* dnorm);
@@ -215,6 +257,25 @@ int qwsg_solve(qwsg_lattice* lat, void* x, const void* b, const qwsg_solve_param
  * to solving it alone.  st receives n entries. */
 int qwsg_solve_batch(qwsg_lattice* lat, int n, void* const* x, const void* const* b, const qwsg_solve_params* p,
                      qwsg_solve_stats* st);
+/* Even/odd family on x-compressed parity fields of V/2 sites h = xh + (Lx/2)(y + Ly(z + Lz t))
+ * (host layout: spinor [V/2][24]; links [2][4][V/2][18], link from site h of parity p;
+ * clover [2][V/2][72]).  Fields come from host arrays or from the lattice's own
+ * fields (all extents even; parity-p site h is x = 2 xh + (p + y + z + t) % 2).
+ * qwsg_deo: out (parity pe) from in (parity 1-pe); set 0 all hops, 1 hops inside the
+ * lattice, 2 hops that wrap (t edge with 2*tbc), 3 none (in[h]); dag: H^+; clv: T^-1
+ * applied; mode 0 out = acc, 1 out += fac acc, 2 out = in0 + fac acc
+ * (deo_*_in/_out_*, dee_*, one_minus_*).  qwsg_mtilde: which 0 mtilde =
+ * 1 - kappa^2 T^-1 H T^-1 H on parity pce, 1 its adjoint, 2 mtilde^+ mtilde
+ * (qws.cc:551-576). */
+int qwsg_eo_load_gauge(qwsg_lattice* lat, const double* U);
+int qwsg_eo_load_clover(qwsg_lattice* lat, const double* C);
+int qwsg_eo_from_lattice(qwsg_lattice* lat);
+int qwsg_eo_spinor_alloc(qwsg_lattice* lat, int prec, void** dev);
+int qwsg_eo_spinor_upload(qwsg_lattice* lat, int prec, void* dev, const double* host);
+int qwsg_eo_spinor_download(qwsg_lattice* lat, int prec, const void* dev, double* host);
+int qwsg_deo(qwsg_lattice* lat, int prec, int pe, int set, int dag, int clv, int mode, double fac, void* out,
+             const void* in, const void* in0);
+int qwsg_mtilde(qwsg_lattice* lat, int prec, int pce, int which, void* out, const void* in);
 int qwsg_norm2(qwsg_lattice* lat, int prec, const void* a, double* out);
 int qwsg_synchronize(qwsg_lattice* lat);
 /* average device time (ms) of `reps` full FP32/FP16/FP64 dslash launches on the

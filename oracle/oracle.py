@@ -46,6 +46,8 @@ def lib():
         L.orc_prec_s.argtypes = [vp, fp, fp, ctypes.c_int, ctypes.c_int]
         L.orc_msap_h.argtypes = [vp, fp, fp, ctypes.c_int, ctypes.c_int]
         L.orc_to_half.argtypes = [ctypes.c_long, fp, fp]
+        L.orc_deo_d.argtypes = [vp, dp, dp] + [ctypes.c_int] * 5 + [ctypes.c_double, dp, dp, dp]
+        L.orc_deo_s.argtypes = [vp, fp, fp] + [ctypes.c_int] * 5 + [ctypes.c_float, fp, fp, fp]
         L.orc_prec_ddd_d.argtypes = [vp, dp, dp, ctypes.c_int, ctypes.c_int]
         L.orc_bicgstab_precdd_s.argtypes = [vp, fp, fp, ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_int]
         L.orc_bicgstab_dd_mix.argtypes = [vp, dp, dp, ctypes.c_double, ctypes.c_int, ctypes.c_double,
@@ -159,6 +161,26 @@ class Oracle:
         """M_SAP with FP16 storage (assign_mult_mSAP_s_h_: nsap - 1 sweeps)."""
         out = self._new(np.float32)
         lib().orc_msap_h(self.h, _p(out), _p(np.ascontiguousarray(psi, dtype=np.float32)), nsap, nm)
+        return out
+
+    def deo(self, Ueo, Ceo, pe, psi, set=0, dag=False, clv=False, mode=0, fac=1.0, out=None, in0=None):
+        """Even/odd hopping (qws_oracle.cpp deo) on x-compressed parity fields of V/2 sites:
+        Ueo (2,4,V/2,18), Ceo (2,V/2,72); psi the parity-(1-pe) field, the result the
+        parity-pe field.  set 0 all hops, 1 hops that do not wrap, 2 hops that wrap;
+        mode 0 out = acc, 1 out += fac acc, 2 out = in0 + fac acc.  FP64 or FP32 after
+        the dtype of psi."""
+        dt = np.float64 if np.asarray(psi).dtype == np.float64 else np.float32
+        vh = self.V // 2
+        psi = np.ascontiguousarray(psi, dtype=dt).reshape(vh, 24)
+        Ueo = np.ascontiguousarray(Ueo, dtype=dt)
+        Ceo = np.ascontiguousarray(Ceo, dtype=dt)
+        assert Ueo.size == 2 * 4 * vh * 18 and Ceo.size == 2 * vh * 72
+        out = np.zeros((vh, 24), dtype=dt) if out is None else out
+        if out.dtype != dt or not out.flags.c_contiguous or out.size != vh * 24:
+            raise ValueError("out must be a contiguous parity field of the input's dtype")
+        in0 = psi if in0 is None else np.ascontiguousarray(in0, dtype=dt).reshape(vh, 24)
+        f = lib().orc_deo_d if dt == np.float64 else lib().orc_deo_s
+        f(self.h, _p(Ueo), _p(Ceo), pe, set, int(dag), int(clv), mode, fac, _p(out), _p(psi), _p(in0))
         return out
 
     def bicgstab_precdd_s(self, b, tol, maxiter, nsap, nm):

@@ -106,14 +106,17 @@ inline void reconstruct(int mu, int dir, const cplx<R> b[2], R tf, cplx<R> o[4])
 //                                   backward R^+( W_mu(x-mu)^+ P^+ psi(x-mu) )
 // with W[i][j] = U.c[j][i]: __m_fg computes b_j = sum_i c[i][j] a_i
 // (wilson_d.h:110-135) and the dagger load conjugates and transposes
-// (wilson_d.h:176-187).
+// (wilson_d.h:176-187).  dag: the hop of H^+, whose projector and reconstruction
+// are those of the opposite direction (1 + gamma_mu forward, 1 - gamma_mu
+// backward) with the same link (deo_dag_in.cc:64-316).
 template <typename R>
-inline void hop(int mu, int dir, const R* u, const R* psi_n, R tf, cplx<R> acc[3][4]) {
+inline void hop(int mu, int dir, const R* u, const R* psi_n, R tf, cplx<R> acc[3][4], bool dag = false) {
+  const int pdir = dag ? -dir : dir;
   cplx<R> a[3][2], b[3][2];
   for (int c = 0; c < 3; ++c) {
     cplx<R> s[4];
     for (int sp = 0; sp < 4; ++sp) s[sp] = cplx<R>(psi_n[(c * 4 + sp) * 2], psi_n[(c * 4 + sp) * 2 + 1]);
-    project(mu, dir, s, a[c]);
+    project(mu, pdir, s, a[c]);
   }
   for (int j = 0; j < 3; ++j)
     for (int h = 0; h < 2; ++h) {
@@ -125,7 +128,7 @@ inline void hop(int mu, int dir, const R* u, const R* psi_n, R tf, cplx<R> acc[3
       }
       b[j][h] = v;
     }
-  for (int c = 0; c < 3; ++c) reconstruct(mu, dir, b[c], tf, acc[c]);
+  for (int c = 0; c < 3; ++c) reconstruct(mu, pdir, b[c], tf, acc[c]);
 }
 
 // Clover-inverse multiply (clover_def.h:88-284): per chirality i the 6-vector
@@ -422,6 +425,79 @@ void msap_h(const Geom& g, const Fields<float>& fh, float kappa, int nsap, int n
   for (long i = 0; i < n; ++i) out[i] = oh[i] * fi;
 }
 
+// ---------------------------------------------------------------- even/odd family
+// Even/odd hopping on one rank (deo_in_d.cc:65-448, deo_out_d.cc:88-290,
+// deo_dag_in.cc:64-448, deo_dag_out.cc:88-290, deo_in_s.cc, deo_out_s.cc), defined
+// as the reference defines it: by index arithmetic on the x-compressed parity
+// fields.  A parity-p field holds Vh = V/2 sites h = xh + (Lx/2)(y + Ly(z + Lz t));
+// its row parity is ick = (p + y + z + t) % 2 (deo_in_d.cc:80-85).  The x-forward
+// neighbour of h is xh (ick 0) or xh + 1 (ick 1) of the other parity, x-backward
+// xh - 1 (ick 0) or xh (ick 1); y/z/t neighbours keep xh.  Every index wraps
+// periodically; a hop "wraps" when it crosses an edge (x: xh = 0 backward with ick
+// 0, xh = Lx/2 - 1 forward with ick 1).  With even extents this is the physical
+// lattice with x = 2 xh + ick; with an odd extent (data/CASE0 has Lt = 3) it is the
+// reference's twisted operator, reproduced as such.
+// Fields: U[p][mu][h][18] (link from site h of parity p), C[p][h][72].
+// For every site h of parity pe: acc = sum of the hops of `set` (0 all, 1 the
+// hops that do not wrap -- "_in_", 2 those that do -- "_out_pre/_pos", the t edge
+// with factor 2*tbc; 3 no hops: acc = in at h itself, for the clover multiply
+// clvd_vm2_) from the parity-(1-pe) field in; dag: H^+; clv: acc = T^-1 acc
+// ("dee_"); mode 0: out = acc, 1: out += fac acc, 2: out = in0 + fac acc.
+template <typename R>
+void deo(const Geom& g, const R* U, const R* C, int pe, int set, bool dag, bool clv, int mode, R fac, R* out,
+         const R* in, const R* in0) {
+  const int nxh = g.L[0] / 2, ny = g.L[1], nz = g.L[2];
+  const long Vh = g.V / 2;
+  const int po = 1 - pe;
+#pragma omp parallel for schedule(static)
+  for (long h = 0; h < Vh; ++h) {
+    long r = h;
+    const int xh = (int)(r % nxh); r /= nxh;
+    const int y = (int)(r % ny); r /= ny;
+    const int z = (int)(r % nz);
+    const int t = (int)(r / nz);
+    const int ick = (pe + y + z + t) & 1;
+    auto idx = [&](int a, int b, int c, int d) { return a + (long)nxh * (b + (long)ny * (c + (long)nz * d)); };
+    cplx<R> acc[3][4];
+    for (int cc = 0; cc < 3; ++cc)
+      for (int sp = 0; sp < 4; ++sp)
+        acc[cc][sp] = set == 3 ? cplx<R>(in[h * 24 + (cc * 4 + sp) * 2], in[h * 24 + (cc * 4 + sp) * 2 + 1])
+                               : cplx<R>(0, 0);
+    for (int mu = 0; mu < 4 && set != 3; ++mu)
+      for (int dir = 1; dir >= -1; dir -= 2) {
+        int n[4] = {xh, y, z, t};
+        bool wraps;
+        if (mu == 0) {
+          const int step = dir > 0 ? ick : ick - 1;  // xh shift of the neighbour
+          n[0] = xh + step;
+          wraps = n[0] < 0 || n[0] >= nxh;
+          n[0] = (n[0] + nxh) % nxh;
+        } else {
+          const int ext = g.L[mu];
+          n[mu] += dir;
+          wraps = n[mu] < 0 || n[mu] >= ext;
+          n[mu] = (n[mu] + ext) % ext;
+        }
+        if ((set == 1 && wraps) || (set == 2 && !wraps)) continue;
+        const long hn = idx(n[0], n[1], n[2], n[3]);
+        const R* u = dir > 0 ? U + (((long)pe * 4 + mu) * Vh + h) * 18 : U + (((long)po * 4 + mu) * Vh + hn) * 18;
+        R tf = 2;
+        if (mu == 3 && wraps) tf = (R)(2 * g.tbc);
+        hop(mu, dir, u, in + hn * 24, tf, acc, dag);
+      }
+    if (clv) clover_mult(C + ((long)pe * Vh + h) * 72, acc);
+    for (int cc = 0; cc < 3; ++cc)
+      for (int sp = 0; sp < 4; ++sp)
+        for (int ri = 0; ri < 2; ++ri) {
+          const long k = h * 24 + (cc * 4 + sp) * 2 + ri;
+          const R v = ri ? acc[cc][sp].imag() : acc[cc][sp].real();
+          if (mode == 0) out[k] = v;
+          else if (mode == 1) out[k] += fac * v;
+          else out[k] = in0[k] + fac * v;
+        }
+  }
+}
+
 // Conjugated inner product <a,b> = sum conj(a) b (bicgstab_dd_mix.cc:180-181).
 // Reductions accumulate in long double so that the oracle's norms are
 // insensitive to summation order at the 1e-14 level check.sh asks for.
@@ -673,6 +749,16 @@ void orc_msap_h(void* h, float* out, const float* in, int nsap, int nm) {
   for (long i = 0; i < V * 72; ++i) cx->Ch[i] = to_half(cx->fs.C[i]);
   Fields<float> fh = {cx->Uh.data(), cx->Ch.data()};
   msap_h(cx->g, fh, (float)cx->kappa, nsap, nm, out, in);
+}
+// Even/odd hopping (see deo) on x-compressed parity fields; U[2][4][V/2][18],
+// C[2][V/2][72] are the even/odd fields (only the lattice extents of h are used).
+void orc_deo_d(void* h, const double* U, const double* C, int pe, int set, int dag, int clv, int mode, double fac,
+               double* out, const double* in, const double* in0) {
+  deo(((Ctx*)h)->g, U, C, pe, set, dag != 0, clv != 0, mode, fac, out, in, in0);
+}
+void orc_deo_s(void* h, const float* U, const float* C, int pe, int set, int dag, int clv, int mode, float fac,
+               float* out, const float* in, const float* in0) {
+  deo(((Ctx*)h)->g, U, C, pe, set, dag != 0, clv != 0, mode, fac, out, in, in0);
 }
 // binary16 rounding of n floats (exposed so that it can be checked on its own)
 void orc_to_half(long n, const float* in, float* out) {

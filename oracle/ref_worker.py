@@ -59,6 +59,41 @@ def clover_to_aosoa(C, nloc, vlen):
     return np.ascontiguousarray(p.transpose(3, 0, 1, 2, 4, 6, 7, 5))
 
 
+# Even/odd reading of the same host arrays (deo_in_d.cc:65-90, qws_bqcd.cc:82-85):
+# a field of one parity p holds vold vectors; vector i = xv + nxv*(y + ny*(z + nz*t)),
+# lane j, is the site with compressed x index xh = xv*vlen + j.  Links:
+# glud[i + vold*mu + 4*vold*p]; clover clvd[i + vold*p].  The even/odd oracle works on
+# x-compressed plain fields: spinor (V/2, 24) with h = xh + (Lx/2)*(y + Ly*(z + Lz*t)),
+# links (2, 4, V/2, 18), clover (2, V/2, 72).
+
+def eo_spinor_from_raw(raw, nloc, vlen):
+    nx, ny, nz, nt = nloc
+    nxv = nx // 2 // vlen
+    r = np.asarray(raw).reshape(nt, nz, ny, nxv, 3, 4, 2, vlen).transpose(0, 1, 2, 3, 7, 4, 5, 6)
+    return np.ascontiguousarray(r).reshape(-1, 24)
+
+
+def eo_spinor_to_raw(a, nloc, vlen):
+    nx, ny, nz, nt = nloc
+    nxv = nx // 2 // vlen
+    r = np.asarray(a).reshape(nt, nz, ny, nxv, vlen, 3, 4, 2).transpose(0, 1, 2, 3, 5, 6, 7, 4)
+    return np.ascontiguousarray(r)
+
+
+def eo_links_from_raw(raw, nloc, vlen):
+    nx, ny, nz, nt = nloc
+    nxv = nx // 2 // vlen
+    r = np.asarray(raw).reshape(2, 4, nt, nz, ny, nxv, 3, 3, 2, vlen).transpose(0, 1, 2, 3, 4, 5, 9, 6, 7, 8)
+    return np.ascontiguousarray(r).reshape(2, 4, -1, 18)
+
+
+def eo_clover_from_raw(raw, nloc, vlen):
+    nx, ny, nz, nt = nloc
+    nxv = nx // 2 // vlen
+    r = np.asarray(raw).reshape(2, nt, nz, ny, nxv, 2, 36, vlen).transpose(0, 1, 2, 3, 4, 7, 5, 6)
+    return np.ascontiguousarray(r).reshape(2, -1, 72)
+
+
 class Ref:
     def __init__(self, nloc, kappa):
         self.lib = ctypes.CDLL(REFLIB)

@@ -66,6 +66,14 @@ def lib():
         L.qwsg_bicgstab_precdd.argtypes = [vp, ctypes.c_int, vp, vp, ctypes.c_double, ctypes.c_int, ctypes.c_int,
                                            ctypes.c_int, ip]
         L.qwsg_msap_h.argtypes = [vp, vp, vp, ctypes.c_int, ctypes.c_int]
+        L.qwsg_eo_load_gauge.argtypes = [vp, vp]
+        L.qwsg_eo_load_clover.argtypes = [vp, vp]
+        L.qwsg_eo_from_lattice.argtypes = [vp]
+        L.qwsg_eo_spinor_alloc.argtypes = [vp, ctypes.c_int, ctypes.POINTER(vp)]
+        L.qwsg_eo_spinor_upload.argtypes = [vp, ctypes.c_int, vp, vp]
+        L.qwsg_eo_spinor_download.argtypes = [vp, ctypes.c_int, vp, vp]
+        L.qwsg_deo.argtypes = [vp] + [ctypes.c_int] * 6 + [ctypes.c_double, vp, vp, vp]
+        L.qwsg_mtilde.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, vp, vp]
         L.qwsg_bicgstab_precdd_h.argtypes = [vp, vp, vp, ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_int, ip]
         L.qwsg_solve.argtypes = [vp, vp, vp, ctypes.POINTER(SolveParams), ctypes.POINTER(SolveStats)]
         L.qwsg_norm2.argtypes = [vp, ctypes.c_int, vp, ctypes.POINTER(ctypes.c_double)]
@@ -137,6 +145,31 @@ class Spinor:
         v = ctypes.c_double()
         check(lib().qwsg_norm2(self.lat.h, self.prec, self.ptr, ctypes.byref(v)))
         return v.value
+
+
+class EoSpinor:
+    """A device spinor of one parity: V/2 x-compressed sites (FP64 or FP32)."""
+
+    def __init__(self, lat, prec):
+        self.lat, self.prec = lat, prec
+        self.ptr = ctypes.c_void_p()
+        check(lib().qwsg_eo_spinor_alloc(lat.h, prec, ctypes.byref(self.ptr)))
+
+    def __del__(self):
+        try:
+            if self.ptr:
+                lib().qwsg_spinor_free(self.ptr)
+        except Exception:
+            pass
+
+    def upload(self, host):
+        a = np.ascontiguousarray(host, dtype=np.float64).reshape(self.lat.V // 2, 24)
+        check(lib().qwsg_eo_spinor_upload(self.lat.h, self.prec, self.ptr, _p(a)))
+
+    def download(self):
+        a = np.empty((self.lat.V // 2, 24))
+        check(lib().qwsg_eo_spinor_download(self.lat.h, self.prec, self.ptr, _p(a)))
+        return a
 
 
 class Lattice:
@@ -244,6 +277,32 @@ class Lattice:
         check(lib().qwsg_bicgstab_precdd(self.h, b.prec, x.ptr, b.ptr, float(tol), maxiter, nsap, nm,
                                          ctypes.byref(it)))
         return it.value
+
+    # ---- even/odd family (x-compressed parity fields of V/2 sites)
+    def eo_load_gauge(self, U):
+        """U: (2, 4, V/2, 18) FP64, the link from parity-p site h."""
+        a = np.ascontiguousarray(U, dtype=np.float64)
+        check(lib().qwsg_eo_load_gauge(self.h, _p(a)))
+
+    def eo_load_clover(self, C):
+        a = np.ascontiguousarray(C, dtype=np.float64)
+        check(lib().qwsg_eo_load_clover(self.h, _p(a)))
+
+    def eo_from_lattice(self):
+        check(lib().qwsg_eo_from_lattice(self.h))
+
+    def eo_spinor(self, prec, host=None):
+        s = EoSpinor(self, prec)
+        if host is not None:
+            s.upload(host)
+        return s
+
+    def deo(self, out, inp, pe, set=0, dag=False, clv=False, mode=0, fac=1.0, in0=None):
+        check(lib().qwsg_deo(self.h, inp.prec, pe, set, int(dag), int(clv), mode, float(fac), out.ptr, inp.ptr,
+                             in0.ptr if in0 is not None else None))
+
+    def mtilde(self, out, inp, pce=0, which=0):
+        check(lib().qwsg_mtilde(self.h, inp.prec, pce, which, out.ptr, inp.ptr))
 
     def msap_h(self, out, inp, nsap, nm):
         """out = M_SAP inp with FP16-stored SAP iterates and fields (FP32 vectors)."""

@@ -199,6 +199,39 @@ int qwsg_solve_batch(qwsg_lattice* h, int n, void* const* x, const void* const* 
     }
   });
 }
+int qwsg_eo_load_gauge(qwsg_lattice* h, const double* U) {
+  return guarded([&] { h->lat->eo_load_gauge(U, 0, 1, 1, 3); });
+}
+int qwsg_eo_load_clover(qwsg_lattice* h, const double* C) {
+  return guarded([&] { h->lat->eo_load_clover(C, 0, 1, 1, 3); });
+}
+int qwsg_eo_from_lattice(qwsg_lattice* h) {
+  return guarded([&] { h->lat->eo_fields_from_lattice(); });
+}
+int qwsg_eo_spinor_alloc(qwsg_lattice* h, int prec, void** dev) {
+  return guarded([&] {
+    if (prec != qws::P64 && prec != qws::P32) throw std::invalid_argument("even/odd spinors are FP64 or FP32");
+    *dev = nullptr;
+    QWS_HIP(hipMalloc(dev, h->lat->eo_spinor_bytes(prec)));
+    QWS_HIP(hipMemset(*dev, 0, h->lat->eo_spinor_bytes(prec)));
+  });
+}
+int qwsg_eo_spinor_upload(qwsg_lattice* h, int prec, void* dev, const double* host) {
+  return guarded([&] { h->lat->eo_upload_spinor(prec, dev, host, 0, 1, 1); });
+}
+int qwsg_eo_spinor_download(qwsg_lattice* h, int prec, const void* dev, double* host) {
+  return guarded([&] { h->lat->eo_download_spinor(prec, dev, host, 0, 1, 1); });
+}
+int qwsg_deo(qwsg_lattice* h, int prec, int pe, int set, int dag, int clv, int mode, double fac, void* out,
+             const void* in, const void* in0) {
+  return guarded([&] { h->lat->deo(prec, pe, set, dag, clv, mode, fac, out, in, in0); });
+}
+int qwsg_mtilde(qwsg_lattice* h, int prec, int pce, int which, void* out, const void* in) {
+  return guarded([&] {
+    const double k = h->lat->kappa();
+    h->lat->mtilde(prec, pce, which, -k * k, out, in);
+  });
+}
 int qwsg_norm2(qwsg_lattice* h, int prec, const void* a, double* out) {
   return guarded([&] { *out = h->lat->norm2(prec, a); });
 }
@@ -232,11 +265,13 @@ namespace {
 struct DropIn {
   Lattice* lat = nullptr;
   bool dirty = true;
+  bool eo_dirty = true;  // the even/odd reading of the same host arrays
   bool default_blocks = true;
   int tbc = -1;
   int* pce = nullptr;
   int* pco = nullptr;
   void* dev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};  // d0 d1 s0 s1 pre_d pre_s
+  void* eo[2][4] = {{nullptr, nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr, nullptr}};  // [P64/P32][in out in0 pre]
   std::vector<char> host_tmp[2];
 };
 DropIn g_qws;
@@ -294,6 +329,12 @@ void release_drop_in() {
       hipFree(p);
       p = nullptr;
     }
+  for (auto& row : g_qws.eo)
+    for (auto& p : row)
+      if (p) {
+        hipFree(p);
+        p = nullptr;
+      }
   delete g_qws.lat;
   g_qws.lat = nullptr;
 }
@@ -303,6 +344,76 @@ void create_drop_in_lattice(const int B[4]) {
   release_drop_in();
   g_qws.lat = new Lattice(L, B, g_qws.tbc, 18);
   g_qws.dirty = true;
+  g_qws.eo_dirty = true;
+}
+
+// The even/odd family reads glud/glus and clvd/clvs as two parity halves of
+// x-compressed sites (qws_bqcd.cc:82-85); upload that reading when they changed.
+Lattice& lat_sync_eo(const char* where) {
+  Lattice& L = lat_sync(where);
+  if (g_qws.eo_dirty) {
+    L.eo_load_gauge(glud, 1, QWS_VLEND, 1, 1);
+    L.eo_load_gauge(glus, 1, QWS_VLENS, 0, 2);
+    L.eo_load_clover(clvd, 1, QWS_VLEND, 1, 1);
+    L.eo_load_clover(clvs, 1, QWS_VLENS, 0, 2);
+    g_qws.eo_dirty = false;
+  }
+  return L;
+}
+
+void* eobuf(int prec, int i) {
+  void*& p = g_qws.eo[prec == qws::P64 ? 0 : 1][i];
+  if (!p) {
+    if (hipMalloc(&p, g_qws.lat->eo_spinor_bytes(prec)) != hipSuccess) throw std::runtime_error("hipMalloc failed");
+  }
+  return p;
+}
+
+// out (parity *pe, one parity field of the host layout) from in (parity 1 - *pe):
+// set/dag/clv/mode as Lattice::deo; mode 1 adds to out, mode 2 starts from in0;
+// set 2 takes its input from the snapshot recorded by the matching *_out_pre_*.
+void eo_call(const char* where, int prec, const int* pe, const int* po, int set, int dag, int clv, int mode, double fac,
+             void* out, const void* in, const void* in0) {
+  if ((*pe != 0 && *pe != 1) || *po != 1 - *pe) fatal(where, "parities must be 0 and 1");
+  drop_in(where, [&] {
+    Lattice& L = lat_sync_eo(where);
+    const int vlen = prec == qws::P64 ? QWS_VLEND : QWS_VLENS;
+    const int hd = prec == qws::P64 ? 1 : 0;
+    void* bi = eobuf(prec, 0);
+    void* bo = eobuf(prec, 1);
+    void* b0 = nullptr;
+    if (set == 2) {
+      bi = eobuf(prec, 3);
+    } else {
+      L.eo_upload_spinor(prec, bi, in, 1, vlen, hd);
+    }
+    if (mode == 1) L.eo_upload_spinor(prec, bo, out, 1, vlen, hd);
+    if (mode == 2) {
+      b0 = eobuf(prec, 2);
+      L.eo_upload_spinor(prec, b0, in0, 1, vlen, hd);
+    }
+    L.deo(prec, *pe, set, dag, clv, mode, fac, bo, bi, b0);
+    L.eo_download_spinor(prec, bo, out, 1, vlen, hd);
+  });
+}
+
+void eo_pre(const char* where, int prec, const void* in) {
+  drop_in(where, [&] {
+    Lattice& L = lat_sync_eo(where);
+    L.eo_upload_spinor(prec, eobuf(prec, 3), in, 1, prec == qws::P64 ? QWS_VLEND : QWS_VLENS, prec == qws::P64);
+  });
+}
+
+void eo_mtilde(const char* where, int prec, int which, double fac, void* out, const void* in) {
+  if (!g_qws.pce || (*g_qws.pce != 0 && *g_qws.pce != 1)) fatal(where, "pce must point at 0 or 1");
+  drop_in(where, [&] {
+    Lattice& L = lat_sync_eo(where);
+    const int vlen = prec == qws::P64 ? QWS_VLEND : QWS_VLENS;
+    const int hd = prec == qws::P64 ? 1 : 0;
+    L.eo_upload_spinor(prec, eobuf(prec, 0), in, 1, vlen, hd);
+    L.mtilde(prec, *g_qws.pce, which, fac, eobuf(prec, 1), eobuf(prec, 0));
+    L.eo_download_spinor(prec, eobuf(prec, 1), out, 1, vlen, hd);
+  });
 }
 
 }  // namespace
@@ -370,7 +481,10 @@ void qws_finalize_(void) {
   clvs = nullptr;
 }
 
-void qws_mi355x_fields_modified_(void) { g_qws.dirty = true; }
+void qws_mi355x_fields_modified_(void) {
+  g_qws.dirty = true;
+  g_qws.eo_dirty = true;
+}
 
 void qws_mi355x_set_sap_blocks_(int* b) {
   if (!g_qws.lat) fatal(__func__, "qws_init_ has not been called");
@@ -607,6 +721,102 @@ static void clvd_generic(int pe, scd_t* out, const scd_t* in) {
   L.download_spinor(qws::P64, b, full.data(), 1, 1);
   std::memcpy(out, full.data() + pe * db, db);
 }
+// ---- even/odd family (deo_in_d.cc, deo_out_d.cc, deo_dag_in.cc, deo_dag_out.cc,
+// deo_in_s.cc, deo_out_s.cc, qws.cc:518-605)
+void deo_in_vm_(int* pe, int* po, scd_t* out, scd_t* in) {
+  eo_call(__func__, qws::P64, pe, po, 1, 0, 0, 0, 0, out, in, nullptr);
+}
+void dee_deo_in_vm_(int* pe, int* po, scd_t* out, scd_t* in) {
+  eo_call(__func__, qws::P64, pe, po, 1, 0, 1, 0, 0, out, in, nullptr);
+}
+void deo_dag_in_vm_(int* pe, int* po, scd_t* out, scd_t* in) {
+  eo_call(__func__, qws::P64, pe, po, 1, 1, 0, 0, 0, out, in, nullptr);
+}
+void dee_deo_dag_in_vm_(int* pe, int* po, scd_t* out, scd_t* in) {
+  eo_call(__func__, qws::P64, pe, po, 1, 1, 1, 0, 0, out, in, nullptr);
+}
+void one_minus_dee_deo_in_vm_(int* pe, int* po, scd_t* out, scd_t* in, scd_t* in0, double factor) {
+  eo_call(__func__, qws::P64, pe, po, 1, 0, 1, 2, factor, out, in, in0);
+}
+void one_minus_deo_dag_in_vm_(int* pe, int* po, scd_t* out, scd_t* in, scd_t* in0, double factor) {
+  eo_call(__func__, qws::P64, pe, po, 1, 1, 0, 2, factor, out, in, in0);
+}
+void deo_out_pre_vm_(int* pe, int* po, scd_t* out, scd_t* in) {
+  (void)pe, (void)po, (void)out;
+  eo_pre(__func__, qws::P64, in);
+}
+void deo_dag_out_pre_vm_(int* pe, int* po, scd_t* out, scd_t* in) {
+  (void)pe, (void)po, (void)out;
+  eo_pre(__func__, qws::P64, in);
+}
+void deo_out_pos_vm_(int* pe, int* po, scd_t* out, scd_t* in, double factor) {
+  eo_call(__func__, qws::P64, pe, po, 2, 0, 0, 1, factor, out, in, nullptr);
+}
+void deo_dag_out_pos_vm_(int* pe, int* po, scd_t* out, scd_t* in, double factor) {
+  eo_call(__func__, qws::P64, pe, po, 2, 1, 0, 1, factor, out, in, nullptr);
+}
+void dee_deo_out_pos_vm_(int* pe, int* po, scd_t* out, scd_t* in, double factor) {
+  eo_call(__func__, qws::P64, pe, po, 2, 0, 1, 1, factor, out, in, nullptr);
+}
+void dee_deo_dag_out_pos_vm_(int* pe, int* po, scd_t* out, scd_t* in, double factor) {
+  eo_call(__func__, qws::P64, pe, po, 2, 1, 1, 1, factor, out, in, nullptr);
+}
+void deo_vm_(int* pe, int* po, scd_t* out, scd_t* in) {
+  eo_call(__func__, qws::P64, pe, po, 0, 0, 0, 0, 0, out, in, nullptr);
+}
+void dee_deo_vm_(int* pe, int* po, scd_t* out, scd_t* in) {
+  eo_call(__func__, qws::P64, pe, po, 0, 0, 1, 0, 0, out, in, nullptr);
+}
+void one_minus_dee_deo_vm_(int* pe, int* po, scd_t* out, scd_t* in, scd_t* in0) {
+  eo_call(__func__, qws::P64, pe, po, 0, 0, 1, 2, -kappa2, out, in, in0);
+}
+void deo_dag_vm_(int* pe, int* po, scd_t* out, scd_t* in) {
+  eo_call(__func__, qws::P64, pe, po, 0, 1, 0, 0, 0, out, in, nullptr);
+}
+void dee_deo_dag_vm_(int* pe, int* po, scd_t* out, scd_t* in) {
+  eo_call(__func__, qws::P64, pe, po, 0, 1, 1, 0, 0, out, in, nullptr);
+}
+void one_minus_deo_dag_vm_(int* pe, int* po, scd_t* out, scd_t* in, scd_t* in0) {
+  eo_call(__func__, qws::P64, pe, po, 0, 1, 0, 2, -kappa2, out, in, in0);
+}
+void mtilde_vm_(scd_t* out, scd_t* in) { eo_mtilde(__func__, qws::P64, 0, -kappa2, out, in); }
+void mtilde_dag_vm_(scd_t* out, scd_t* in) { eo_mtilde(__func__, qws::P64, 1, -kappa2, out, in); }
+void mtdagmt_vm_(scd_t* out, scd_t* in) { eo_mtilde(__func__, qws::P64, 2, -kappa2, out, in); }
+
+void deo_in_s_(int* pe, int* po, scs_t* out, scs_t* in) {
+  eo_call(__func__, qws::P32, pe, po, 1, 0, 0, 0, 0, out, in, nullptr);
+}
+void dee_deo_in_s_(int* pe, int* po, scs_t* out, scs_t* in) {
+  eo_call(__func__, qws::P32, pe, po, 1, 0, 1, 0, 0, out, in, nullptr);
+}
+void one_minus_dee_deo_in_s_(int* pe, int* po, scs_t* out, scs_t* in, scs_t* in0, float factor) {
+  eo_call(__func__, qws::P32, pe, po, 1, 0, 1, 2, factor, out, in, in0);
+}
+void deo_out_pre_s_(const int* pe, const int* po, const scs_t* out, const scs_t* in) {
+  (void)pe, (void)po, (void)out;
+  eo_pre(__func__, qws::P32, in);
+}
+void deo_out_pos_s_(const int* pe, const int* po, const scs_t* out, const scs_t* in, float factor) {
+  eo_call(__func__, qws::P32, pe, po, 2, 0, 0, 1, factor, const_cast<scs_t*>(out), in, nullptr);
+}
+void dee_deo_out_pos_s_(int* pe, int* po, scs_t* out, scs_t* in, float factor) {
+  eo_call(__func__, qws::P32, pe, po, 2, 0, 1, 1, factor, out, in, nullptr);
+}
+void clv_s_(int* pe, scs_t* inout) {
+  const int po = 1 - *pe;
+  eo_call(__func__, qws::P32, pe, &po, 3, 0, 1, 0, 0, inout, inout, nullptr);
+}
+void deo_s_(int* pe, int* po, scs_t* out, scs_t* in) {
+  eo_call(__func__, qws::P32, pe, po, 0, 0, 0, 0, 0, out, in, nullptr);
+}
+void dee_deo_s_(int* pe, int* po, scs_t* out, scs_t* in) {
+  eo_call(__func__, qws::P32, pe, po, 0, 0, 1, 0, 0, out, in, nullptr);
+}
+void one_minus_dee_deo_s_(int* pe, int* po, scs_t* out, scs_t* in, scs_t* in0) {
+  eo_call(__func__, qws::P32, pe, po, 0, 0, 1, 2, -(float)kappa2, out, in, in0);
+}
+void mtilde_s_(scs_t* out, scs_t* in) { eo_mtilde(__func__, qws::P32, 0, -(float)kappa2, out, in); }
+
 void clvd_vm_(int* pe, scd_t* inout) {
   drop_in(__func__, [&] { clvd_generic(*pe, inout, inout); });
 }
@@ -698,6 +908,7 @@ void qws_loadg_dd_(double* u, int* volh_tot, double* in_kappa) {
                 }
         }
   g_qws.dirty = true;
+  g_qws.eo_dirty = true;
 }
 
 void qws_loadc_dd_(double* c, int* volh) {
@@ -722,6 +933,7 @@ void qws_loadc_dd_(double* c, int* volh) {
             }
         }
   g_qws.dirty = true;
+  g_qws.eo_dirty = true;
 }
 
 }  // extern "C"

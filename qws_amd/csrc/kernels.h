@@ -106,4 +106,19 @@ void launch_export(const Geo& g, int kind, int layout, int vlen, const S* src, v
 // FP64 links (kind 1) or clover (kind 2) to the plain host layout (dst in device memory)
 void launch_export_field(const Geo& g, int kind, const double* src, double* dst, hipStream_t st);
 
+// ---- even/odd family on x-compressed parity fields (eo.hip)
+// out (parity pe) from in (parity 1-pe): set 0 all hops, 1 no wrap, 2 wrap only, 3 none;
+// dag: H^+; clv: T^-1 of parity pe; mode 0 out = acc, 1 out += fac acc, 2 out = in0 + fac acc
+template <typename S>
+void launch_deo(const Geo& g, int nl, int pe, int set, int dag, int clv, int mode, typename Store<S>::R fac, S* out,
+                const S* in, const S* in0, const S* U, const S* C, hipStream_t st);
+// kind 0 spinor (one parity), 1 links, 2 clover (both parities); layout 0 plain, 1 QWS AoSoA
+template <typename S>
+void launch_eo_import(const Geo& g, int kind, int layout, int vlen, int nl, const void* src, int src_is_double, S* dst,
+                      hipStream_t st);
+template <typename S>
+void launch_eo_export(const Geo& g, int layout, int vlen, const S* src, void* dst, int dst_is_double, hipStream_t st);
+template <typename S>
+void launch_eo_gather(const Geo& g, int nl, const S* Ud, const S* Cd, S* Ue, S* Ce, hipStream_t st);
+
 }  // namespace qws

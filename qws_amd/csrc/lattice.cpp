@@ -85,6 +85,8 @@ Lattice::Lattice(const int L[4], const int B[4], int tbc, int links32) {
 Lattice::~Lattice() {
   (void)hipStreamSynchronize(st_);
   batch_.reset();
+  for (auto* p : {(void*)Ueo_d_, (void*)Ceo_d_, (void*)Ueo_s_, (void*)Ceo_s_})
+    if (p) (void)hipFree(p);
   for (auto* p : {(void*)Ud_, (void*)Cd_, (void*)Us_, (void*)Cs_, (void*)Uh_, (void*)Ch_, (void*)part_, (void*)sc_in_,
                   (void*)sc_one_, (void*)sc_h_, (void*)d_inner_mask_, (void*)d_blocks_[0], (void*)d_blocks_[1]})
     if (p) (void)hipFree(p);
@@ -509,6 +511,116 @@ double Lattice::norm2(int prec, const void* a) {
   else throw std::invalid_argument("norm2: FP16 vectors are not reduced directly");
   launch_finalize(single(), part, 1, sc_one_, FIN_RAW, st_);
   return read_scalar(sc_one_, offsetof(Scalars, red));
+}
+
+// ------------------------------------------------------------------ even/odd family
+void Lattice::eo_load_gauge(const void* U, int layout, int vlen, int is_double, int target) {
+  const size_t bytes = (size_t)4 * g_.V * 18 * (is_double ? 8 : 4);
+  void* stage = work("stage", bytes);
+  QWS_HIP(hipMemcpyAsync(stage, U, bytes, hipMemcpyHostToDevice, st_));
+  if (target & 1) {
+    if (!Ueo_d_) Ueo_d_ = dalloc<double>((size_t)4 * 18 * g_.V);
+    launch_eo_import<double>(g_, 1, layout, vlen, 18, stage, is_double, Ueo_d_, st_);
+  }
+  if (target & 2) {
+    if (!Ueo_s_) Ueo_s_ = dalloc<float>((size_t)4 * nl32_ * g_.V);
+    launch_eo_import<float>(g_, 1, layout, vlen, nl32_, stage, is_double, Ueo_s_, st_);
+  }
+  QWS_HIP(hipStreamSynchronize(st_));
+}
+
+void Lattice::eo_load_clover(const void* C, int layout, int vlen, int is_double, int target) {
+  const size_t bytes = (size_t)g_.V * 72 * (is_double ? 8 : 4);
+  void* stage = work("stage", bytes);
+  QWS_HIP(hipMemcpyAsync(stage, C, bytes, hipMemcpyHostToDevice, st_));
+  if (target & 1) {
+    if (!Ceo_d_) Ceo_d_ = dalloc<double>((size_t)72 * g_.V);
+    launch_eo_import<double>(g_, 2, layout, vlen, 0, stage, is_double, Ceo_d_, st_);
+  }
+  if (target & 2) {
+    if (!Ceo_s_) Ceo_s_ = dalloc<float>((size_t)72 * g_.V);
+    launch_eo_import<float>(g_, 2, layout, vlen, 0, stage, is_double, Ceo_s_, st_);
+  }
+  QWS_HIP(hipStreamSynchronize(st_));
+}
+
+void Lattice::eo_fields_from_lattice() {
+  for (int m = 0; m < 4; ++m)
+    if (g_.L[m] % 2) throw std::invalid_argument("even/odd fields from the lattice need even extents");
+  if (Ud_ && Cd_) {
+    if (!Ueo_d_) Ueo_d_ = dalloc<double>((size_t)4 * 18 * g_.V);
+    if (!Ceo_d_) Ceo_d_ = dalloc<double>((size_t)72 * g_.V);
+    launch_eo_gather<double>(g_, 18, Ud_, Cd_, Ueo_d_, Ceo_d_, st_);
+  }
+  if (Us_ && Cs_) {
+    if (!Ueo_s_) Ueo_s_ = dalloc<float>((size_t)4 * nl32_ * g_.V);
+    if (!Ceo_s_) Ceo_s_ = dalloc<float>((size_t)72 * g_.V);
+    launch_eo_gather<float>(g_, nl32_, Us_, Cs_, Ueo_s_, Ceo_s_, st_);
+  }
+  QWS_HIP(hipStreamSynchronize(st_));
+}
+
+void Lattice::eo_upload_spinor(int prec, void* dev, const void* host, int layout, int vlen, int host_is_double) {
+  const size_t bytes = (size_t)(g_.V / 2) * 24 * (host_is_double ? 8 : 4);
+  void* stage = work("stage_spinor", bytes);
+  QWS_HIP(hipMemcpyAsync(stage, host, bytes, hipMemcpyHostToDevice, st_));
+  if (prec == P64) launch_eo_import<double>(g_, 0, layout, vlen, 0, stage, host_is_double, (double*)dev, st_);
+  else if (prec == P32) launch_eo_import<float>(g_, 0, layout, vlen, 0, stage, host_is_double, (float*)dev, st_);
+  else throw std::invalid_argument("even/odd spinors are FP64 or FP32");
+  QWS_HIP(hipStreamSynchronize(st_));
+}
+
+void Lattice::eo_download_spinor(int prec, const void* dev, void* host, int layout, int vlen, int host_is_double) {
+  const size_t bytes = (size_t)(g_.V / 2) * 24 * (host_is_double ? 8 : 4);
+  void* stage = work("stage_spinor", bytes);
+  if (prec == P64) launch_eo_export<double>(g_, layout, vlen, (const double*)dev, stage, host_is_double, st_);
+  else if (prec == P32) launch_eo_export<float>(g_, layout, vlen, (const float*)dev, stage, host_is_double, st_);
+  else throw std::invalid_argument("even/odd spinors are FP64 or FP32");
+  QWS_HIP(hipMemcpyAsync(host, stage, bytes, hipMemcpyDeviceToHost, st_));
+  QWS_HIP(hipStreamSynchronize(st_));
+}
+
+void Lattice::deo(int prec, int pe, int set, int dag, int clv, int mode, double fac, void* out, const void* in,
+                  const void* in0) {
+  if (pe != 0 && pe != 1) throw std::invalid_argument("parity must be 0 or 1");
+  if (set < 0 || set > 3 || mode < 0 || mode > 2) throw std::invalid_argument("deo: set in 0..3, mode in 0..2");
+  if (mode == 2 && !in0) throw std::invalid_argument("deo: mode 2 needs in0");
+  if (prec == P64) {
+    if (!Ueo_d_ || !Ceo_d_) throw std::runtime_error("FP64 even/odd fields are not loaded");
+    launch_deo<double>(g_, 18, pe, set, dag, clv, mode, fac, (double*)out, (const double*)in, (const double*)in0,
+                       Ueo_d_, Ceo_d_, st_);
+  } else if (prec == P32) {
+    if (!Ueo_s_ || !Ceo_s_) throw std::runtime_error("FP32 even/odd fields are not loaded");
+    launch_deo<float>(g_, nl32_, pe, set, dag, clv, mode, (float)fac, (float*)out, (const float*)in,
+                      (const float*)in0, Ueo_s_, Ceo_s_, st_);
+  } else {
+    throw std::invalid_argument("even/odd operators are FP64 or FP32");
+  }
+}
+
+void Lattice::mtilde(int prec, int pce, int which, double fac, void* out, const void* in) {
+  if (which < 0 || which > 2) throw std::invalid_argument("mtilde: which in 0..2");
+  if (pce != 0 && pce != 1) throw std::invalid_argument("parity must be 0 or 1");
+  const int pco = 1 - pce;
+  const size_t b = eo_spinor_bytes(prec);
+  void* t0 = work("eo_t0", b);
+  void* t1 = work("eo_t1", b);
+  void* t2 = work("eo_t2", b);
+  auto mt = [&](void* o, const void* i) {  // qws.cc:551-558
+    deo(prec, pco, 0, 0, 1, 0, 0, t0, i, nullptr);
+    deo(prec, pce, 0, 0, 1, 2, fac, o, t0, i);
+  };
+  auto mtdag = [&](void* o, const void* i) {  // qws.cc:559-568
+    deo(prec, pce, 3, 0, 1, 0, 0, t0, i, nullptr);
+    deo(prec, pco, 0, 1, 1, 0, 0, t1, t0, nullptr);
+    deo(prec, pce, 0, 1, 0, 2, fac, o, t1, i);
+  };
+  if (which == 0) mt(out, in);
+  else if (which == 1) mtdag(out, in);
+  else {
+    mt(t2, in);
+    mtdag(out, t2);
+  }
 }
 
 // ------------------------------------------------------------------ inner BiCGStab

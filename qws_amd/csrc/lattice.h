@@ -139,6 +139,23 @@ class Lattice {
   double time_dslash(int prec, int reps);
   void sync() { QWS_HIP(hipStreamSynchronize(st_)); }
 
+  // ---- even/odd family (deo_*, mtilde) on x-compressed parity fields of V/2 sites
+  size_t eo_spinor_bytes(int prec) const { return spinor_bytes(prec) / 2; }
+  // host fields in layout 0 (plain: links [2][4][V/2][18], clover [2][V/2][72]) or 1 (QWS
+  // AoSoA with lane width vlen, read as parity halves); target bit0 FP64, bit1 FP32
+  void eo_load_gauge(const void* U, int layout, int vlen, int is_double, int target);
+  void eo_load_clover(const void* C, int layout, int vlen, int is_double, int target);
+  // even/odd fields from the lattice's own fields (all extents even)
+  void eo_fields_from_lattice();
+  void eo_upload_spinor(int prec, void* dev, const void* host, int layout, int vlen, int host_is_double);
+  void eo_download_spinor(int prec, const void* dev, void* host, int layout, int vlen, int host_is_double);
+  // out (parity pe) from in (parity 1-pe); see launch_deo
+  void deo(int prec, int pe, int set, int dag, int clv, int mode, double fac, void* out, const void* in,
+           const void* in0);
+  // which 0: mtilde = 1 + fac T^-1 H T^-1 H on parity pce (qws.cc:551-558; fac = -kappa^2),
+  // 1: its adjoint (qws.cc:559-568), 2: mtilde^+ mtilde (qws.cc:569-574)
+  void mtilde(int prec, int pce, int which, double fac, void* out, const void* in);
+
   // ---- batched building blocks (used by OuterBatch)
   void dslash_batch(const Batch& B, double* out, const double* in);
   void approx_inv(const Batch& B, Scalars* sc, const std::vector<int>& live, double* u, const double* p,
@@ -178,6 +195,8 @@ class Lattice {
   double *Ud_ = nullptr, *Cd_ = nullptr;
   float *Us_ = nullptr, *Cs_ = nullptr;
   __half *Uh_ = nullptr, *Ch_ = nullptr;
+  double *Ueo_d_ = nullptr, *Ceo_d_ = nullptr;  // even/odd fields
+  float *Ueo_s_ = nullptr, *Ceo_s_ = nullptr;
   bool half_valid_ = false;
   bool use_fused_ = true;
   // reductions

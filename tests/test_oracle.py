@@ -313,3 +313,80 @@ def test_msap_h_is_fp16_prec_s_with_one_sweep_less(L, B):
         assert rel(h, orc.prec_s(b, nsap, nm)) > 1e-2
     h1 = orc.msap_h(b, 3, 2)
     assert np.array_equal(orc.msap_h(b * np.float32(1024.0), 3, 2), h1 * np.float32(1024.0))
+
+
+# ---------------------------------------------------------------- even/odd family (main.cc:584-790)
+def _eo_case0(vlen, dtype):
+    """data/CASE0 inputs read as the even/odd family reads them: the same host arrays
+    (glud/glus, clvd/clvs, b/bs) taken as two parity halves of x-compressed sites."""
+    nloc = [32, 6, 4, 3]
+    _, U, C, b, _ = O.case_inputs(nloc)
+    vol = int(np.prod(nloc)) // 2 // vlen
+    Ue = RW.eo_links_from_raw(RW.links_to_aosoa(U.astype(dtype), nloc, vlen), nloc, vlen)
+    Ce = RW.eo_clover_from_raw(RW.clover_to_aosoa(C.astype(dtype), nloc, vlen), nloc, vlen)
+    raw = RW.spinor_to_aosoa(b.astype(dtype), nloc, vlen).reshape(2 * vol, -1)
+    inp = RW.eo_spinor_from_raw(raw[:vol], nloc, vlen)
+    orc = O.Oracle(nloc, [16, 6, 4, 3], -1, 0.05, U, C)
+    return (lambda *a, **k: orc.deo(Ue, Ce, *a, **k)), inp
+
+
+def test_even_odd_family_case0_fp64():
+    """Every FP64 even/odd operator of the reference driver (main.cc:584-700) reproduces
+    data/CASE0:4-24 (check.sh: 1e-14 relative).  Called as the driver calls them:
+    out parity po = 1 from the parity pe = 0 input, fac = -kappa^2; mtilde with
+    pce = 0, pco = 1 (qws.cc:551-576)."""
+    D, inp = _eo_case0(8, np.float64)
+    fac = -0.05 ** 2
+    got = {
+        "deo_in_vm_": D(1, inp, set=1),
+        "dee_deo_in_vm_": D(1, inp, set=1, clv=True),
+        "deo_dag_in_vm_": D(1, inp, set=1, dag=True),
+        "dee_deo_dag_in_vm_": D(1, inp, set=1, dag=True, clv=True),
+        "one_minus_dee_deo_in_vm_": D(1, inp, set=1, clv=True, mode=2, fac=fac, in0=inp),
+        "one_minus_deo_dag_in_vm_": D(1, inp, set=1, dag=True, mode=2, fac=fac, in0=inp),
+        "deo_out_pre|pos_vm_": D(1, inp, set=2, mode=1, fac=fac),
+        "deo_dag_out_pre|pos_vm_": D(1, inp, set=2, dag=True, mode=1, fac=fac),
+        "dee_deo_out_pre|pos_vm_": D(1, inp, set=2, clv=True, mode=1, fac=fac),
+        "dee_deo_dag_out_pre|pos_vm_": D(1, inp, set=2, dag=True, clv=True, mode=1, fac=fac),
+        "clvd_vm2_": D(0, inp, set=3, clv=True),
+        "deo_vm_": D(1, inp),
+        "dee_deo_vm_": D(1, inp, clv=True),
+        "deo_dag_vm_": D(1, inp, dag=True),
+        "dee_deo_dag_vm_": D(1, inp, dag=True, clv=True),
+        "one_minus_dee_deo_vm_": D(1, inp, clv=True, mode=2, fac=fac, in0=inp),
+        "one_minus_deo_dag_vm_": D(1, inp, dag=True, mode=2, fac=fac, in0=inp),
+    }
+    mt = D(0, D(1, inp, clv=True), clv=True, mode=2, fac=fac, in0=inp)
+    got["mtilde_vm_"] = mt
+    mtdag = lambda v: D(0, D(1, D(0, v, set=3, clv=True), dag=True, clv=True), dag=True, mode=2, fac=fac, in0=v)
+    got["mtilde_dag_vm_"] = mtdag(inp)
+    got["mtdagmt_vm_"] = mtdag(mt)
+    for name, v in got.items():
+        assert K.check_value(name, O.norm2(v), K.expected("CASE0", name)), name
+    # in + out = all hops, and the dagger is the adjoint: <a, H b> = <H^+ a, b>
+    assert np.allclose(D(1, inp, set=1) + D(1, inp, set=2), D(1, inp), rtol=0, atol=1e-12)
+    rng = np.random.default_rng(4)
+    a = rng.standard_normal(inp.shape)
+    lhs = np.vdot(a.view(np.complex128), D(1, inp).view(np.complex128))
+    rhs = np.vdot(D(0, a, dag=True).view(np.complex128), inp.view(np.complex128))
+    assert abs(lhs - rhs) < 1e-9 * abs(lhs)
+
+
+def test_even_odd_family_case0_fp32():
+    """FP32 even/odd operators (main.cc:724-760) against data/CASE0:30-39 (3e-6)."""
+    S, inp = _eo_case0(16, np.float32)
+    fs = np.float32(-0.05 ** 2)
+    got = {
+        "deo_in_s_": S(1, inp, set=1),
+        "dee_deo_in_s_": S(1, inp, set=1, clv=True),
+        "one_minus_dee_deo_in_s_": S(1, inp, set=1, clv=True, mode=2, fac=fs, in0=inp),
+        "deo_out_pre|pos_s_": S(1, inp, set=2, mode=1, fac=fs),
+        "dee_deo_out_pre|pos_s_": S(1, inp, set=2, clv=True, mode=1, fac=fs),
+        "clv_s_": S(0, inp, set=3, clv=True),
+        "deo_s_": S(1, inp),
+        "dee_deo_s_": S(1, inp, clv=True),
+        "one_minus_dee_deo_s_": S(1, inp, clv=True, mode=2, fac=fs, in0=inp),
+        "mtilde_s_": S(0, S(1, inp, clv=True), clv=True, mode=2, fac=fs, in0=inp),
+    }
+    for name, v in got.items():
+        assert K.check_value(name, O.norm2(v), K.expected("CASE0", name)), name
