@@ -277,6 +277,41 @@ int qwsg_deo(qwsg_lattice* lat, int prec, int pe, int set, int dag, int clv, int
              const void* in, const void* in0);
 int qwsg_mtilde(qwsg_lattice* lat, int prec, int pce, int which, void* out, const void* in);
 int qwsg_norm2(qwsg_lattice* lat, int prec, const void* a, double* out);
+
+/* No counterpart in the reference (it solves one source per call, serially): a solve
+ * server on one GPU.  The gauge fields stay resident; each request names a field, a
+ * kappa and 1..64 Gaussian sources (source k uses seed*1000 + k).  qwsg_server_run
+ * replays a trace in real time: a request is released at its arrival time, admitted in
+ * arrival order when one of `slots` (1..16) is free and its memory estimate fits
+ * `mem_budget` bytes (0: 90% of the free HBM), and runs on its own HIP stream at the
+ * same time as the other admitted requests: clover inverse for its kappa on the shared
+ * links, then one batched solve of all its sources.  Results carry arrival, admission
+ * and finish times (seconds from the start of the trace), iteration counts, the
+ * largest FP64 true residual and an order-independent digest of the solutions, which
+ * is the same whether the request runs alone or among others. */
+typedef struct qwsg_server qwsg_server;
+typedef struct {
+  int L[4], B[4];
+  double eps;                /* links exp(i eps H) */
+  unsigned long long seed;
+} qwsg_field_spec;
+typedef struct {
+  int field;
+  double kappa;
+  int nrhs;
+  unsigned long long seed;
+  double arrival;
+} qwsg_request;
+typedef struct {
+  double arrival, admit, finish;
+  int outer_iters, inner_iters, restarts;
+  double true_residual_max, bytes;
+  unsigned long long digest;
+} qwsg_request_result;
+int qwsg_server_create(const qwsg_field_spec* fields, int nfields, int links32, int tbc, double csw,
+                       const qwsg_solve_params* p, int slots, double mem_budget, qwsg_server** out);
+int qwsg_server_run(qwsg_server* srv, const qwsg_request* reqs, int nreq, qwsg_request_result* out);
+int qwsg_server_destroy(qwsg_server* srv);
 int qwsg_synchronize(qwsg_lattice* lat);
 /* average device time (ms) of `reps` full FP32/FP16/FP64 dslash launches on the
  * lattice's stream, measured with HIP events */

@@ -66,6 +66,10 @@ def lib():
         L.qwsg_bicgstab_precdd.argtypes = [vp, ctypes.c_int, vp, vp, ctypes.c_double, ctypes.c_int, ctypes.c_int,
                                            ctypes.c_int, ip]
         L.qwsg_msap_h.argtypes = [vp, vp, vp, ctypes.c_int, ctypes.c_int]
+        L.qwsg_server_create.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                         ctypes.POINTER(SolveParams), ctypes.c_int, ctypes.c_double, ctypes.POINTER(vp)]
+        L.qwsg_server_run.argtypes = [vp, vp, ctypes.c_int, vp]
+        L.qwsg_server_destroy.argtypes = [vp]
         L.qwsg_eo_load_gauge.argtypes = [vp, vp]
         L.qwsg_eo_load_clover.argtypes = [vp, vp]
         L.qwsg_eo_from_lattice.argtypes = [vp]
@@ -145,6 +149,64 @@ class Spinor:
         v = ctypes.c_double()
         check(lib().qwsg_norm2(self.lat.h, self.prec, self.ptr, ctypes.byref(v)))
         return v.value
+
+
+class FieldSpec(ctypes.Structure):
+    _fields_ = [("L", ctypes.c_int * 4), ("B", ctypes.c_int * 4), ("eps", ctypes.c_double),
+                ("seed", ctypes.c_ulonglong)]
+
+
+class Request(ctypes.Structure):
+    _fields_ = [("field", ctypes.c_int), ("kappa", ctypes.c_double), ("nrhs", ctypes.c_int),
+                ("seed", ctypes.c_ulonglong), ("arrival", ctypes.c_double)]
+
+
+class RequestResult(ctypes.Structure):
+    _fields_ = [("arrival", ctypes.c_double), ("admit", ctypes.c_double), ("finish", ctypes.c_double),
+                ("outer_iters", ctypes.c_int), ("inner_iters", ctypes.c_int), ("restarts", ctypes.c_int),
+                ("true_residual_max", ctypes.c_double), ("bytes", ctypes.c_double), ("digest", ctypes.c_ulonglong)]
+
+
+class Server:
+    """Solve server with resident gauge fields (qwsg_server_*).
+
+    fields: list of (L, B, eps, seed); requests: list of dicts with keys field, kappa,
+    nrhs, seed, arrival (seconds).  run() replays the trace in real time and returns one
+    dict per request (arrival/admit/finish times, iterations, true residual, digest)."""
+
+    def __init__(self, fields, links32=12, tbc=-1, csw=1.0, slots=4, mem_budget=0.0, nsap=4, nm=2, tol=1e-11,
+                 maxiter=40, tol_s=1e-3, maxiter_s=40, true_tol=1e-10, max_restarts=4, sap_precision=F32):
+        fs = (FieldSpec * len(fields))()
+        for i, (L, B, eps, seed) in enumerate(fields):
+            fs[i].L[:] = list(L)
+            fs[i].B[:] = list(B)
+            fs[i].eps = eps
+            fs[i].seed = seed
+        p = SolveParams(nsap, nm, tol, maxiter, tol_s, maxiter_s, true_tol, max_restarts, sap_precision)
+        self.h = ctypes.c_void_p()
+        check(lib().qwsg_server_create(fs, len(fields), links32, tbc, csw, ctypes.byref(p), slots,
+                                       float(mem_budget), ctypes.byref(self.h)))
+
+    def run(self, requests):
+        n = len(requests)
+        rq = (Request * n)()
+        for i, r in enumerate(requests):
+            rq[i].field, rq[i].kappa, rq[i].nrhs = r["field"], r["kappa"], r["nrhs"]
+            rq[i].seed, rq[i].arrival = r["seed"], r["arrival"]
+        out = (RequestResult * n)()
+        check(lib().qwsg_server_run(self.h, rq, n, out))
+        return [{k: getattr(o, k) for k, _ in RequestResult._fields_} for o in out]
+
+    def close(self):
+        if self.h:
+            check(lib().qwsg_server_destroy(self.h))
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class EoSpinor:

@@ -11,6 +11,7 @@
 
 #include "../../include/qws_mi355x.h"
 #include "lattice.h"
+#include "serve.h"
 
 using qws::Lattice;
 
@@ -230,6 +231,67 @@ int qwsg_mtilde(qwsg_lattice* h, int prec, int pce, int which, void* out, const 
   return guarded([&] {
     const double k = h->lat->kappa();
     h->lat->mtilde(prec, pce, which, -k * k, out, in);
+  });
+}
+struct qwsg_server {
+  qws::Server* srv;
+};
+int qwsg_server_create(const qwsg_field_spec* fields, int nfields, int links32, int tbc, double csw,
+                       const qwsg_solve_params* p, int slots, double mem_budget, qwsg_server** out) {
+  return guarded([&] {
+    if (nfields < 1) throw std::invalid_argument("a server needs at least one field");
+    std::vector<qws::FieldSpec> fs(nfields);
+    for (int i = 0; i < nfields; ++i) {
+      for (int m = 0; m < 4; ++m) {
+        fs[i].L[m] = fields[i].L[m];
+        fs[i].B[m] = fields[i].B[m];
+      }
+      fs[i].eps = fields[i].eps;
+      fs[i].seed = fields[i].seed;
+    }
+    qws::ServeConfig cfg;
+    cfg.links32 = links32;
+    cfg.tbc = tbc;
+    cfg.csw = csw;
+    cfg.slots = slots;
+    cfg.mem_budget = mem_budget;
+    cfg.prm.nsap = p->nsap;
+    cfg.prm.nm = p->nm;
+    cfg.prm.tol = p->tol;
+    cfg.prm.maxiter = p->maxiter;
+    cfg.prm.tol_s = p->tol_s;
+    cfg.prm.maxiter_s = p->maxiter_s;
+    cfg.prm.true_tol = p->true_tol;
+    cfg.prm.max_restarts = p->max_restarts;
+    cfg.prm.sap_prec = p->sap_precision;
+    if (cfg.prm.nsap > 10 || cfg.prm.nm > 10) throw std::invalid_argument("nsap and nm must be <= 10");
+    *out = new qwsg_server{new qws::Server(fs, cfg)};
+  });
+}
+int qwsg_server_run(qwsg_server* h, const qwsg_request* reqs, int nreq, qwsg_request_result* out) {
+  return guarded([&] {
+    std::vector<qws::Request> tr(nreq);
+    for (int i = 0; i < nreq; ++i) tr[i] = {reqs[i].field, reqs[i].kappa, reqs[i].nrhs, reqs[i].seed, reqs[i].arrival};
+    std::vector<qws::RequestResult> r = h->srv->run(tr);
+    for (int i = 0; i < nreq; ++i) {
+      out[i].arrival = r[i].arrival;
+      out[i].admit = r[i].admit;
+      out[i].finish = r[i].finish;
+      out[i].outer_iters = r[i].outer_iters;
+      out[i].inner_iters = r[i].inner_iters;
+      out[i].restarts = r[i].restarts;
+      out[i].true_residual_max = r[i].true_residual_max;
+      out[i].bytes = r[i].bytes;
+      out[i].digest = r[i].digest;
+    }
+  });
+}
+int qwsg_server_destroy(qwsg_server* h) {
+  return guarded([&] {
+    if (h) {
+      delete h->srv;
+      delete h;
+    }
   });
 }
 int qwsg_norm2(qwsg_lattice* h, int prec, const void* a, double* out) {

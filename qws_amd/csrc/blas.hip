@@ -359,6 +359,23 @@ k_scale_rms(long long V, long long so, long long si, const int* active, SO* out,
   }
 }
 
+// Order-independent 64-bit digest of n 8-byte words: XOR over i of a SplitMix64
+// finalisation of (w[i] + (i + 1) * golden ratio).  Equal for bit-identical data
+// whatever the scheduling; used to compare solutions without copying them out.
+__device__ __forceinline__ unsigned long long mix64(unsigned long long z) {
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+  return z ^ (z >> 31);
+}
+__global__ void __launch_bounds__(RED_THREADS) k_digest(long long n, const unsigned long long* w,
+                                                        unsigned long long* out) {
+  unsigned long long h = 0;
+  GRID_STRIDE(i, n) h ^= mix64(w[i] + 0x9E3779B97F4A7C15ULL * (unsigned long long)(i + 1));
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) h ^= __shfl_xor(h, off, 64);
+  if ((threadIdx.x & 63) == 0) atomicXor(out, h);
+}
+
 template <typename R>
 __global__ void __launch_bounds__(RED_THREADS) k_sub(long long n, R* out, const R* a, const R* b) {
   GRID_STRIDE(i, n) out[i] = a[i] - b[i];
@@ -423,6 +440,9 @@ template <typename SO, typename SI>
 void launch_scale_rms(const Batch& B, long long V, long long so, long long si, SO* out, const SI* in, const Scalars* sc,
                       int dir, hipStream_t st) {
   LAUNCH_B((k_scale_rms<SO, SI>), B.n, V, so, si, B.active, out, in, sc, dir);
+}
+void launch_digest(long long nwords, const void* data, unsigned long long* out, hipStream_t st) {
+  LAUNCH_B(k_digest, 1, nwords, (const unsigned long long*)data, out);
 }
 template <typename R> void launch_sub(long long n, R* out, const R* a, const R* b, hipStream_t st) {
   LAUNCH_B((k_sub<R>), 1, n, out, a, b);

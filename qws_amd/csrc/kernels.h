@@ -91,6 +91,8 @@ template <typename SO, typename SI> void launch_convert_reals(long long n, SO* o
 template <typename SO, typename SI>
 void launch_scale_rms(const Batch& B, long long V, long long so, long long si, SO* out, const SI* in, const Scalars* sc,
                       int dir, hipStream_t st);
+// *out ^= order-independent digest of nwords 8-byte words (clear *out first)
+void launch_digest(long long nwords, const void* data, unsigned long long* out, hipStream_t st);
 // out = a - b (real arrays of length n)
 template <typename R> void launch_sub(long long n, R* out, const R* a, const R* b, hipStream_t st);
 

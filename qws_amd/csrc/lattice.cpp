@@ -87,6 +87,7 @@ Lattice::~Lattice() {
   batch_.reset();
   for (auto* p : {(void*)Ueo_d_, (void*)Ceo_d_, (void*)Ueo_s_, (void*)Ceo_s_})
     if (p) (void)hipFree(p);
+  if (!owns_gauge_) Ud_ = nullptr, Us_ = nullptr;
   for (auto* p : {(void*)Ud_, (void*)Cd_, (void*)Us_, (void*)Cs_, (void*)Uh_, (void*)Ch_, (void*)part_, (void*)sc_in_,
                   (void*)sc_one_, (void*)sc_h_, (void*)d_inner_mask_, (void*)d_blocks_[0], (void*)d_blocks_[1]})
     if (p) (void)hipFree(p);
@@ -147,6 +148,7 @@ void* Lattice::alloc_spinor(int prec) {
 
 // ------------------------------------------------------------------ field loading
 void Lattice::load_gauge(const void* U, int layout, int is_double, int target) {
+  if (!owns_gauge_) throw std::runtime_error("load_gauge: this lattice borrows its links");
   const size_t es = is_double ? 8 : 4;
   const size_t bytes = (size_t)4 * g_.V * 18 * es;
   void* stage = work("stage", bytes);
@@ -183,12 +185,30 @@ void Lattice::load_clover(const void* C, int layout, int is_double, int target) 
 }
 
 void Lattice::generate_gauge(double eps, unsigned long long seed) {
+  if (!owns_gauge_) throw std::runtime_error("generate_gauge: this lattice borrows its links");
   if (!Ud_) Ud_ = dalloc<double>((size_t)4 * 18 * g_.V);
   if (!Us_) Us_ = dalloc<float>((size_t)4 * nl32_ * g_.V);
   launch_gen_links(g_, eps, (uint64_t)seed, Ud_, st_);
   launch_links_d2s(g_, nl32_, Ud_, Us_, st_);
   half_valid_ = false;
   QWS_HIP(hipStreamSynchronize(st_));
+}
+
+void Lattice::share_gauge(const Lattice& src) {
+  for (int m = 0; m < 4; ++m)
+    if (src.g_.L[m] != g_.L[m]) throw std::invalid_argument("share_gauge: lattice extents differ");
+  if (src.nl32_ != nl32_ || src.g_.tbc != g_.tbc) throw std::invalid_argument("share_gauge: link storage differs");
+  if (!src.Ud_) throw std::invalid_argument("share_gauge: the source has no links");
+  if (src.g_.Vb != g_.Vb || src.g_.B[0] != g_.B[0] || src.g_.B[1] != g_.B[1] || src.g_.B[2] != g_.B[2])
+    throw std::invalid_argument("share_gauge: block shapes differ (the device site order follows the blocks)");
+  if (owns_gauge_) {
+    if (Ud_) QWS_HIP(hipFree(Ud_));
+    if (Us_) QWS_HIP(hipFree(Us_));
+  }
+  Ud_ = src.Ud_;
+  Us_ = src.Us_;
+  owns_gauge_ = false;
+  half_valid_ = false;
 }
 
 void Lattice::compute_clover(double csw) {

@@ -105,6 +105,9 @@ class Lattice {
   void load_clover(const void* C, int layout, int is_double, int target);
   // generated inputs: SU(3) links exp(i eps H), clover inverse from the links
   void generate_gauge(double eps, unsigned long long seed);
+  // use another lattice's links (same extents and link storage); they stay owned by src,
+  // which must outlive this lattice and not change its links meanwhile
+  void share_gauge(const Lattice& src);
   void compute_clover(double csw);
   void export_field(int kind, double* host_plain);  // kind 1 links, 2 clover (FP64)
   void gaussian_spinor(int prec, void* dev, unsigned long long seed);
@@ -198,6 +201,7 @@ class Lattice {
   double *Ueo_d_ = nullptr, *Ceo_d_ = nullptr;  // even/odd fields
   float *Ueo_s_ = nullptr, *Ceo_s_ = nullptr;
   bool half_valid_ = false;
+  bool owns_gauge_ = true;
   bool use_fused_ = true;
   // reductions
   double* part_ = nullptr;

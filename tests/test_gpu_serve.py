@@ -1,0 +1,63 @@
+"""Solve server: resident gauge fields, requests of mixed volume / kappa / RHS count
+admitted into slots as slots and memory free up, running at the same time on their
+own streams.  Each result is the same alone and in the mix (bit for bit: equal
+digests of the FP64 solutions) and meets the FP64 true residual."""
+import numpy as np
+import pytest
+
+import qws_amd as Q
+
+pytestmark = pytest.mark.gpu
+
+FIELDS = [([8, 8, 8, 8], [4, 4, 4, 4], 0.3, 8), ([8, 8, 8, 16], [4, 4, 4, 4], 0.3, 9),
+          ([16, 8, 8, 8], [4, 4, 4, 4], 0.3, 10)]
+PRM = dict(nsap=4, nm=2, tol=1e-11, maxiter=40, tol_s=1e-3, maxiter_s=40, true_tol=1e-10, max_restarts=4)
+
+
+def _trace(n, seed=7, rate=200.0):
+    rng = np.random.default_rng(seed)
+    t = np.cumsum(rng.exponential(1.0 / rate, n))
+    return [dict(field=int(rng.integers(len(FIELDS))), kappa=float(rng.uniform(0.110, 0.125)),
+                 nrhs=int(rng.integers(1, 5)), seed=100 + i, arrival=float(t[i])) for i in range(n)]
+
+
+@pytest.fixture(scope="module")
+def server():
+    s = Q.Server(FIELDS, links32=12, slots=4, **PRM)
+    yield s
+    s.close()
+
+
+def test_same_alone_and_in_the_mix(server):
+    tr = _trace(10)
+    mix = server.run(tr)
+    for i, r in enumerate(tr):
+        alone = server.run([dict(r, arrival=0.0)])[0]
+        assert alone["digest"] == mix[i]["digest"], i
+        assert alone["inner_iters"] == mix[i]["inner_iters"]
+        assert mix[i]["true_residual_max"] <= 1e-10
+    # admitted in arrival order, never before arrival
+    adm = [m["admit"] for m in mix]
+    arr = [m["arrival"] for m in mix]
+    order = np.argsort(arr, kind="stable")
+    assert all(adm[order[k]] <= adm[order[k + 1]] + 1e-9 for k in range(len(order) - 1))
+    assert all(m["admit"] >= m["arrival"] - 1e-6 for m in mix)
+    # with 4 slots some requests ran at the same time
+    iv = sorted((m["admit"], m["finish"]) for m in mix)
+    assert any(iv[k + 1][0] < iv[k][1] for k in range(len(iv) - 1))
+
+
+def test_memory_admission():
+    """A budget that holds one request at a time serialises the requests."""
+    one = max(1.1 * int(np.prod(f[0])) * (864 + 2976 * 4) + 64 * 2 ** 20 for f in FIELDS)
+    s = Q.Server(FIELDS, links32=12, slots=4, mem_budget=one + 1.0, **PRM)
+    try:
+        tr = _trace(6, seed=3, rate=1000.0)
+        for r in tr:
+            r["nrhs"] = 4
+        res = s.run(tr)
+    finally:
+        s.close()
+    iv = sorted((m["admit"], m["finish"]) for m in res)
+    assert all(iv[k + 1][0] >= iv[k][1] - 1e-6 for k in range(len(iv) - 1))
+    assert all(m["true_residual_max"] <= 1e-10 for m in res)
