@@ -55,6 +55,51 @@ def run_solves(Q, lat, b, n, params):
     return time.perf_counter() - t0, stats, x
 
 
+def batched_point_sources(Q, lat, params):
+    """The 12 spin-colour point sources at the origin solved as one batch."""
+    bs = [lat.point(Q.F64, (0, 0, 0, 0), sp, c) for c in range(3) for sp in range(4)]
+    xs = [lat.spinor(Q.F64) for _ in bs]
+    # warm-up: one outer iteration allocates the batch's work buffers
+    lat.solve_batch(xs, bs, **dict(params, maxiter=2, true_tol=-1.0, max_restarts=0))
+    for x in xs:
+        x.zero()
+    lat.sync()
+    t0 = time.perf_counter()
+    st = lat.solve_batch(xs, bs, **params)
+    lat.sync()
+    dt = time.perf_counter() - t0
+    return dt, st
+
+
+SERVE_VOLUMES = ([8, 8, 8, 8], [16, 16, 16, 32], [24, 24, 24, 48], [32, 32, 32, 64])
+
+
+def serve_trace(rate, n, seed=7):
+    """Poisson arrivals at `rate` req/s; volume uniform over SERVE_VOLUMES (field i has
+    volume i), kappa ~ U[0.125, 0.137], 1..12 Gaussian right-hand sides."""
+    rng = np.random.default_rng(seed)
+    t = np.cumsum(rng.exponential(1.0 / rate, n))
+    return [dict(field=int(rng.integers(4)), kappa=float(rng.uniform(0.125, 0.137)), nrhs=int(rng.integers(1, 13)),
+                 seed=1000 + i, arrival=float(t[i])) for i in range(n)]
+
+
+def serve_stats(res, trace):
+    lat_ms = np.array([(r["finish"] - r["arrival"]) * 1e3 for r in res])
+    svc_ms = np.array([(r["finish"] - r["admit"]) * 1e3 for r in res])
+    span = max(r["finish"] for r in res) - min(r["arrival"] for r in res)
+    rhs = sum(t["nrhs"] for t in trace)
+    return {"requests": len(res), "rhs": rhs, "solves_per_s": rhs / span, "requests_per_s": len(res) / span,
+            "p50_ms": float(np.percentile(lat_ms, 50)), "p99_ms": float(np.percentile(lat_ms, 99)),
+            "service_p50_ms": float(np.percentile(svc_ms, 50)), "service_p99_ms": float(np.percentile(svc_ms, 99)),
+            "true_residual_max": max(r["true_residual_max"] for r in res)}
+
+
+def make_server(Q, slots=4):
+    fields = [(L, [4, 4, 4, 4], 0.3, 8 + i) for i, L in enumerate(SERVE_VOLUMES)]
+    return Q.Server(fields, links32=WORKLOAD["links32"], tbc=WORKLOAD["tbc"], csw=WORKLOAD["csw"], slots=slots,
+                    **SOLVE)
+
+
 def cpu_baseline(Q, lat, stats, workdir):
     """QWS reference (oracle/_ref/libqws_ref.so) on this host: time per inner
     iteration (bicgstab_precdd_s_, nsap=4, nm=2) on the same lattice and fields, in
@@ -101,6 +146,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--dslash-only", type=int, default=0, help="only time N dslash launches (for counter runs)")
     ap.add_argument("--sweep", action="store_true", help="print solve statistics for a grid of solver settings")
+    ap.add_argument("--serve", action="store_true",
+                    help="serving sweep: Poisson rates 5..200 req/s over four resident fields (p50/p99 latency)")
+    ap.add_argument("--serve-requests", type=int, default=24)
     ap.add_argument("--copy-calib", type=int, default=0,
                     help="only run N device-to-device copies of 1 GiB (HBM counter calibration)")
     a = ap.parse_args()
@@ -123,6 +171,15 @@ def main():
         return
     import qws_amd as Q
 
+    if a.serve:
+        srv = make_server(Q)
+        srv.run(serve_trace(1000.0, 4, seed=1))  # warm-up
+        for rate in (5, 10, 20, 50, 100, 200):
+            tr = serve_trace(rate, a.serve_requests)
+            res = srv.run(tr)
+            print(json.dumps({"rate_req_per_s": rate, **serve_stats(res, tr)}), flush=True)
+        srv.close()
+        return
     lat, b = setup(Q)
     V = lat.V
     if a.dslash_only:
@@ -172,6 +229,30 @@ def main():
                      "ms_per_launch": ms_d},
         "cpu_baseline": None,
     }
+    # batched: the 12 point sources on the same field, one batch
+    dt_b, st_b = batched_point_sources(Q, lat, SOLVE)
+    line["batched"] = {"workload": "12 spin-colour point sources at the origin, one batched solve, 32^3x64",
+                       "solves_per_s": 12 / dt_b, "vs_plain": (12 / dt_b) / line["value"],
+                       "true_residual_max": max(s["true_residual"] for s in st_b),
+                       "inner_iters": [s["inner_iters"] for s in st_b]}
+    # uneven batch and serving: four resident fields of different volume
+    srv = make_server(Q, slots=4)
+    srv.run(serve_trace(1000.0, 4, seed=1))  # warm-up
+    rng = np.random.default_rng(4)
+    uneven = [dict(field=i, kappa=float(rng.uniform(0.130, 0.137)), nrhs=1, seed=4 + k, arrival=0.0)
+              for k, i in enumerate((1, 2, 3))]
+    res = srv.run(uneven)
+    span = max(r["finish"] for r in res)
+    line["uneven"] = {"workload": "16^3x32, 24^3x48, 32^3x64 (one Gaussian source each, seeds 4-6, kappa "
+                                  + ", ".join("%.4f" % u["kappa"] for u in uneven) + ") on concurrent streams",
+                      "solves_per_s": 3 / span, "seconds": span,
+                      "true_residual_max": max(r["true_residual_max"] for r in res)}
+    tr = serve_trace(20.0, 16)
+    res = srv.run(tr)
+    line["served"] = {"workload": "Poisson 20 req/s, 16 requests, volumes 8^4/16^3x32/24^3x48/32^3x64, "
+                                  "kappa U[0.125,0.137], 1-12 RHS, 4 slots",
+                      **serve_stats(res, tr)}
+    srv.close()
     if not a.no_cpu_baseline:
         with tempfile.TemporaryDirectory() as td:
             line["cpu_baseline"] = cpu_baseline(Q, lat, stats, td)
