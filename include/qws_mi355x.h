@@ -139,6 +139,9 @@ void one_minus_deo_dag_vm_(int* pe, int* po, scd_t* out, scd_t* in, scd_t* in0);
 void mtilde_vm_(scd_t* out, scd_t* in);
 void mtilde_dag_vm_(scd_t* out, scd_t* in);
 void mtdagmt_vm_(scd_t* out, scd_t* in);
+/* bicgstab.cc:65 -- FP64 BiCGStab on mtilde_vm_ (x: initial guess in, solution out;
+ * stops at sqrt(|r|^2/|b|^2) < 1e-14 or after maxiter-1 iterations) */
+void bicgstab_vm_(scd_t* x, scd_t* b, int* conviter, int* maxiter);
 /* FP32: deo_in_s.cc:65,191,317; deo_out_s.cc:93,155,229; qws.cc:576-605; clover_s.cc (clv_s_) */
 void deo_in_s_(int* pe, int* po, scs_t* out, scs_t* in);
 void dee_deo_in_s_(int* pe, int* po, scs_t* out, scs_t* in);
@@ -276,6 +279,8 @@ int qwsg_eo_spinor_download(qwsg_lattice* lat, int prec, const void* dev, double
 int qwsg_deo(qwsg_lattice* lat, int prec, int pe, int set, int dag, int clv, int mode, double fac, void* out,
              const void* in, const void* in0);
 int qwsg_mtilde(qwsg_lattice* lat, int prec, int pce, int which, void* out, const void* in);
+/* bicgstab.cc:65 -- FP64 BiCGStab on mtilde (parity pce) from the initial guess in x */
+int qwsg_eo_bicgstab(qwsg_lattice* lat, void* x, const void* b, int maxiter, double tol, int pce, int* iters);
 int qwsg_norm2(qwsg_lattice* lat, int prec, const void* a, double* out);
 
 /* No counterpart in the reference (it solves one source per call, serially): a solve

@@ -78,6 +78,7 @@ def lib():
         L.qwsg_eo_spinor_download.argtypes = [vp, ctypes.c_int, vp, vp]
         L.qwsg_deo.argtypes = [vp] + [ctypes.c_int] * 6 + [ctypes.c_double, vp, vp, vp]
         L.qwsg_mtilde.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, vp, vp]
+        L.qwsg_eo_bicgstab.argtypes = [vp, vp, vp, ctypes.c_int, ctypes.c_double, ctypes.c_int, ip]
         L.qwsg_bicgstab_precdd_h.argtypes = [vp, vp, vp, ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_int, ip]
         L.qwsg_solve.argtypes = [vp, vp, vp, ctypes.POINTER(SolveParams), ctypes.POINTER(SolveStats)]
         L.qwsg_norm2.argtypes = [vp, ctypes.c_int, vp, ctypes.POINTER(ctypes.c_double)]
@@ -365,6 +366,12 @@ class Lattice:
 
     def mtilde(self, out, inp, pce=0, which=0):
         check(lib().qwsg_mtilde(self.h, inp.prec, pce, which, out.ptr, inp.ptr))
+
+    def eo_bicgstab(self, x, b, maxiter=100, tol=1e-14, pce=0):
+        """BiCGStab on mtilde (FP64 parity fields); x holds the initial guess."""
+        it = ctypes.c_int()
+        check(lib().qwsg_eo_bicgstab(self.h, x.ptr, b.ptr, maxiter, float(tol), pce, ctypes.byref(it)))
+        return it.value
 
     def msap_h(self, out, inp, nsap, nm):
         """out = M_SAP inp with FP16-stored SAP iterates and fields (FP32 vectors)."""

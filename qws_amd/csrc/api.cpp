@@ -294,6 +294,13 @@ int qwsg_server_destroy(qwsg_server* h) {
     }
   });
 }
+int qwsg_eo_bicgstab(qwsg_lattice* h, void* x, const void* b, int maxiter, double tol, int pce, int* iters) {
+  return guarded([&] {
+    const double k = h->lat->kappa();
+    const int it = h->lat->bicgstab_eo((double*)x, (const double*)b, maxiter, tol, pce, -k * k);
+    if (iters) *iters = it;
+  });
+}
 int qwsg_norm2(qwsg_lattice* h, int prec, const void* a, double* out) {
   return guarded([&] { *out = h->lat->norm2(prec, a); });
 }
@@ -842,6 +849,21 @@ void one_minus_deo_dag_vm_(int* pe, int* po, scd_t* out, scd_t* in, scd_t* in0) 
   eo_call(__func__, qws::P64, pe, po, 0, 1, 0, 2, -kappa2, out, in, in0);
 }
 void mtilde_vm_(scd_t* out, scd_t* in) { eo_mtilde(__func__, qws::P64, 0, -kappa2, out, in); }
+// bicgstab.cc:65 -- BiCGStab on mtilde_vm_ from the initial guess in x, stopping at
+// sqrt(|r|^2/|b|^2) < 1e-14
+void bicgstab_vm_(scd_t* x, scd_t* b, int* conviter, int* maxiter) {
+  if (!g_qws.pce || (*g_qws.pce != 0 && *g_qws.pce != 1)) fatal(__func__, "pce must point at 0 or 1");
+  drop_in(__func__, [&] {
+    Lattice& L = lat_sync_eo("bicgstab_vm_");
+    double* xx = (double*)eobuf(qws::P64, 0);
+    double* bb = (double*)eobuf(qws::P64, 2);
+    L.eo_upload_spinor(qws::P64, xx, x, 1, QWS_VLEND, 1);
+    L.eo_upload_spinor(qws::P64, bb, b, 1, QWS_VLEND, 1);
+    const int it = L.bicgstab_eo(xx, bb, *maxiter, 1e-14, *g_qws.pce, -kappa2);
+    L.eo_download_spinor(qws::P64, xx, x, 1, QWS_VLEND, 1);
+    if (conviter) *conviter = it;
+  });
+}
 void mtilde_dag_vm_(scd_t* out, scd_t* in) { eo_mtilde(__func__, qws::P64, 1, -kappa2, out, in); }
 void mtdagmt_vm_(scd_t* out, scd_t* in) { eo_mtilde(__func__, qws::P64, 2, -kappa2, out, in); }
 

@@ -89,7 +89,7 @@ Lattice::~Lattice() {
     if (p) (void)hipFree(p);
   if (!owns_gauge_) Ud_ = nullptr, Us_ = nullptr;
   for (auto* p : {(void*)Ud_, (void*)Cd_, (void*)Us_, (void*)Cs_, (void*)Uh_, (void*)Ch_, (void*)part_, (void*)sc_in_,
-                  (void*)sc_one_, (void*)sc_h_, (void*)d_inner_mask_, (void*)d_blocks_[0], (void*)d_blocks_[1]})
+                  (void*)sc_one_, (void*)sc_h_, (void*)sc_eo_, (void*)d_inner_mask_, (void*)d_blocks_[0], (void*)d_blocks_[1]})
     if (p) (void)hipFree(p);
   for (auto& w : work_) (void)hipFree(w.second.first);
   if (sc_host_) (void)hipHostFree(sc_host_);
@@ -387,22 +387,23 @@ void Lattice::sap_sweeps(const Batch& B, const S* in, S* s, S* q, S* tmp, int ns
     }
 }
 
-// Fused path: one launch per half-sweep (sap.hip), for FP32 storage, proper
+// Fused path: one launch per half-sweep (sap.hip), for FP32 or FP16 storage, proper
 // colouring and 256-site blocks.
 template <typename S> bool Lattice::fused_ok() const {
-  return std::is_same<S, float>::value && use_fused_ && !improper_ && sap_fusable(g_);
+  return !std::is_same<S, double>::value && use_fused_ && !improper_ && sap_fusable(g_);
 }
 
 // the nsap sweeps of qws.cc:809-841 with the face update of each half-sweep
 // done at the start of the next one
-void Lattice::sap_sweeps_fused(const Batch& B, const float* in, float* s, float* q, int nsap, int nm) {
-  const FieldSet<float> f = fields<float>();
+template <typename S>
+void Lattice::sap_sweeps_fused(const Batch& B, const S* in, S* s, S* q, int nsap, int nm) {
+  const FieldSet<S> f = fields<S>();
   for (int isap = 0; isap < nsap; ++isap)
     for (int c = 0; c < 2; ++c) {
       int flags = SAP_EPI_SUPDATE;
       if (isap == 0) flags |= SAP_SRC_IN;
       if (isap > 0 || c == 1) flags |= SAP_FACE_PRO;
-      launch_sap_block(g_, sel(c), B, f.nl, (float)kappa_, nm, flags, s, in, q, q, nullptr, f.U, f.C, st_);
+      launch_sap_block<S>(g_, sel(c), B, f.nl, (float)kappa_, nm, flags, s, in, q, q, nullptr, f.U, f.C, st_);
     }
 }
 
@@ -419,12 +420,12 @@ void Lattice::prec_ddd_t(const Batch& B, S* out, const S* in, int nsap, int nm) 
     if (fused_ok<S>()) {
       const FieldSet<float> f = fields<float>();
       const int src = nsap == 0 ? SAP_SRC_IN : 0;
-      sap_sweeps_fused(B, in, s, q, nsap, nm);
+      sap_sweeps_fused<float>(B, in, s, q, nsap, nm);
       // final e: q_e = A_e s_e, out_e = D_in q_e
-      launch_sap_block(g_, sel(0), B, f.nl, (float)kappa_, nm, SAP_EPI_OUT | src | (nsap > 0 ? SAP_FACE_PRO : 0), s,
+      launch_sap_block<float>(g_, sel(0), B, f.nl, (float)kappa_, nm, SAP_EPI_OUT | src | (nsap > 0 ? SAP_FACE_PRO : 0), s,
                        in, q, q, out, f.U, f.C, st_);
       // final o: s_o += kappa T^-1 H_face q, q_o = A_o s_o, out_o = D_oe q_e + D_in q_o
-      launch_sap_block(g_, sel(1), B, f.nl, (float)kappa_, nm, SAP_EPI_OUT_FACE | src | SAP_FACE_PRO, s, in, q, q,
+      launch_sap_block<float>(g_, sel(1), B, f.nl, (float)kappa_, nm, SAP_EPI_OUT_FACE | src | SAP_FACE_PRO, s, in, q, q,
                        out, f.U, f.C, st_);
       hop<S>(sel(0), B, HOPS_FACE, -k, out, out, nullptr, nullptr, q);  // out_e += D_eo q_o
       return;
@@ -457,15 +458,15 @@ void Lattice::prec_s_t(const Batch& B, S* out, const S* in, int nsap, int nm) {
   const int pr = sizeof(S) == 8 ? P64 : sizeof(S) == 4 ? P32 : P16;
   S* s = (S*)work("sap_s", spinor_bytes(pr) * B.n);
   S* q = (S*)work("sap_q", spinor_bytes(pr) * B.n);
-  if constexpr (std::is_same<S, float>::value) {
+  if constexpr (!std::is_same<S, double>::value) {
     if (fused_ok<S>()) {
-      const FieldSet<float> f = fields<float>();
+      const FieldSet<S> f = fields<S>();
       const int src = nsap == 0 ? SAP_SRC_IN : 0;
-      sap_sweeps_fused(B, in, s, q, nsap, nm);
-      launch_sap_block(g_, sel(0), B, f.nl, (float)kappa_, nm, SAP_EPI_NONE | src | (nsap > 0 ? SAP_FACE_PRO : 0), s,
-                       in, q, out, nullptr, f.U, f.C, st_);
-      launch_sap_block(g_, sel(1), B, f.nl, (float)kappa_, nm, SAP_EPI_NONE | src | SAP_FACE_PRO, s, in, out, out,
-                       nullptr, f.U, f.C, st_);
+      sap_sweeps_fused<S>(B, in, s, q, nsap, nm);
+      launch_sap_block<S>(g_, sel(0), B, f.nl, (float)kappa_, nm, SAP_EPI_NONE | src | (nsap > 0 ? SAP_FACE_PRO : 0),
+                          s, in, q, out, nullptr, f.U, f.C, st_);
+      launch_sap_block<S>(g_, sel(1), B, f.nl, (float)kappa_, nm, SAP_EPI_NONE | src | SAP_FACE_PRO, s, in, out, out,
+                          nullptr, f.U, f.C, st_);
       return;
     }
   }
@@ -641,6 +642,44 @@ void Lattice::mtilde(int prec, int pce, int which, double fac, void* out, const 
     mt(t2, in);
     mtdag(out, t2);
   }
+}
+
+int Lattice::bicgstab_eo(double* x, const double* b, int maxiter, double tol, int pce, double fac) {
+  const long long n = g_.V / 2 * 12;  // complex components of a parity field
+  const size_t bytes = eo_spinor_bytes(P64);
+  double* q = (double*)work("eob_q", bytes);
+  double* r = (double*)work("eob_r", bytes);
+  double* p = (double*)work("eob_p", bytes);
+  double* t = (double*)work("eob_t", bytes);
+  double* r0 = (double*)work("eob_r0", bytes);
+  if (!sc_eo_) sc_eo_ = dalloc<Scalars>(1);
+  double* part = reduction_partials(1);
+  const Batch B = single();
+  mtilde(P64, pce, 0, fac, q, x);
+  launch_bicg_init_outer<double>(B, n, r, p, r0, b, q, part, st_);
+  launch_finalize(B, part, 2, sc_eo_, FIN_INIT_OUTER, st_);
+  auto converged = [&] {
+    read_scalars(sc_eo_, 1);
+    return std::sqrt(sc_host_[0].rnorm / sc_host_[0].bnorm) < tol;
+  };
+  int iter;
+  for (iter = 1; iter < maxiter; ++iter) {
+    mtilde(P64, pce, 0, fac, q, p);
+    launch_dot<double>(B, n, r0, q, part, st_);
+    launch_finalize(B, part, 2, sc_eo_, FIN_ALPHA, st_);
+    launch_update_xr<double>(B, n, x, p, r, q, sc_eo_, 0, nullptr, part, st_);
+    launch_finalize(B, part, 1, sc_eo_, FIN_RNORM, st_);
+    if (converged()) break;
+    mtilde(P64, pce, 0, fac, t, r);
+    launch_norm_dot<double>(B, n, t, r, part, st_);
+    launch_finalize(B, part, 3, sc_eo_, FIN_OMEGA, st_);
+    launch_update_xr<double>(B, n, x, r, r, t, sc_eo_, 1, r0, part, st_);
+    launch_finalize(B, part, 3, sc_eo_, FIN_RHO, st_);
+    if (converged()) break;
+    launch_update_p<double>(B, n, p, r, q, sc_eo_, st_);
+  }
+  sync();
+  return iter;
 }
 
 // ------------------------------------------------------------------ inner BiCGStab

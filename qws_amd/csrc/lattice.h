@@ -158,6 +158,10 @@ class Lattice {
   // which 0: mtilde = 1 + fac T^-1 H T^-1 H on parity pce (qws.cc:551-558; fac = -kappa^2),
   // 1: its adjoint (qws.cc:559-568), 2: mtilde^+ mtilde (qws.cc:569-574)
   void mtilde(int prec, int pce, int which, double fac, void* out, const void* in);
+  // bicgstab_vm_ (bicgstab.cc:65-290): FP64 BiCGStab on mtilde (parity pce, fac =
+  // -kappa^2) from the initial guess in x, iterations 1..maxiter-1, stopping when
+  // sqrt(|r|^2/|b|^2) < tol (the reference's 1e-14); returns the last iteration
+  int bicgstab_eo(double* x, const double* b, int maxiter, double tol, int pce, double fac);
 
   // ---- batched building blocks (used by OuterBatch)
   void dslash_batch(const Batch& B, double* out, const double* in);
@@ -178,7 +182,7 @@ class Lattice {
   template <typename S> void jinv_t(const Batch& B, int colour, S* x, const S* b, S* tmp, int nm);
   template <typename S> void sap_sweeps(const Batch& B, const S* in, S* s, S* q, S* tmp, int nsap, int nm);
   template <typename S> bool fused_ok() const;
-  void sap_sweeps_fused(const Batch& B, const float* in, float* s, float* q, int nsap, int nm);
+  template <typename S> void sap_sweeps_fused(const Batch& B, const S* in, S* s, S* q, int nsap, int nm);
   template <typename S> void prec_ddd_t(const Batch& B, S* out, const S* in, int nsap, int nm);
   template <typename S> void prec_s_t(const Batch& B, S* out, const S* in, int nsap, int nm);
   void msap_h_t(const Batch& B, float* out, const float* in, int nsap, int nm);
@@ -211,6 +215,7 @@ class Lattice {
   Scalars* sc_one_ = nullptr;
   Scalars* sc_host_ = nullptr;  // pinned, room for MAX_BATCH
   Scalars* sc_h_ = nullptr;     // FP16 SAP input norms, room for MAX_BATCH
+  Scalars* sc_eo_ = nullptr;    // even/odd solver
   int* d_inner_mask_ = nullptr;
   std::vector<std::pair<std::string, std::pair<void*, size_t>>> work_;
 

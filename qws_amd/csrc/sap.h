@@ -18,8 +18,8 @@ enum SapFlags {
 };
 
 bool sap_fusable(const Geo& g);
-void launch_sap_block(const Geo& g, BlockSel sel, const Batch& B, int nl, float kappa, int nm, int flags, float* s,
-                      const float* in, const float* qface, float* qout, float* out, const float* U, const float* C,
-                      hipStream_t st);
+template <typename S>
+void launch_sap_block(const Geo& g, BlockSel sel, const Batch& B, int nl, float kappa, int nm, int flags, S* s,
+                      const S* in, const S* qface, S* qout, S* out, const S* U, const S* C, hipStream_t st);
 
 }  // namespace qws

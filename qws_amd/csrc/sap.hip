@@ -10,17 +10,28 @@
 // The block's forward links and the iterate x live in LDS, the site's clover
 // inverse in registers, so links and clover cross HBM once per half-sweep instead
 // of once per operator application.  Requires proper colouring (face neighbours
-// belong to the other colour) and Vb == SAP_THREADS sites per block.
+// belong to the other colour) and Vb == SAP_THREADS sites per block.  Instantiated
+// for FP32 storage and for FP16 storage with FP32 arithmetic.
 #include "kernels.h"
 #include "sap.h"
 
 namespace qws {
 
-template <int NL>
-__device__ __forceinline__ void load_link_lds(const float* lU, int mu, int t, Cx<float> u[9]) {
+// LDS element type: the storage type (FP16 links and iterate stay FP16 in LDS)
+__device__ __forceinline__ float lds_get(const float* p, int i) { return p[i]; }
+__device__ __forceinline__ float lds_get(const __half* p, int i) { return __half2float(p[i]); }
+__device__ __forceinline__ void lds_put(float* p, int i, float v) { p[i] = v; }
+__device__ __forceinline__ void lds_put(__half* p, int i, float v) { p[i] = __float2half_rn(v); }
+// value as stored: identity for FP32, binary16 rounding for FP16
+__device__ __forceinline__ float stored(float v, const float*) { return v; }
+__device__ __forceinline__ float stored(float v, const __half*) { return __half2float(__float2half_rn(v)); }
+
+template <typename S, int NL>
+__device__ __forceinline__ void load_link_lds(const S* lU, int mu, int t, Cx<float> u[9]) {
   constexpr int NC = NL / 2;
 #pragma unroll
-  for (int k = 0; k < NC; ++k) u[k] = {lU[((mu * NC + k) * 2 + 0) * SAP_THREADS + t], lU[((mu * NC + k) * 2 + 1) * SAP_THREADS + t]};
+  for (int k = 0; k < NC; ++k)
+    u[k] = {lds_get(lU, ((mu * NC + k) * 2 + 0) * SAP_THREADS + t), lds_get(lU, ((mu * NC + k) * 2 + 1) * SAP_THREADS + t)};
   if (NL == 12) {
     Cx<float> a = cmul(u[1], u[5]), b = cmul(u[2], u[4]);
     u[6] = cconj(csub(a, b));
@@ -31,21 +42,23 @@ __device__ __forceinline__ void load_link_lds(const float* lU, int mu, int t, Cx
   }
 }
 
-__device__ __forceinline__ void lds_load_spinor(const float* lx, int t, Cx<float> v[12]) {
+template <typename S>
+__device__ __forceinline__ void lds_load_spinor(const S* lx, int t, Cx<float> v[12]) {
 #pragma unroll
-  for (int k = 0; k < 12; ++k) v[k] = {lx[(2 * k) * SAP_THREADS + t], lx[(2 * k + 1) * SAP_THREADS + t]};
+  for (int k = 0; k < 12; ++k) v[k] = {lds_get(lx, (2 * k) * SAP_THREADS + t), lds_get(lx, (2 * k + 1) * SAP_THREADS + t)};
 }
-__device__ __forceinline__ void lds_store_spinor(float* lx, int t, const Cx<float> v[12]) {
+template <typename S>
+__device__ __forceinline__ void lds_store_spinor(S* lx, int t, const Cx<float> v[12]) {
 #pragma unroll
   for (int k = 0; k < 12; ++k) {
-    lx[(2 * k) * SAP_THREADS + t] = v[k].re;
-    lx[(2 * k + 1) * SAP_THREADS + t] = v[k].im;
+    lds_put(lx, (2 * k) * SAP_THREADS + t, v[k].re);
+    lds_put(lx, (2 * k + 1) * SAP_THREADS + t, v[k].im);
   }
 }
 
 // in-block hop along MU, DIR from LDS
-template <int NL, int MU, int DIR>
-__device__ __forceinline__ void lds_hop(const Geo& g, const SiteC& c, int t, const float* lx, const float* lU,
+template <typename S, int NL, int MU, int DIR>
+__device__ __forceinline__ void lds_hop(const Geo& g, const SiteC& c, int t, const S* lx, const S* lU,
                                         Cx<float> acc[12]) {
   const int l = c.l[MU] + DIR;
   if (l < 0 || l >= g.B[MU]) return;
@@ -53,35 +66,80 @@ __device__ __forceinline__ void lds_hop(const Geo& g, const SiteC& c, int t, con
   Cx<float> v[12], a[6], b[6], u[9];
   lds_load_spinor(lx, tn, v);
   project<float, MU, DIR>(v, a);
-  load_link_lds<NL>(lU, MU, DIR > 0 ? t : tn, u);
+  load_link_lds<S, NL>(lU, MU, DIR > 0 ? t : tn, u);
   link_mult<float, DIR>(u, a, b);
   reconstruct<float, MU, DIR>(b, 2.0f, acc);
 }
 
-template <int NL>
-__device__ __forceinline__ void lds_hop_sum(const Geo& g, const SiteC& c, int t, const float* lx, const float* lU,
+template <typename S, int NL>
+__device__ __forceinline__ void lds_hop_sum(const Geo& g, const SiteC& c, int t, const S* lx, const S* lU,
                                             Cx<float> acc[12]) {
 #pragma unroll
   for (int k = 0; k < 12; ++k) acc[k] = {0, 0};
-  lds_hop<NL, 0, +1>(g, c, t, lx, lU, acc);
-  lds_hop<NL, 0, -1>(g, c, t, lx, lU, acc);
-  lds_hop<NL, 1, +1>(g, c, t, lx, lU, acc);
-  lds_hop<NL, 1, -1>(g, c, t, lx, lU, acc);
-  lds_hop<NL, 2, +1>(g, c, t, lx, lU, acc);
-  lds_hop<NL, 2, -1>(g, c, t, lx, lU, acc);
-  lds_hop<NL, 3, +1>(g, c, t, lx, lU, acc);
-  lds_hop<NL, 3, -1>(g, c, t, lx, lU, acc);
+  lds_hop<S, NL, 0, +1>(g, c, t, lx, lU, acc);
+  lds_hop<S, NL, 0, -1>(g, c, t, lx, lU, acc);
+  lds_hop<S, NL, 1, +1>(g, c, t, lx, lU, acc);
+  lds_hop<S, NL, 1, -1>(g, c, t, lx, lU, acc);
+  lds_hop<S, NL, 2, +1>(g, c, t, lx, lU, acc);
+  lds_hop<S, NL, 2, -1>(g, c, t, lx, lU, acc);
+  lds_hop<S, NL, 3, +1>(g, c, t, lx, lU, acc);
+  lds_hop<S, NL, 3, -1>(g, c, t, lx, lU, acc);
 }
 
-// clover inverse held in registers: a[0..35] chirality 0, a[36..71] chirality 1
-__device__ __forceinline__ void clover_regs(const float cl[72], Cx<float> v[12]) {
+// y = A x for the packed Hermitian 6x6 whose reals k sit in a(k)
+template <typename A>
+__device__ __forceinline__ void herm6_mult_f(A a, const Cx<float> x[6], Cx<float> y[6]) {
+#pragma unroll
+  for (int r = 0; r < 6; ++r) y[r] = cscale(a(r), x[r]);
+  int p = 6;
+#pragma unroll
+  for (int r = 0; r < 6; ++r)
+#pragma unroll
+    for (int c = r + 1; c < 6; ++c) {
+      Cx<float> m = {a(p), a(p + 1)};
+      y[r] = cadd(y[r], cmul(m, x[c]));
+      y[c] = cadd(y[c], cmulc(m, x[r]));
+      p += 2;
+    }
+}
+
+// The site's clover inverse held in registers for the whole launch: 72 FP32 reals,
+// or for FP16 storage 36 packed pairs (converted to FP32 at each use).
+template <typename S> struct CloverRegs;
+template <> struct CloverRegs<float> {
+  float c[72];
+  __device__ __forceinline__ void load(const float* C, long long s, long long V) {
+    CloverIO<float>::load_chir(C, s, V, 0, c);
+    CloverIO<float>::load_chir(C, s, V, 1, c + 36);
+  }
+  __device__ __forceinline__ float get(int k) const { return c[k]; }
+};
+template <> struct CloverRegs<__half> {
+  __half2 c[36];
+  __device__ __forceinline__ void load(const __half* C, long long s, long long V) {
+    const Half4* p = reinterpret_cast<const Half4*>(C);
+#pragma unroll
+    for (int f = 0; f < 18; ++f) {
+      const Half4 h = p[(long long)f * V + s];
+      c[2 * f] = h.a;
+      c[2 * f + 1] = h.b;
+    }
+  }
+  __device__ __forceinline__ float get(int k) const {
+    const float2 f = __half22float2(c[k >> 1]);
+    return (k & 1) ? f.y : f.x;
+  }
+};
+
+template <typename S>
+__device__ __forceinline__ void clover_regs(const CloverRegs<S>& cl, Cx<float> v[12]) {
   Cx<float> x[6], y0[6], y1[6];
 #pragma unroll
   for (int c = 0; c < 3; ++c) { x[c] = cadd(v[c * 4 + 0], v[c * 4 + 2]); x[3 + c] = cadd(v[c * 4 + 1], v[c * 4 + 3]); }
-  herm6_mult(cl, x, y0);
+  herm6_mult_f([&](int k) { return cl.get(k); }, x, y0);
 #pragma unroll
   for (int c = 0; c < 3; ++c) { x[c] = csub(v[c * 4 + 0], v[c * 4 + 2]); x[3 + c] = csub(v[c * 4 + 1], v[c * 4 + 3]); }
-  herm6_mult(cl + 36, x, y1);
+  herm6_mult_f([&](int k) { return cl.get(36 + k); }, x, y1);
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
     v[c * 4 + 0] = cadd(y0[c], y1[c]);
@@ -91,14 +149,17 @@ __device__ __forceinline__ void clover_regs(const float cl[72], Cx<float> v[12])
   }
 }
 
-template <int NL>
+// Storage S = float: everything FP32.  S = __half: spinors, links, clover and the
+// LDS copies in FP16, arithmetic in FP32, every stored value rounded to FP16 where
+// the unfused FP16 path stores it (face-updated s, each Jacobi iterate, s, q).
+template <typename S, int NL>
 __global__ void __launch_bounds__(SAP_THREADS, 1)
 k_sap_block(Geo g, const int* __restrict__ list, int nrhs, long long stride, const int* __restrict__ active, float kappa,
-            int nm, int flags, float* s, const float* __restrict__ in, const float* qface, float* qout, float* out,
-            const float* __restrict__ U, const float* __restrict__ C) {
+            int nm, int flags, S* s, const S* __restrict__ in, const S* qface, S* qout, S* out,
+            const S* __restrict__ U, const S* __restrict__ C) {
   constexpr int NC = NL / 2;
-  __shared__ float lU[4 * NC * 2 * SAP_THREADS];
-  __shared__ float lx[24 * SAP_THREADS];
+  __shared__ S lU[4 * NC * 2 * SAP_THREADS];
+  __shared__ S lx[24 * SAP_THREADS];
   const int t = threadIdx.x;
   const int blk = list[xcd_remap(g, blockIdx.x, gridDim.x)];
   const long long site = (long long)blk * SAP_THREADS + t;
@@ -106,44 +167,39 @@ k_sap_block(Geo g, const int* __restrict__ list, int nrhs, long long stride, con
   site_coords(g, site, c);
 
   // forward links of this site into LDS (coalesced [mu][k][V] reads), once for all RHS
-  {
-    const float2* Ug = reinterpret_cast<const float2*>(U);
 #pragma unroll
-    for (int mu = 0; mu < 4; ++mu)
+  for (int mu = 0; mu < 4; ++mu)
 #pragma unroll
-      for (int k = 0; k < NC; ++k) {
-        float2 v = Ug[((long long)mu * NC + k) * g.V + site];
-        lU[((mu * NC + k) * 2 + 0) * SAP_THREADS + t] = v.x;
-        lU[((mu * NC + k) * 2 + 1) * SAP_THREADS + t] = v.y;
-      }
-  }
-  float cl[72];
-  CloverIO<float>::load_chir(C, site, g.V, 0, cl);
-  CloverIO<float>::load_chir(C, site, g.V, 1, cl + 36);
+    for (int k = 0; k < NC; ++k) {
+      const Cx<float> v = load_cx<S>(U, ((long long)mu * NC + k) * g.V + site);
+      lds_put(lU, ((mu * NC + k) * 2 + 0) * SAP_THREADS + t, v.re);
+      lds_put(lU, ((mu * NC + k) * 2 + 1) * SAP_THREADS + t, v.im);
+    }
+  CloverRegs<S> cl;
+  cl.load(C, site, g.V);
   const int epi = flags & SAP_EPI_MASK;
 
   for (int r = 0; r < nrhs; ++r) {
     if (active && !active[r]) continue;  // uniform over the workgroup
     const long long off = r * stride;
-    Cx<float> sv[12], acc[12], f[12];
-    SpinorIO<float>::load(((flags & SAP_SRC_IN) ? in : s) + off, site, g.V, sv);
+    Cx<float> sv[12], acc[12];
+    SpinorIO<S>::load(((flags & SAP_SRC_IN) ? in : s) + off, site, g.V, sv);
 
     if (flags & SAP_FACE_PRO) {
-      hop_sum<float, NL, HOPS_FACE>(g, site, c, qface + off, opaque(U), f);
+      Cx<float> f[12];
+      hop_sum<S, NL, HOPS_FACE>(g, site, c, qface + off, opaque(U), f);
       clover_regs(cl, f);
 #pragma unroll
-      for (int k = 0; k < 12; ++k) sv[k] = cadd(sv[k], cscale(kappa, f[k]));
+      for (int k = 0; k < 12; ++k) sv[k] = {stored(sv[k].re + kappa * f[k].re, lx), stored(sv[k].im + kappa * f[k].im, lx)};
     }
 
     // block solve: x_0 = s_c ; x_{k+1} = s_c + kappa T^-1 H_in x_k
     Cx<float> x[12];
-#pragma unroll
-    for (int k = 0; k < 12; ++k) x[k] = sv[k];
     __syncthreads();  // the previous RHS's readers of lx are done (and lU is complete)
-    lds_store_spinor(lx, t, x);
+    lds_store_spinor(lx, t, sv);
     __syncthreads();
     for (int it = 0; it < nm; ++it) {
-      lds_hop_sum<NL>(g, c, t, lx, lU, acc);
+      lds_hop_sum<S, NL>(g, c, t, lx, lU, acc);
       clover_regs(cl, acc);
 #pragma unroll
       for (int k = 0; k < 12; ++k) x[k] = cadd(sv[k], cscale(kappa, acc[k]));
@@ -151,46 +207,56 @@ k_sap_block(Geo g, const int* __restrict__ list, int nrhs, long long stride, con
       lds_store_spinor(lx, t, x);
       __syncthreads();
     }
-    SpinorIO<float>::store(qout + off, site, g.V, x);
+    lds_load_spinor(lx, t, x);  // x as stored
+    SpinorIO<S>::store(qout + off, site, g.V, x);
 
     if (epi == SAP_EPI_NONE) continue;
     // acc = T^-1 H_in x
-    lds_hop_sum<NL>(g, c, t, lx, lU, acc);
+    lds_hop_sum<S, NL>(g, c, t, lx, lU, acc);
     clover_regs(cl, acc);
     Cx<float> o[12];
     if (epi == SAP_EPI_SUPDATE) {
       // s_c = s_c + in_c - x + kappa T^-1 H_in x
       Cx<float> iv[12];
-      SpinorIO<float>::load(in + off, site, g.V, iv);
+      SpinorIO<S>::load(in + off, site, g.V, iv);
 #pragma unroll
       for (int k = 0; k < 12; ++k) o[k] = cadd(csub(cadd(sv[k], iv[k]), x[k]), cscale(kappa, acc[k]));
-      SpinorIO<float>::store(s + off, site, g.V, o);
+      SpinorIO<S>::store(s + off, site, g.V, o);
     } else {
-      // out_c = [- kappa f] + x - kappa T^-1 H_in x
+      // out_c = [- kappa T^-1 H_face q] + x - kappa T^-1 H_in x; the face term is
+      // s_c - sv (the prologue added kappa T^-1 H_face q to s_c)
 #pragma unroll
       for (int k = 0; k < 12; ++k) o[k] = csub(x[k], cscale(kappa, acc[k]));
       if (epi == SAP_EPI_OUT_FACE) {
+        Cx<float> s0[12];
+        SpinorIO<S>::load(((flags & SAP_SRC_IN) ? in : s) + off, site, g.V, s0);
 #pragma unroll
-        for (int k = 0; k < 12; ++k) o[k] = csub(o[k], cscale(kappa, f[k]));
+        for (int k = 0; k < 12; ++k) o[k] = cadd(o[k], csub(s0[k], sv[k]));
       }
-      SpinorIO<float>::store(out + off, site, g.V, o);
+      SpinorIO<S>::store(out + off, site, g.V, o);
     }
   }
 }
 
 bool sap_fusable(const Geo& g) { return g.Vb == SAP_THREADS; }
 
-void launch_sap_block(const Geo& g, BlockSel sel, const Batch& B, int nl, float kappa, int nm, int flags, float* s,
-                      const float* in, const float* qface, float* qout, float* out, const float* U, const float* C,
-                      hipStream_t st) {
+template <typename S>
+void launch_sap_block(const Geo& g, BlockSel sel, const Batch& B, int nl, float kappa, int nm, int flags, S* s,
+                      const S* in, const S* qface, S* qout, S* out, const S* U, const S* C, hipStream_t st) {
   if (sel.n == 0) return;
   if (nm < 1) nm = 1;  // the first application is unconditional (static_solver.cc:77-101)
 #define SAP_ARGS g, sel.list, B.n, B.stride, B.active, kappa, nm, flags, s, in, qface, qout, out, U, C
   if (nl == 12)
-    hipLaunchKernelGGL((k_sap_block<12>), dim3(sel.n), dim3(SAP_THREADS), 0, st, SAP_ARGS);
+    hipLaunchKernelGGL((k_sap_block<S, 12>), dim3(sel.n), dim3(SAP_THREADS), 0, st, SAP_ARGS);
   else
-    hipLaunchKernelGGL((k_sap_block<18>), dim3(sel.n), dim3(SAP_THREADS), 0, st, SAP_ARGS);
+    hipLaunchKernelGGL((k_sap_block<S, 18>), dim3(sel.n), dim3(SAP_THREADS), 0, st, SAP_ARGS);
 #undef SAP_ARGS
 }
+
+template void launch_sap_block<float>(const Geo&, BlockSel, const Batch&, int, float, int, int, float*, const float*,
+                                      const float*, float*, float*, const float*, const float*, hipStream_t);
+template void launch_sap_block<__half>(const Geo&, BlockSel, const Batch&, int, float, int, int, __half*,
+                                       const __half*, const __half*, __half*, __half*, const __half*, const __half*,
+                                       hipStream_t);
 
 }  // namespace qws

@@ -208,3 +208,36 @@ def test_mtilde_is_schur_complement_of_d():
     lat.mtilde(mm, lat.eo_spinor(Q.F64, x), 0, 2)
     lat.mtilde(md, mt, 0, 1)
     assert rel(mm.download(), md.download()) < 1e-14
+
+
+def test_dropin_bicgstab_vm_case0():
+    """bicgstab_vm_ on mtilde_vm_ with the driver's settings (main.cc:200-245: x from
+    zero, maxiter 100) reaches data/CASE0's solution (xnorm^2 under check.sh's 1e-14)."""
+    L = Q.lib()
+    nloc = [32, 6, 4, 3]
+    _, U, C, b, _ = O.case_inputs(nloc)
+    ci = lambda v: ctypes.byref(ctypes.c_int(v))
+    keep = [(ctypes.c_int * 4)(1, 1, 1, 1), (ctypes.c_int * 4)(1, 1, 1, -1), ctypes.c_int(0), ctypes.c_int(1),
+            (ctypes.c_int * 4)(1, 2, 2, 2)]
+    L.qws_init_(ci(32), ci(6), ci(4), ci(3), keep[0], keep[1], ctypes.byref(keep[2]), ctypes.byref(keep[3]), keep[4])
+    vold = ctypes.c_int.in_dll(L, "vold").value
+    glud = np.ctypeslib.as_array((ctypes.c_double * (vold * 8 * 144)).from_address(
+        ctypes.c_void_p.in_dll(L, "glud").value))
+    clvd = np.ctypeslib.as_array((ctypes.c_double * (vold * 2 * 576)).from_address(
+        ctypes.c_void_p.in_dll(L, "clvd").value))
+    glud[:] = RW.links_to_aosoa(U, nloc, 8).ravel()
+    clvd[:] = RW.clover_to_aosoa(C, nloc, 8).ravel()
+    ctypes.c_double.in_dll(L, "kappa").value = 0.05
+    ctypes.c_double.in_dll(L, "kappa2").value = 0.0025
+    L.qws_mi355x_fields_modified_()
+    bd = np.ascontiguousarray(RW.spinor_to_aosoa(b, nloc, 8).reshape(2 * vold, -1)[:vold])
+    x = np.zeros_like(bd)
+    it = ctypes.c_int(0)
+    L.bicgstab_vm_(_ptr(x), _ptr(bd), ctypes.byref(it), ci(100))
+    r = np.zeros_like(bd)
+    L.mtilde_vm_(_ptr(r), _ptr(x))
+    L.qws_finalize_()
+    assert 1 < it.value < 100
+    assert K.check_value("bnorm^2", O.norm2(bd), K.expected("CASE0", "bnorm^2", "EO"))
+    assert O.norm2(bd - r) < 1e-22
+    assert K.check_value("xnorm^2", O.norm2(x), K.expected("CASE0", "xnorm^2", "EO"))
