@@ -732,7 +732,8 @@ void Lattice::bicg_t(const Batch& Bin, S* x, const S* b, double tol, int maxiter
     bool changed = false, any = false;
     for (int k = 0; k < nb; ++k) {
       if (!live[k]) continue;
-      if (std::sqrt(sc_host_[k].rnorm / bnorm[k]) < tol) {
+      // a non-finite residual (breakdown) ends the RHS's iteration as well
+      if (std::sqrt(sc_host_[k].rnorm / bnorm[k]) < tol || !std::isfinite(sc_host_[k].rnorm)) {
         live[k] = 0;
         changed = true;
       } else {
@@ -884,7 +885,7 @@ void OuterBatch::check(std::vector<int>& live) {
   const Scalars* h = L_.host_scalars();
   bool changed = false;
   for (int k = 0; k < cap_; ++k)
-    if (live[k] && std::sqrt(h[k].rnorm / h[k].bnorm) < prm_.tol) {
+    if (live[k] && (std::sqrt(h[k].rnorm / h[k].bnorm) < prm_.tol || !std::isfinite(h[k].rnorm))) {
       live[k] = 0;
       state_[k] = FINISHED;
       changed = true;
