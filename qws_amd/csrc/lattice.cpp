@@ -390,6 +390,7 @@ void Lattice::sap_sweeps(const Batch& B, const S* in, S* s, S* q, S* tmp, int ns
 // Fused path: one launch per half-sweep (sap.hip), for FP32 or FP16 storage, proper
 // colouring and 256-site blocks.
 template <typename S> bool Lattice::fused_ok() const {
+  if (std::is_same<S, float>::value && g_.Vb == 512 && nl32_ == 18) return false;  // LDS
   return !std::is_same<S, double>::value && use_fused_ && !improper_ && sap_fusable(g_);
 }
 

@@ -5,7 +5,6 @@
 
 namespace qws {
 
-constexpr int SAP_THREADS = 256;  // sites per block handled by the fused kernel (e.g. 4^4)
 
 enum SapFlags {
   SAP_EPI_SUPDATE = 0,   // s_c = s_c + in_c - x + kappa T^-1 H_in x ; q_c = x

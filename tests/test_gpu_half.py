@@ -31,6 +31,7 @@ GEOMS = [
     ("case0", [32, 6, 4, 3], [16, 6, 4, 3]),   # improper colouring (QWS single rank)
     ("mixed", [8, 4, 2, 4], [4, 2, 2, 4]),     # z, t single block: improper
     ("b4", [16, 8, 8, 8], [4, 4, 4, 4]),       # 4^4 blocks, proper colouring
+    ("b512", [16, 8, 8, 8], [8, 4, 4, 4]),     # 512-site blocks (fused kernel)
 ]
 
 

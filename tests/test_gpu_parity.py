@@ -129,12 +129,14 @@ def test_prec_ddd_and_prec_s(prob, nsap, nm):
     assert rel(out.download(), p.orc.prec_s(bs, nsap, nm)) < 5e-6
 
 
-@pytest.mark.parametrize("L,B", [([16, 8, 8, 16], [4, 4, 4, 4]), ([16, 8, 8, 8], [8, 4, 4, 2])])
+@pytest.mark.parametrize("L,B,su3", [([16, 8, 8, 16], [4, 4, 4, 4], False), ([16, 8, 8, 8], [8, 4, 4, 2], False),
+                                     ([16, 8, 8, 8], [8, 4, 4, 4], True)])
 @pytest.mark.parametrize("nsap,nm", [(4, 2), (1, 3), (0, 2)])
-def test_fused_sap_matches_unfused_and_oracle(L, B, nsap, nm):
-    """The fused half-sweep kernel (256-site blocks, proper colouring) gives the
-    same prec_ddd_s_/prec_s_ as the step-by-step kernels and the oracle."""
-    p = Problem(L, B, kappa=0.125, seed=8)
+def test_fused_sap_matches_unfused_and_oracle(L, B, su3, nsap, nm):
+    """The fused half-sweep kernel (256- and 512-site blocks, proper colouring; the
+    512-site FP32 kernel needs 12-real links) gives the same prec_ddd_s_/prec_s_ as the
+    step-by-step kernels and the oracle."""
+    p = Problem(L, B, kappa=0.125, seed=8, su3=su3)
     bs = p.b.astype(np.float32)
     inp = p.lat.spinor(Q.F32, bs)
     res = {}
