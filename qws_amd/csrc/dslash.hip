@@ -67,6 +67,35 @@ k_hop(Geo g, const int* __restrict__ list, int wg_per_blk, int nrhs, long long s
   }
 }
 
+// Full operator out = in + gamma T^-1 H in for one right-hand side over every
+// site: the kernel for D itself (ddd_d_, ddd_s_ and the outer solver's D).  Written
+// without the generality of k_hop (RHS loop, a/b/c terms, block lists) it keeps the
+// live state of a thread small (about 80 VGPRs in FP32), so more waves hide the
+// neighbour-gather latency.
+template <typename S, int NL>
+__global__ void __launch_bounds__(HOP_THREADS)
+k_dslash(Geo g, const int* __restrict__ order, int wg_per_blk, const int* __restrict__ active,
+         typename Store<S>::R gamma, S* __restrict__ out, const S* __restrict__ in, const S* __restrict__ U,
+         const S* __restrict__ C) {
+  using R = typename Store<S>::R;
+  if (active && !*active) return;
+  const int wg = xcd_remap(g, blockIdx.x, gridDim.x);
+  const int bi = wg / wg_per_blk;
+  const int loc = (wg - bi * wg_per_blk) * HOP_THREADS + threadIdx.x;
+  if (loc >= g.Vb) return;
+  const long long s = (long long)order[bi] * g.Vb + loc;
+  SiteC sc;
+  site_coords(g, s, sc);
+  Cx<R> acc[12];
+  hop_sum<S, NL, HOPS_ALL>(g, s, sc, in, U, acc);
+  clover_apply<S>(C, s, g.V, acc);
+  Cx<R> v[12];
+  SpinorIO<S>::load(in, s, g.V, v);
+#pragma unroll
+  for (int k = 0; k < 12; ++k) v[k] = cadd(v[k], cscale(gamma, acc[k]));
+  SpinorIO<S>::store(out, s, g.V, v);
+}
+
 template <typename S>
 __global__ void __launch_bounds__(HOP_THREADS)
 k_zero_blocks(Geo g, const int* list, int wg_per_blk, int nrhs, long long stride, const int* active, S* out) {
@@ -86,6 +115,14 @@ k_zero_blocks(Geo g, const int* list, int wg_per_blk, int nrhs, long long stride
 template <typename S, int NL>
 void launch_hop(const Geo& g, BlockSel sel, const Batch& B, int set, typename Store<S>::R gamma, S* out, const S* a,
                 const S* b, const S* c, const S* in, const S* U, const S* C, hipStream_t st) {
+  if (set == HOPS_ALL && sel.n == g.nblk && a == in && b == nullptr && c == nullptr && out != in) {
+    // the full operator: one lean launch per right-hand side
+    const int wpb = (g.Vb + HOP_THREADS - 1) / HOP_THREADS;
+    for (int r = 0; r < B.n; ++r)
+      hipLaunchKernelGGL((k_dslash<S, NL>), dim3(sel.n * wpb), dim3(HOP_THREADS), 0, st, g, sel.list, wpb,
+                         B.active ? B.active + r : nullptr, gamma, out + r * B.stride, in + r * B.stride, U, C);
+    return;
+  }
   const int wpb = (g.Vb + HOP_THREADS - 1) / HOP_THREADS;
   dim3 grid(sel.n * wpb), block(HOP_THREADS);
   const int skip = (set == HOPS_FACE && out == a && b == nullptr && c == nullptr) ? 1 : 0;

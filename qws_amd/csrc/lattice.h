@@ -197,6 +197,7 @@ class Lattice {
   double kappa_ = 0;
   hipStream_t st_ = nullptr;
   int* d_blocks_[2] = {nullptr, nullptr};
+  int* d_order_ = nullptr;  // every block, Morton order
   int nblk_colour_[2] = {0, 0};
   // fields
   double *Ud_ = nullptr, *Cd_ = nullptr;
