@@ -6,10 +6,12 @@ seed 2), kappa = 0.1365, c_sw = 1.0 clover term built from the links, antiperiod
 t, Gaussian FP64 source (seed 2); FP64 outer BiCGStab with FP32 SAP-preconditioned
 inner BiCGStab on 4^4 blocks; FP32 links stored as 12 reals.
 
-Prints one JSON line.  value = solves/s; roofline = the FP32 dslash kernel (full
-D = 1 - kappa T^-1 H), timed with HIP events on the library's stream, against the
-MI355X HBM peak; cpu_baseline = the QWS reference library (oracle/_ref) timed on
-this host on a bounded sample of the same solve.
+Prints one JSON line.  value = solves/s; roofline = the dominant kernel of the
+solve, the fused FP32 SAP half-sweep (about three quarters of the device time),
+timed with HIP events on the library's stream, its algorithmic FP32 flops against
+the MI355X FP32 peak; roofline_d = the FP32 dslash kernel (full D = 1 - kappa
+T^-1 H) against the HBM peak; cpu_baseline = the QWS reference library
+(oracle/_ref) timed on this host on a bounded sample of the same solve.
 
     python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu-baseline]
 """
@@ -27,12 +29,41 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+FP32_PEAK_TFLOPS = 157.3  # MI355X published FP32 (vector, with packed FP32 FMA)
 WORKLOAD = dict(L=[32, 32, 32, 64], B=[4, 4, 4, 4], eps=0.3, seed=2, kappa=0.1365, csw=1.0, tbc=-1, links32=12)
 # Solver settings of the measured workload (chosen from a sweep of nsap, nm and tol_s on MI355X)
 SOLVE = dict(nsap=2, nm=6, tol=1e-11, maxiter=40, tol_s=1e-3, maxiter_s=80, true_tol=1e-10, max_restarts=4)
 # algorithmic bytes per site of one FP32 D application with 12-real links:
 # 4 links x 12 x 4 B + clover 72 x 4 B + input 24 x 4 B + output 24 x 4 B
 BYTES_PER_SITE_F32_12 = 4 * 12 * 4 + 72 * 4 + 24 * 4 + 24 * 4
+
+
+def sap_flops_per_site(B, nsap, nm):
+    """Algorithmic flops per site of one fused SAP launch (the sites of one colour),
+    averaged over the 2*nsap+2 launches of one preconditioner application.
+
+    One hop along x, y or z: spin projection 12 + SU(3) times two colour vectors 132
+    + accumulation of the 4-spinor 24 = 168; along t (diagonal gamma_4, so the
+    projection is a selection and only two spin components are accumulated)
+    132 + 12 = 144.  Hops inside a block per site: 2 (B_mu - 1) / B_mu along mu;
+    across a block face: 2 / B_mu.  Clover inverse: two packed Hermitian 6x6
+    products 2 x 252 + spin rotation in and out 48 = 552.  Jacobi step x = s +
+    kappa A x: in-block hops + clover + 48.  Launch sequence (qws.cc:809-925): the
+    first sweep's first colour has no face prologue; every other launch starts with
+    the face update (face hops + clover + 48); the epilogues are the residual
+    update (in-block hops + clover + 96) for the 2*nsap sweep launches, then D_in
+    (in-block hops + clover + 48) and D_in plus the face term (+ 96).
+    """
+    hop = [168.0, 168.0, 168.0, 144.0]
+    h_in = sum(2.0 * (b - 1) / b * hop[mu] for mu, b in enumerate(B))
+    h_face = sum(2.0 / b * hop[mu] for mu, b in enumerate(B))
+    clv = 552.0
+    jac = nm * (h_in + clv + 48.0)
+    pro = h_face + clv + 48.0
+    e_s, e_o, e_of = h_in + clv + 96.0, h_in + clv + 48.0, h_in + clv + 96.0
+    n = 2 * nsap + 2
+    total = n * jac + 2 * nsap * e_s + e_o + e_of + (n - 1) * pro
+    return total / n
 
 
 def progress(msg):
@@ -80,12 +111,12 @@ PRECISION_WORKLOAD = dict(L=[48, 48, 48, 96], B=[4, 4, 4, 4], eps=0.25, seed=3, 
                           links32=12)
 
 
-def precision_variant(Q):
+def precision_variant(Q, kappa=None):
     """48^3x96: FP16-stored SAP (FP32 arithmetic), FP32 inner, FP64 outer to 1e-10; its
     error against a reference solution from the FP64 outer solver driven to a 1e-13
     true residual (the preconditioner's precision bounds only the convergence rate,
     not the accuracy of the FP64-refined solution)."""
-    w = PRECISION_WORKLOAD
+    w = dict(PRECISION_WORKLOAD, kappa=kappa if kappa is not None else PRECISION_WORKLOAD["kappa"])
     lat, b = setup(Q, w)
     progress("48^3x96 fields ready")
     x = lat.spinor(Q.F64)
@@ -107,7 +138,8 @@ def precision_variant(Q):
     err = float(np.linalg.norm(a - r) / np.linalg.norm(r))
     del a, r
     lat.close()
-    return {"workload": "48^3x96, eps=0.25 seed 3, kappa=0.137, csw=1.1, FP16-stored SAP / FP32 inner / FP64 outer",
+    return {"workload": "48^3x96, eps=0.25 seed 3, kappa=%g, csw=1.1, FP16-stored SAP / FP32 inner / FP64 outer"
+                        % w["kappa"],
             "solves_per_s": 1.0 / dt, "seconds": dt, "true_residual": st["true_residual"],
             "error_vs_fp64_reference": err, "reference_true_residual": st_r["true_residual"],
             "iterations": {"outer": st["outer_iters"], "inner": st["inner_iters"], "restarts": st["restarts"]},
@@ -188,12 +220,16 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--dslash-only", type=int, default=0, help="only time N dslash launches (for counter runs)")
+    ap.add_argument("--sap-only", type=int, default=0,
+                    help="only time N preconditioner applications of fused SAP launches (for counter runs)")
     ap.add_argument("--sweep", action="store_true", help="print solve statistics for a grid of solver settings")
     ap.add_argument("--serve", action="store_true",
                     help="serving sweep: Poisson rates 5..200 req/s over four resident fields (p50/p99 latency)")
     ap.add_argument("--serve-requests", type=int, default=24)
+    ap.add_argument("--only-plain", action="store_true", help="only the plain solves and the D roofline (profiling)")
     ap.add_argument("--precision", action="store_true",
                     help="the 48^3x96 FP16-SAP precision variant and its error against an FP64-refined solution")
+    ap.add_argument("--precision-kappa", type=float, default=None, help="kappa of the precision variant")
     ap.add_argument("--copy-calib", type=int, default=0,
                     help="only run N device-to-device copies of 1 GiB (HBM counter calibration)")
     a = ap.parse_args()
@@ -217,7 +253,7 @@ def main():
     import qws_amd as Q
 
     if a.precision:
-        print(json.dumps(precision_variant(Q)), flush=True)
+        print(json.dumps(precision_variant(Q, a.precision_kappa)), flush=True)
         return
     if a.serve:
         srv = make_server(Q)
@@ -233,6 +269,11 @@ def main():
     if a.dslash_only:
         ms = lat.time_dslash(Q.F32, a.dslash_only)
         print(json.dumps({"dslash_ms": ms, "GBs": V * BYTES_PER_SITE_F32_12 / (ms * 1e-3) / 1e9}))
+        return
+    if a.sap_only:
+        ms = lat.time_sap(Q.F32, a.sap_only, SOLVE["nsap"], SOLVE["nm"])
+        fl = sap_flops_per_site(WORKLOAD["B"], SOLVE["nsap"], SOLVE["nm"]) * V / 2
+        print(json.dumps({"sap_ms_per_launch": ms, "TFLOPs": fl / (ms * 1e-3) / 1e12}))
         return
     if a.sweep:
         t_start = time.perf_counter()
@@ -251,14 +292,16 @@ def main():
     dt, stats, x = run_solves(Q, lat, b, a.steps, SOLVE)
     ms_per = dt / a.steps * 1e3
     worst = max(s["true_residual"] for s in stats)
-    # dslash roofline: HIP events on the library stream, same fields
-    reps = 50
-    ms_d = lat.time_dslash(Q.F32, reps)
+    # rooflines: HIP events on the library stream, same fields
+    ms_d = lat.time_dslash(Q.F32, 50)
     achieved = V * BYTES_PER_SITE_F32_12 / (ms_d * 1e-3) / 1e9
-    traffic = None
-    tfile = os.path.join(ROOT, "profiles", "dslash_f32_32c64_traffic.json")
-    if os.path.exists(tfile):
-        traffic = json.load(open(tfile)).get("bytes_per_launch")
+    ms_s = lat.time_sap(Q.F32, 10, SOLVE["nsap"], SOLVE["nm"])
+    flops_s = sap_flops_per_site(WORKLOAD["B"], SOLVE["nsap"], SOLVE["nm"]) * V / 2
+    achieved_s = flops_s / (ms_s * 1e-3) / 1e12
+
+    def traffic(name):
+        tfile = os.path.join(ROOT, "profiles", name)
+        return json.load(open(tfile)).get("bytes_per_launch") if os.path.exists(tfile) else None
     line = {
         "metric": "solves/s at 1e-10 residual (batched/served vs plain); dslash GB/s; p99 ms",
         "value": 1e3 / ms_per, "unit": "solves/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
@@ -271,13 +314,23 @@ def main():
         "iterations": {"outer": stats[-1]["outer_iters"], "inner": stats[-1]["inner_iters"],
                        "restarts": stats[-1]["restarts"]},
         "dslash_GBs": achieved,
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "k_hop<float,12,HOPS_ALL> (FP32 D, 672 B/site algorithmic)",
-                     "ms_per_launch": ms_d},
+        "roofline": {"bound": "valu", "achieved": achieved_s, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved_s / FP32_PEAK_TFLOPS, "traffic": traffic("sap_f32_32c64_traffic.json"),
+                     "kernel": "k_sap_block<float,12> (fused SAP half-sweep, FP32, %.0f flop/site algorithmic, "
+                               "V/2 sites per launch)" % (flops_s / (V / 2)),
+                     "ms_per_launch": ms_s,
+                     "note": "FP32 vector arithmetic (3x3 complex link products: no MFMA shape); the peak "
+                             "counts packed FP32 FMA, which this build does not emit (unpacked peak 78.6)"},
+        "roofline_d": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                       "frac": achieved / HBM_PEAK_GBS, "traffic": traffic("dslash_f32_32c64_traffic.json"),
+                       "kernel": "k_dslash<float,12> (FP32 D, 672 B/site algorithmic)",
+                       "ms_per_launch": ms_d},
         "cpu_baseline": None,
     }
     progress("plain solves and dslash done")
+    if a.only_plain:
+        print(json.dumps(line), flush=True)
+        return
     # batched: the 12 point sources on the same field, one batch
     dt_b, st_b = batched_point_sources(Q, lat, SOLVE)
     line["batched"] = {"workload": "12 spin-colour point sources at the origin, one batched solve, 32^3x64",

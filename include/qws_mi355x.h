@@ -321,6 +321,11 @@ int qwsg_synchronize(qwsg_lattice* lat);
 /* average device time (ms) of `reps` full FP32/FP16/FP64 dslash launches on the
  * lattice's stream, measured with HIP events */
 int qwsg_time_dslash(qwsg_lattice* lat, int prec, int reps, double* ms_per_call);
+/* average device time (ms) of one fused SAP half-sweep launch (FP32 or FP16
+ * storage), from `reps` times the 2*nsap+2 launches of one preconditioner
+ * application on a Gaussian source, measured with HIP events on the lattice's
+ * stream.  Measurement only; no counterpart in the reference. */
+int qwsg_time_sap(qwsg_lattice* lat, int prec, int reps, int nsap, int nm, double* ms_per_launch);
 
 #ifdef __cplusplus
 }

@@ -85,6 +85,8 @@ def lib():
         L.qwsg_solve_batch.argtypes = [vp, ctypes.c_int, vp, vp, ctypes.POINTER(SolveParams), ctypes.POINTER(SolveStats)]
         L.qwsg_synchronize.argtypes = [vp]
         L.qwsg_time_dslash.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
+        L.qwsg_time_sap.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                    ctypes.POINTER(ctypes.c_double)]
         L.qwsg_device_count.argtypes = [ip]
         L.qwsg_generate_gauge.argtypes = [vp, ctypes.c_double, ctypes.c_ulonglong]
         L.qwsg_compute_clover.argtypes = [vp, ctypes.c_double]
@@ -408,6 +410,12 @@ class Lattice:
     def time_dslash(self, prec, reps):
         ms = ctypes.c_double()
         check(lib().qwsg_time_dslash(self.h, prec, reps, ctypes.byref(ms)))
+        return ms.value
+
+    def time_sap(self, prec, reps, nsap, nm):
+        """Average device ms of one fused SAP launch (FP32 or FP16 storage)."""
+        ms = ctypes.c_double()
+        check(lib().qwsg_time_sap(self.h, prec, reps, nsap, nm, ctypes.byref(ms)))
         return ms.value
 
 

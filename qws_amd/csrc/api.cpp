@@ -310,6 +310,9 @@ int qwsg_synchronize(qwsg_lattice* h) {
 int qwsg_time_dslash(qwsg_lattice* h, int prec, int reps, double* ms) {
   return guarded([&] { *ms = h->lat->time_dslash(prec, reps); });
 }
+int qwsg_time_sap(qwsg_lattice* h, int prec, int reps, int nsap, int nm, double* ms) {
+  return guarded([&] { *ms = h->lat->time_sap(prec, reps, nsap, nm); });
+}
 
 // ============================================================================ QWS drop-in
 // Host mirrors of QWS's globals (qws.cc:74-124).

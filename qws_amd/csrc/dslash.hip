@@ -70,10 +70,11 @@ k_hop(Geo g, const int* __restrict__ list, int wg_per_blk, int nrhs, long long s
 // Full operator out = in + gamma T^-1 H in for one right-hand side over every
 // site: the kernel for D itself (ddd_d_, ddd_s_ and the outer solver's D).  Written
 // without the generality of k_hop (RHS loop, a/b/c terms, block lists) it keeps the
-// live state of a thread small (about 80 VGPRs in FP32), so more waves hide the
-// neighbour-gather latency.
+// live state of a thread small (about 90 VGPRs in FP32).  Residency is capped at 3
+// waves per SIMD: with 6 waves the neighbour gathers of more sites are in flight at
+// once and evict each other from L2 (FP32 32^3x64: 1.3x more HBM fetch, 2% slower).
 template <typename S, int NL>
-__global__ void __launch_bounds__(HOP_THREADS)
+__global__ void __launch_bounds__(HOP_THREADS) __attribute__((amdgpu_waves_per_eu(1, 3)))
 k_dslash(Geo g, const int* __restrict__ order, int wg_per_blk, const int* __restrict__ active,
          typename Store<S>::R gamma, S* __restrict__ out, const S* __restrict__ in, const S* __restrict__ U,
          const S* __restrict__ C) {

@@ -1038,4 +1038,53 @@ double Lattice::time_dslash(int prec, int reps) {
   return ms / reps;
 }
 
+template <typename S>
+double Lattice::time_sap_t(int reps, int nsap, int nm) {
+  if (!fused_ok<S>()) throw std::invalid_argument("time_sap: the fused SAP kernel does not apply to this lattice");
+  const int pr = sizeof(S) == 4 ? P32 : P16;
+  S* in = (S*)work("time_in", spinor_bytes(pr));
+  S* s = (S*)work("sap_s", spinor_bytes(pr));
+  S* q = (S*)work("sap_q", spinor_bytes(pr));
+  S* out = (S*)work("time_out", spinor_bytes(pr));
+  if constexpr (std::is_same<S, float>::value) {
+    gaussian_spinor(P32, in, 11);
+  } else {
+    float* tmp = (float*)work("time_tmp", spinor_bytes(P32));
+    gaussian_spinor(P32, tmp, 11);
+    launch_convert<__half, float>(g_.V, in, tmp, st_);
+  }
+  const Batch B = single();
+  const FieldSet<S> f = fields<S>();
+  auto apply = [&] {
+    sap_sweeps_fused<S>(B, in, s, q, nsap, nm);
+    launch_sap_block<S>(g_, sel(0), B, f.nl, (float)kappa_, nm, SAP_EPI_OUT | SAP_FACE_PRO, s, in, q, q, out, f.U, f.C,
+                        st_);
+    launch_sap_block<S>(g_, sel(1), B, f.nl, (float)kappa_, nm, SAP_EPI_OUT_FACE | SAP_FACE_PRO, s, in, q, q, out, f.U,
+                        f.C, st_);
+  };
+  apply();  // warm up
+  hipEvent_t e0, e1;
+  QWS_HIP(hipEventCreate(&e0));
+  QWS_HIP(hipEventCreate(&e1));
+  QWS_HIP(hipEventRecord(e0, st_));
+  for (int i = 0; i < reps; ++i) apply();
+  QWS_HIP(hipEventRecord(e1, st_));
+  QWS_HIP(hipEventSynchronize(e1));
+  float ms = 0;
+  QWS_HIP(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return ms / (reps * (2.0 * nsap + 2));
+}
+
+double Lattice::time_sap(int prec, int reps, int nsap, int nm) {
+  if (reps < 1 || nsap < 1 || nm < 1) throw std::invalid_argument("time_sap: reps, nsap and nm must be positive");
+  if (prec == P32) return time_sap_t<float>(reps, nsap, nm);
+  if (prec == P16) {
+    ensure_half_fields();
+    return time_sap_t<__half>(reps, nsap, nm);
+  }
+  throw std::invalid_argument("time_sap: FP32 or FP16 storage");
+}
+
 }  // namespace qws

@@ -140,6 +140,9 @@ class Lattice {
   double norm2(int prec, const void* a);
   double true_residual(const double* x, const double* b);
   double time_dslash(int prec, int reps);
+  // average device time (ms) of one fused SAP launch: reps times the 2*nsap + 2
+  // launches of one preconditioner application (prec_ddd_s_), single RHS
+  double time_sap(int prec, int reps, int nsap, int nm);
   void sync() { QWS_HIP(hipStreamSynchronize(st_)); }
 
   // ---- even/odd family (deo_*, mtilde) on x-compressed parity fields of V/2 sites
@@ -182,6 +185,7 @@ class Lattice {
   template <typename S> void jinv_t(const Batch& B, int colour, S* x, const S* b, S* tmp, int nm);
   template <typename S> void sap_sweeps(const Batch& B, const S* in, S* s, S* q, S* tmp, int nsap, int nm);
   template <typename S> bool fused_ok() const;
+  template <typename S> double time_sap_t(int reps, int nsap, int nm);
   template <typename S> void sap_sweeps_fused(const Batch& B, const S* in, S* s, S* q, int nsap, int nm);
   template <typename S> void prec_ddd_t(const Batch& B, S* out, const S* in, int nsap, int nm);
   template <typename S> void prec_s_t(const Batch& B, S* out, const S* in, int nsap, int nm);

@@ -131,6 +131,24 @@ template <typename R> __device__ __forceinline__ Cx<R> cimul(Cx<R> a) { return {
 template <typename R> __device__ __forceinline__ Cx<R> cmimul(Cx<R> a) { return {a.im, -a.re}; }  // -i*a
 template <typename R> __device__ __forceinline__ Cx<R> cscale(R s, Cx<R> a) { return {s * a.re, s * a.im}; }
 
+// Complex multiply-accumulate as chains of fused multiply-adds (4 FMAs; written as
+// c + a*b through cmul and cadd it costs 2 multiplies, 2 FMAs and 2 adds, since the
+// compiler may not reassociate the sum).
+__device__ __forceinline__ float fmar(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ __forceinline__ double fmar(double a, double b, double c) { return __builtin_fma(a, b, c); }
+// c + a b
+template <typename R> __device__ __forceinline__ Cx<R> cfma(Cx<R> a, Cx<R> b, Cx<R> c) {
+  return {fmar(-a.im, b.im, fmar(a.re, b.re, c.re)), fmar(a.im, b.re, fmar(a.re, b.im, c.im))};
+}
+// c + conj(a) b
+template <typename R> __device__ __forceinline__ Cx<R> cfmac(Cx<R> a, Cx<R> b, Cx<R> c) {
+  return {fmar(a.im, b.im, fmar(a.re, b.re, c.re)), fmar(-a.im, b.re, fmar(a.re, b.im, c.im))};
+}
+// c - a b
+template <typename R> __device__ __forceinline__ Cx<R> cfnma(Cx<R> a, Cx<R> b, Cx<R> c) {
+  return {fmar(a.im, b.im, fmar(-a.re, b.re, c.re)), fmar(-a.im, b.re, fmar(-a.re, b.im, c.im))};
+}
+
 // ------------------------------------------------------------------ storage types
 // Storage precision S, arithmetic precision R.  (float,float), (double,double) and
 // (__half,float) are used.
@@ -205,6 +223,15 @@ template <> struct SpinorIO<__half> {
   }
 };
 
+// ---- streaming loads for the SAP block kernel.  Its links and clover are read once
+// per block and half-sweep and never re-read from L2 within the launch; loading them
+// nontemporally leaves L2 to the spinors of neighbouring blocks.  (The full operator
+// measured slower with them and keeps ordinary loads.)
+typedef float f2v __attribute__((ext_vector_type(2)));
+typedef float f4v __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ f2v ld_nt(const f2v* p) { return __builtin_nontemporal_load(p); }
+__device__ __forceinline__ f4v ld_nt(const f4v* p) { return __builtin_nontemporal_load(p); }
+
 // ---- links
 template <typename S> __device__ __forceinline__ Cx<typename Store<S>::R> load_cx(const S* base, long long i);
 template <> __device__ __forceinline__ Cx<float> load_cx<float>(const float* base, long long i) {
@@ -219,6 +246,20 @@ template <> __device__ __forceinline__ Cx<float> load_cx<__half>(const __half* b
   float2 q = __half22float2(reinterpret_cast<const __half2*>(base)[i]);
   return {q.x, q.y};
 }
+template <typename S> __device__ __forceinline__ Cx<float> load_cx_stream(const S* base, long long i) {
+  return load_cx<S>(base, i);
+}
+template <> __device__ __forceinline__ Cx<float> load_cx_stream<float>(const float* base, long long i) {
+  const f2v q = ld_nt(reinterpret_cast<const f2v*>(base) + i);
+  return {q.x, q.y};
+}
+
+// row 2 of a 12-real link = conj(row0 x row1) (wilson_d.h:160-171)
+template <typename R> __device__ __forceinline__ void link_row2(Cx<R> u[9]) {
+  u[6] = cconj(cfnma(u[2], u[4], cmul(u[1], u[5])));
+  u[7] = cconj(cfnma(u[0], u[5], cmul(u[2], u[3])));
+  u[8] = cconj(cfnma(u[1], u[3], cmul(u[0], u[4])));
+}
 
 // u[i*3+j] = QWS c[i][j] of the link at site s, direction mu.
 template <typename S, int NL>
@@ -228,15 +269,7 @@ __device__ __forceinline__ void load_link(const S* U, int mu, long long s, long 
   const long long base = (long long)mu * NC * V + s;
 #pragma unroll
   for (int k = 0; k < NC; ++k) u[k] = load_cx<S>(U, base + k * V);
-  if (NL == 12) {
-    // row 2 = conj(row0 x row1) (wilson_d.h:160-171)
-    Cx<R> a = cmul(u[1], u[5]), b = cmul(u[2], u[4]);
-    u[6] = cconj(csub(a, b));
-    a = cmul(u[2], u[3]); b = cmul(u[0], u[5]);
-    u[7] = cconj(csub(a, b));
-    a = cmul(u[0], u[4]); b = cmul(u[1], u[3]);
-    u[8] = cconj(csub(a, b));
-  }
+  if (NL == 12) link_row2(u);
 }
 
 // ------------------------------------------------------------------ site algebra
@@ -272,12 +305,10 @@ __device__ __forceinline__ void link_mult(const Cx<R> u[9], const Cx<R> a[6], Cx
   for (int j = 0; j < 3; ++j)
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
-      Cx<R> acc = {0, 0};
+      Cx<R> acc = DIR > 0 ? cmul(u[j], a[h]) : cmulc(u[j * 3], a[h]);
 #pragma unroll
-      for (int i = 0; i < 3; ++i) {
-        Cx<R> t = DIR > 0 ? cmul(u[i * 3 + j], a[i * 2 + h]) : cmulc(u[j * 3 + i], a[i * 2 + h]);
-        acc = cadd(acc, t);
-      }
+      for (int i = 1; i < 3; ++i)
+        acc = DIR > 0 ? cfma(u[i * 3 + j], a[i * 2 + h], acc) : cfmac(u[j * 3 + i], a[i * 2 + h], acc);
       b[j * 2 + h] = acc;
     }
 }
@@ -314,12 +345,14 @@ template <typename S> struct CloverIO;
 
 template <> struct CloverIO<float> {
   // reals k of chirality i: float4 index (36i+k)/4
+  template <bool STREAM = false>
   static __device__ __forceinline__ void load_chir(const float* C, long long s, long long V, int i, float a[36]) {
-    const float4* p = reinterpret_cast<const float4*>(C);
+    const f4v* p = reinterpret_cast<const f4v*>(C);
 #pragma unroll
     for (int f = 0; f < 9; ++f) {
-      float4 q = p[(long long)(9 * i + f) * V + s];
-      a[4 * f] = q.x; a[4 * f + 1] = q.y; a[4 * f + 2] = q.z; a[4 * f + 3] = q.w;
+      const f4v* q = p + (long long)(9 * i + f) * V + s;
+      const f4v v = STREAM ? ld_nt(q) : *q;
+      a[4 * f] = v.x; a[4 * f + 1] = v.y; a[4 * f + 2] = v.z; a[4 * f + 3] = v.w;
     }
   }
 };
@@ -355,8 +388,8 @@ __device__ __forceinline__ void herm6_mult(const R a[36], const Cx<R> x[6], Cx<R
 #pragma unroll
     for (int c = r + 1; c < 6; ++c) {
       Cx<R> m = {a[p], a[p + 1]};
-      y[r] = cadd(y[r], cmul(m, x[c]));
-      y[c] = cadd(y[c], cmulc(m, x[r]));
+      y[r] = cfma(m, x[c], y[r]);
+      y[c] = cfmac(m, x[r], y[c]);
       p += 2;
     }
 }

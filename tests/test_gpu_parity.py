@@ -154,6 +154,25 @@ def test_fused_sap_matches_unfused_and_oracle(L, B, su3, nsap, nm):
     assert rel(res[True][0], res[False][0]) < 5e-6
 
 
+def test_time_sap_runs_the_fused_kernel():
+    """The SAP timing entry point runs the fused launches of one preconditioner
+    application (FP32 and FP16 storage), leaves the operator's results unchanged and
+    reports a positive device time; blocks the fused kernel cannot take are refused."""
+    p = Problem([16, 8, 8, 16], [4, 4, 4, 4], kappa=0.125, seed=8, su3=True)
+    bs = p.b.astype(np.float32)
+    inp = p.lat.spinor(Q.F32, bs)
+    o1, o2 = p.lat.spinor(Q.F32), p.lat.spinor(Q.F32)
+    p.lat.prec_ddd(o1, inp, 2, 3)
+    for prec in (Q.F32, Q.F16):
+        ms = p.lat.time_sap(prec, 3, 2, 3)
+        assert 0 < ms < 100, (prec, ms)
+    p.lat.prec_ddd(o2, inp, 2, 3)
+    assert np.array_equal(o1.download(), o2.download())
+    q = Problem([8, 4, 4, 8], [2, 2, 2, 2])
+    with pytest.raises(Q.QwsError):
+        q.lat.time_sap(Q.F32, 1, 2, 2)
+
+
 def test_inner_bicgstab(prob):
     p, lat = prob, prob.lat
     bs = (p.b / np.sqrt(O.norm2(p.b))).astype(np.float32)
