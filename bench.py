@@ -293,6 +293,7 @@ def main():
     ms_per = dt / a.steps * 1e3
     worst = max(s["true_residual"] for s in stats)
     # rooflines: HIP events on the library stream, same fields
+    d_kernel = lat.autotune_dslash(20)
     ms_d = lat.time_dslash(Q.F32, 50)
     achieved = V * BYTES_PER_SITE_F32_12 / (ms_d * 1e-3) / 1e9
     ms_s = lat.time_sap(Q.F32, 10, SOLVE["nsap"], SOLVE["nm"])
@@ -323,7 +324,8 @@ def main():
                              "counts packed FP32 FMA, which this build does not emit (unpacked peak 78.6)"},
         "roofline_d": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                        "frac": achieved / HBM_PEAK_GBS, "traffic": traffic("dslash_f32_32c64_traffic.json"),
-                       "kernel": "k_dslash<float,12> (FP32 D, 672 B/site algorithmic)",
+                       "kernel": ("k_dslash<float,12>" if d_kernel == 0 else "k_dslash_lds<12,%d>" % d_kernel)
+                                 + " (FP32 D, 672 B/site algorithmic; kernel chosen by autotune_dslash)",
                        "ms_per_launch": ms_d},
         "cpu_baseline": None,
     }

@@ -321,6 +321,13 @@ int qwsg_synchronize(qwsg_lattice* lat);
 /* average device time (ms) of `reps` full FP32/FP16/FP64 dslash launches on the
  * lattice's stream, measured with HIP events */
 int qwsg_time_dslash(qwsg_lattice* lat, int prec, int reps, double* ms_per_call);
+/* Kernel of the FP32 full operator on 256-site blocks: 0 = neighbour gathers from
+ * global memory, 3 or 4 = the block's spinor staged in LDS (compiled for 3 or 4
+ * waves per SIMD).  The variants give the same results.  qwsg_autotune_dslash
+ * times each over `reps` launches on this lattice, keeps the fastest and returns
+ * it in *chosen.  Tuning only; no counterpart in the reference. */
+int qwsg_set_dslash_kernel(qwsg_lattice* lat, int kernel);
+int qwsg_autotune_dslash(qwsg_lattice* lat, int reps, int* chosen);
 /* average device time (ms) of one fused SAP half-sweep launch (FP32 or FP16
  * storage), from `reps` times the 2*nsap+2 launches of one preconditioner
  * application on a Gaussian source, measured with HIP events on the lattice's

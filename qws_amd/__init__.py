@@ -85,6 +85,8 @@ def lib():
         L.qwsg_solve_batch.argtypes = [vp, ctypes.c_int, vp, vp, ctypes.POINTER(SolveParams), ctypes.POINTER(SolveStats)]
         L.qwsg_synchronize.argtypes = [vp]
         L.qwsg_time_dslash.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
+        L.qwsg_set_dslash_kernel.argtypes = [vp, ctypes.c_int]
+        L.qwsg_autotune_dslash.argtypes = [vp, ctypes.c_int, ip]
         L.qwsg_time_sap.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                     ctypes.POINTER(ctypes.c_double)]
         L.qwsg_device_count.argtypes = [ip]
@@ -411,6 +413,16 @@ class Lattice:
         ms = ctypes.c_double()
         check(lib().qwsg_time_dslash(self.h, prec, reps, ctypes.byref(ms)))
         return ms.value
+
+    def set_dslash_kernel(self, k):
+        """FP32 full-operator kernel on 256-site blocks: 0 global gathers, 3/4 LDS-staged."""
+        check(lib().qwsg_set_dslash_kernel(self.h, int(k)))
+
+    def autotune_dslash(self, reps=20):
+        """Time the full-operator kernels on this lattice, keep and return the fastest."""
+        k = ctypes.c_int()
+        check(lib().qwsg_autotune_dslash(self.h, int(reps), ctypes.byref(k)))
+        return k.value
 
     def time_sap(self, prec, reps, nsap, nm):
         """Average device ms of one fused SAP launch (FP32 or FP16 storage)."""

@@ -310,6 +310,15 @@ int qwsg_synchronize(qwsg_lattice* h) {
 int qwsg_time_dslash(qwsg_lattice* h, int prec, int reps, double* ms) {
   return guarded([&] { *ms = h->lat->time_dslash(prec, reps); });
 }
+int qwsg_set_dslash_kernel(qwsg_lattice* h, int k) {
+  return guarded([&] { h->lat->set_dslash_kernel(k); });
+}
+int qwsg_autotune_dslash(qwsg_lattice* h, int reps, int* chosen) {
+  return guarded([&] {
+    const int k = h->lat->autotune_dslash(reps);
+    if (chosen) *chosen = k;
+  });
+}
 int qwsg_time_sap(qwsg_lattice* h, int prec, int reps, int nsap, int nm, double* ms) {
   return guarded([&] { *ms = h->lat->time_sap(prec, reps, nsap, nm); });
 }

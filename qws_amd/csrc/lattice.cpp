@@ -60,6 +60,11 @@ Lattice::Lattice(const int L[4], const int B[4], int tbc, int links32) {
   // workgroup placement: XCD-contiguous block ranges unless QWS_MI355X_XCD_REMAP=0
   const char* xr = std::getenv("QWS_MI355X_XCD_REMAP");
   g_.xcd_remap = (xr && xr[0] == '0') ? 0 : 1;
+  // full-operator kernel (FP32, 256-site blocks): LDS-staged, compiled for 4 waves per
+  // SIMD, unless QWS_MI355X_DSLASH names another (0, 3, 4); autotune_dslash() times them
+  const char* dk = std::getenv("QWS_MI355X_DSLASH");
+  g_.dslash_kernel = dk ? std::atoi(dk) : 4;
+  if (g_.dslash_kernel != 0 && g_.dslash_kernel != 3) g_.dslash_kernel = 4;
   nl32_ = links32;
   QWS_HIP(hipStreamCreate(&st_));
   // Launch order of the blocks: index order.  (A Morton order of the block
@@ -1075,6 +1080,27 @@ double Lattice::time_sap_t(int reps, int nsap, int nm) {
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   return ms / (reps * (2.0 * nsap + 2));
+}
+
+void Lattice::set_dslash_kernel(int k) {
+  if (k != 0 && k != 3 && k != 4) throw std::invalid_argument("dslash kernel: 0 (global gathers), 3 or 4 (LDS-staged)");
+  g_.dslash_kernel = k;
+}
+
+int Lattice::autotune_dslash(int reps) {
+  if (reps < 1) throw std::invalid_argument("autotune_dslash: reps must be positive");
+  int best = g_.dslash_kernel;
+  double best_ms = 1e30;
+  for (int k : {0, 3, 4}) {
+    g_.dslash_kernel = k;
+    const double ms = time_dslash(P32, reps);
+    if (ms < best_ms) {
+      best_ms = ms;
+      best = k;
+    }
+  }
+  g_.dslash_kernel = best;
+  return best;
 }
 
 double Lattice::time_sap(int prec, int reps, int nsap, int nm) {

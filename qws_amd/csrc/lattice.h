@@ -143,6 +143,12 @@ class Lattice {
   // average device time (ms) of one fused SAP launch: reps times the 2*nsap + 2
   // launches of one preconditioner application (prec_ddd_s_), single RHS
   double time_sap(int prec, int reps, int nsap, int nm);
+  // FP32 full-operator kernel on 256-site blocks (0 global gathers, 3/4 LDS-staged);
+  // the variants give the same results, autotune_dslash keeps the fastest on this
+  // lattice (timed over `reps` launches each) and returns it
+  void set_dslash_kernel(int k);
+  int dslash_kernel() const { return g_.dslash_kernel; }
+  int autotune_dslash(int reps);
   void sync() { QWS_HIP(hipStreamSynchronize(st_)); }
 
   // ---- even/odd family (deo_*, mtilde) on x-compressed parity fields of V/2 sites

@@ -41,6 +41,8 @@ struct Geo {
                   // runtime value it keeps each hop in its own basic block, so the
                   // compiler cannot hoist all eight neighbour loads (256 -> ~146 VGPRs)
   int xcd_remap;  // 1: map workgroups to blocks XCD by XCD (xcd_remap), 0: identity
+  int dslash_kernel;  // FP32 full operator on 256-site blocks: 0 global gathers (k_dslash),
+                      // 3/4: block spinor staged in LDS, compiled for that many waves per SIMD
 };
 
 // MI355X dispatches the workgroups of a launch round-robin over its 8 XCDs, each
