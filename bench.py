@@ -299,6 +299,8 @@ def main():
     ms_s = lat.time_sap(Q.F32, 10, SOLVE["nsap"], SOLVE["nm"])
     flops_s = sap_flops_per_site(WORKLOAD["B"], SOLVE["nsap"], SOLVE["nm"]) * V / 2
     achieved_s = flops_s / (ms_s * 1e-3) / 1e12
+    # practical HBM ceiling: streaming kernels with D's read:write mix (6:1) and a copy
+    stream = {"read6_write1_GBs": Q.stream_bandwidth(256 << 20, 6, 20), "copy_GBs": Q.stream_bandwidth(1 << 30, 1, 10)}
 
     def traffic(name):
         tfile = os.path.join(ROOT, "profiles", name)
@@ -326,7 +328,8 @@ def main():
                        "frac": achieved / HBM_PEAK_GBS, "traffic": traffic("dslash_f32_32c64_traffic.json"),
                        "kernel": ("k_dslash<float,12>" if d_kernel == 0 else "k_dslash_lds<12,%d>" % d_kernel)
                                  + " (FP32 D, 672 B/site algorithmic; kernel chosen by autotune_dslash)",
-                       "ms_per_launch": ms_d},
+                       "ms_per_launch": ms_d, "stream_ceiling": stream,
+                       "frac_of_stream": achieved / stream["read6_write1_GBs"]},
         "cpu_baseline": None,
     }
     progress("plain solves and dslash done")

@@ -193,6 +193,12 @@ typedef struct {
 
 const char* qwsg_last_error(void);
 int qwsg_device_count(int* n);
+/* Measured HBM bandwidth (GB/s, reads + writes) of a streaming kernel that sums
+ * nin arrays of `bytes` each into one (nin = 1: a copy), over reps launches; the
+ * practical bandwidth ceiling the operator kernels are compared with.  Allocates
+ * (nin + 1) * bytes of device memory for the call.  Measurement only; no
+ * counterpart in the reference. */
+int qwsg_stream_bandwidth(long long bytes, int nin, int reps, double* gbs);
 
 /* L: lattice extents, B: SAP block extents (must divide L; each of the
  * y/z/t block counts even or 1, x block count even), tbc: +1 periodic / -1

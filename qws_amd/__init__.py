@@ -90,6 +90,8 @@ def lib():
         L.qwsg_time_sap.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                     ctypes.POINTER(ctypes.c_double)]
         L.qwsg_device_count.argtypes = [ip]
+        L.qwsg_stream_bandwidth.argtypes = [ctypes.c_longlong, ctypes.c_int, ctypes.c_int,
+                                            ctypes.POINTER(ctypes.c_double)]
         L.qwsg_generate_gauge.argtypes = [vp, ctypes.c_double, ctypes.c_ulonglong]
         L.qwsg_compute_clover.argtypes = [vp, ctypes.c_double]
         L.qwsg_gauge_download.argtypes = [vp, vp]
@@ -429,6 +431,13 @@ class Lattice:
         ms = ctypes.c_double()
         check(lib().qwsg_time_sap(self.h, prec, reps, nsap, nm, ctypes.byref(ms)))
         return ms.value
+
+
+def stream_bandwidth(nbytes, nin=1, reps=20):
+    """Measured GB/s of a streaming kernel summing nin arrays of nbytes into one."""
+    g = ctypes.c_double()
+    check(lib().qwsg_stream_bandwidth(int(nbytes), int(nin), int(reps), ctypes.byref(g)))
+    return g.value
 
 
 def device_count():

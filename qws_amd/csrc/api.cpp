@@ -46,6 +46,9 @@ const char* qwsg_last_error(void) { return g_err.c_str(); }
 int qwsg_device_count(int* n) {
   return guarded([&] { QWS_HIP(hipGetDeviceCount(n)); });
 }
+int qwsg_stream_bandwidth(long long bytes, int nin, int reps, double* gbs) {
+  return guarded([&] { *gbs = qws::measure_stream(bytes, nin, reps); });
+}
 
 int qwsg_create(const int* L, const int* B, int tbc, int links32, qwsg_lattice** out) {
   return guarded([&] {

@@ -448,6 +448,46 @@ template <typename R> void launch_sub(long long n, R* out, const R* a, const R* 
   LAUNCH_B((k_sub<R>), 1, n, out, a, b);
 }
 
+// HBM probe: out = the sum of nin input arrays of n4 float4 each (nin reads per write;
+// nin = 1 is a copy), coalesced 128-bit accesses.  Each thread handles U elements a
+// grid apart, with all U x nin loads issued before the sums; grid-stride beyond that.
+template <int U>
+__global__ void __launch_bounds__(256) k_stream(long long n4, int nin, const float4* __restrict__ in,
+                                                float4* __restrict__ out) {
+  const long long step = (long long)gridDim.x * 256;
+  for (long long i0 = blockIdx.x * 256LL + threadIdx.x; i0 < n4; i0 += step * U) {
+    float4 s[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) s[u] = i0 + u * step < n4 ? in[i0 + u * step] : make_float4(0, 0, 0, 0);
+    for (int k = 1; k < nin; ++k) {
+      float4 v[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) v[u] = i0 + u * step < n4 ? in[k * n4 + i0 + u * step] : make_float4(0, 0, 0, 0);
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        s[u].x += v[u].x;
+        s[u].y += v[u].y;
+        s[u].z += v[u].z;
+        s[u].w += v[u].w;
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+      if (i0 + u * step < n4) out[i0 + u * step] = s[u];
+  }
+}
+
+// cfg 0: one element per thread; 1: 4 per thread; 2: grid-stride over 2048 workgroups
+void launch_stream(int cfg, long long n4, int nin, const float4* in, float4* out, hipStream_t st) {
+  if (cfg == 0) {
+    hipLaunchKernelGGL(k_stream<1>, dim3((unsigned)((n4 + 255) / 256)), dim3(256), 0, st, n4, nin, in, out);
+  } else if (cfg == 1) {
+    hipLaunchKernelGGL(k_stream<4>, dim3((unsigned)((n4 + 1023) / 1024)), dim3(256), 0, st, n4, nin, in, out);
+  } else {
+    hipLaunchKernelGGL(k_stream<1>, dim3(2048), dim3(256), 0, st, n4, nin, in, out);
+  }
+}
+
 #define INST(R)                                                                                                       \
   template void launch_bicg_init_inner<R>(const Batch&, long long, R*, R*, R*, R*, const R*, double*, hipStream_t);   \
   template void launch_bicg_init_outer<R>(const Batch&, long long, R*, R*, R*, const R*, const R*, double*,           \

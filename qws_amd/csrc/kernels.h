@@ -93,6 +93,9 @@ void launch_scale_rms(const Batch& B, long long V, long long so, long long si, S
                       int dir, hipStream_t st);
 // *out ^= order-independent digest of nwords 8-byte words (clear *out first)
 void launch_digest(long long nwords, const void* data, unsigned long long* out, hipStream_t st);
+// HBM probe: out = sum of the nin consecutive arrays of n4 float4 in `in` (nin = 1: copy);
+// cfg 0/1/2 = launch shape (1 or 4 elements per thread, or grid-stride)
+void launch_stream(int cfg, long long n4, int nin, const float4* in, float4* out, hipStream_t st);
 // out = a - b (real arrays of length n)
 template <typename R> void launch_sub(long long n, R* out, const R* a, const R* b, hipStream_t st);
 

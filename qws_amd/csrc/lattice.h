@@ -13,6 +13,11 @@
 
 namespace qws {
 
+// GB/s (reads + writes) of a streaming kernel summing nin arrays of `bytes` into one
+// (nin = 1: copy): the fastest of three launch shapes, each averaged over reps
+// launches; allocates and frees its buffers
+double measure_stream(long long bytes, int nin, int reps);
+
 struct HipError : std::runtime_error {
   using std::runtime_error::runtime_error;
 };

@@ -195,6 +195,16 @@ def test_time_sap_runs_the_fused_kernel():
         q.lat.time_sap(Q.F32, 1, 2, 2)
 
 
+def test_stream_bandwidth_probe():
+    """The HBM probe runs (copy and a 6:1 read-mostly sum) and reports a plausible
+    rate; bad arguments are refused."""
+    for nin in (1, 6):
+        g = Q.stream_bandwidth(64 << 20, nin, 3)
+        assert 100 < g < 20000, (nin, g)
+    with pytest.raises(Q.QwsError):
+        Q.stream_bandwidth(1024, 1, 1)
+
+
 def test_inner_bicgstab(prob):
     p, lat = prob, prob.lat
     bs = (p.b / np.sqrt(O.norm2(p.b))).astype(np.float32)
