@@ -142,6 +142,14 @@ void mtdagmt_vm_(scd_t* out, scd_t* in);
 /* bicgstab.cc:65 -- FP64 BiCGStab on mtilde_vm_ (x: initial guess in, solution out;
  * stops at sqrt(|r|^2/|b|^2) < 1e-14 or after maxiter-1 iterations) */
 void bicgstab_vm_(scd_t* x, scd_t* b, int* conviter, int* maxiter);
+/* cg.cc:65 -- CG on mtdagmt_vm_ (x: initial guess in, solution out; stops at
+ * sqrt(|r|^2/|b|^2) < 1e-14 or after maxiter-1 iterations).  *conviter receives the
+ * iterations run (the reference does not set it). */
+void cg_vm_(scd_t* x, scd_t* b, int* conviter, int* maxiter);
+/* mcg.cc:65 -- multi-shift CG: for k < *n, xx[k*vold ...] (vold sites each) solves
+ * (mtdagmt_vm_ + sigma[k]) x = b, from zero, with the stopping rule of cg_vm_ on the
+ * unshifted residual.  1 <= *n <= 64. */
+void mcg_vm_(scd_t* xx, scd_t* b, int* n, double* sigma, int* maxiter);
 /* FP32: deo_in_s.cc:65,191,317; deo_out_s.cc:93,155,229; qws.cc:576-605; clover_s.cc (clv_s_) */
 void deo_in_s_(int* pe, int* po, scs_t* out, scs_t* in);
 void dee_deo_in_s_(int* pe, int* po, scs_t* out, scs_t* in);
@@ -287,6 +295,12 @@ int qwsg_deo(qwsg_lattice* lat, int prec, int pe, int set, int dag, int clv, int
 int qwsg_mtilde(qwsg_lattice* lat, int prec, int pce, int which, void* out, const void* in);
 /* bicgstab.cc:65 -- FP64 BiCGStab on mtilde (parity pce) from the initial guess in x */
 int qwsg_eo_bicgstab(qwsg_lattice* lat, void* x, const void* b, int maxiter, double tol, int pce, int* iters);
+/* cg.cc:65 -- FP64 CG on mtilde^+ mtilde (parity pce) from the initial guess in x */
+int qwsg_eo_cg(qwsg_lattice* lat, void* x, const void* b, int maxiter, double tol, int pce, int* iters);
+/* mcg.cc:65 -- FP64 multi-shift CG: the parity field xs[k] (k < nshift, device
+ * pointers) solves (mtilde^+ mtilde + sigma[k]) x = b from zero */
+int qwsg_eo_mcg(qwsg_lattice* lat, void* const* xs, const void* b, int nshift, const double* sigma, int maxiter,
+                double tol, int pce, int* iters);
 int qwsg_norm2(qwsg_lattice* lat, int prec, const void* a, double* out);
 
 /* No counterpart in the reference (it solves one source per call, serially): a solve

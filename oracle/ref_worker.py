@@ -223,6 +223,37 @@ def run(inp):
                                  _i(nsap), _i(nm))
             res[op] = r.back_d(x)
             res[op + "_iter"] = np.array(conv.value)
+        elif op in ("cg_vm", "mcg_vm"):
+            # even/odd solvers on mtdagmt_vm_: the parity field is the first x-domain of the
+            # driver's AoSoA spinor (as main.cc:200-213 passes it); also runs the oracle's
+            # restatement (oracle/eo_solvers.py) with the reference's own operator
+            from oracle import eo_solvers as ES
+            bd = np.ascontiguousarray(r.d(b).reshape(2 * vold, -1)[:vold])
+            mi = int(inp["maxiter"])
+
+            def apply(v):
+                out = np.zeros_like(bd)
+                lib.mtdagmt_vm_(_ptr(out), _ptr(np.ascontiguousarray(v.reshape(bd.shape))))
+                return out.ravel()
+
+            if op == "cg_vm":
+                x = np.zeros_like(bd)
+                conv = ctypes.c_int(0)
+                lib.cg_vm_(_ptr(x), _ptr(bd), ctypes.byref(conv), _i(mi))
+                res[op] = x
+                xo, it = ES.cg(apply, bd.ravel(), np.zeros(bd.size), mi)
+                res[op + "_oracle"] = xo.reshape(bd.shape)
+                res[op + "_oracle_iter"] = np.array(it)
+            else:
+                sg = np.ascontiguousarray(inp["sigma"], dtype=np.float64)
+                n = len(sg)
+                xx = np.zeros((n,) + bd.shape)
+                lib.mcg_vm_(_ptr(xx), _ptr(bd), _i(n), sg.ctypes.data_as(ctypes.c_void_p), _i(mi))
+                res[op] = xx
+                xs, it = ES.mcg(apply, bd.ravel(), list(sg), mi)
+                res[op + "_oracle"] = np.stack([v.reshape(bd.shape) for v in xs])
+                res[op + "_oracle_iter"] = np.array(it)
+            res[op + "_b"] = bd
         else:
             raise ValueError(op)
     return res
@@ -239,7 +270,7 @@ if __name__ == "__main__":
 
 
 def call(nloc, kappa, U, C, b, ops, x0=None, nsap=4, nm=2, tol=-1.0, maxiter=6, tol_s=-1.0,
-         maxiter_s=50, timeout=600):
+         maxiter_s=50, timeout=600, sigma=()):
     """Run `ops` in a child process and return {op: plain-layout array}."""
     import subprocess
     import tempfile
@@ -249,7 +280,7 @@ def call(nloc, kappa, U, C, b, ops, x0=None, nsap=4, nm=2, tol=-1.0, maxiter=6, 
         np.savez(fin, nloc=np.array(nloc), kappa=np.array(kappa), U=U, C=C, b=b,
                  x0=x0 if x0 is not None else np.zeros_like(b), ops=np.array(ops), nsap=np.array(nsap),
                  nm=np.array(nm), tol=np.array(tol), maxiter=np.array(maxiter), tol_s=np.array(tol_s),
-                 maxiter_s=np.array(maxiter_s))
+                 maxiter_s=np.array(maxiter_s), sigma=np.array(sigma, dtype=np.float64))
         p = subprocess.run([sys.executable, "-m", "oracle.ref_worker", fin, fout], cwd=root,
                            capture_output=True, text=True, timeout=timeout)
         if p.returncode != 0:

@@ -79,6 +79,9 @@ def lib():
         L.qwsg_deo.argtypes = [vp] + [ctypes.c_int] * 6 + [ctypes.c_double, vp, vp, vp]
         L.qwsg_mtilde.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, vp, vp]
         L.qwsg_eo_bicgstab.argtypes = [vp, vp, vp, ctypes.c_int, ctypes.c_double, ctypes.c_int, ip]
+        L.qwsg_eo_cg.argtypes = [vp, vp, vp, ctypes.c_int, ctypes.c_double, ctypes.c_int, ip]
+        L.qwsg_eo_mcg.argtypes = [vp, vp, vp, ctypes.c_int, ctypes.POINTER(ctypes.c_double), ctypes.c_int,
+                                  ctypes.c_double, ctypes.c_int, ip]
         L.qwsg_bicgstab_precdd_h.argtypes = [vp, vp, vp, ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_int, ip]
         L.qwsg_solve.argtypes = [vp, vp, vp, ctypes.POINTER(SolveParams), ctypes.POINTER(SolveStats)]
         L.qwsg_norm2.argtypes = [vp, ctypes.c_int, vp, ctypes.POINTER(ctypes.c_double)]
@@ -378,6 +381,23 @@ class Lattice:
         it = ctypes.c_int()
         check(lib().qwsg_eo_bicgstab(self.h, x.ptr, b.ptr, maxiter, float(tol), pce, ctypes.byref(it)))
         return it.value
+
+    def eo_cg(self, x, b, maxiter=100, tol=1e-14, pce=0):
+        """CG on mtilde^+ mtilde (FP64 parity fields); x holds the initial guess."""
+        it = ctypes.c_int()
+        check(lib().qwsg_eo_cg(self.h, x.ptr, b.ptr, maxiter, float(tol), pce, ctypes.byref(it)))
+        return it.value
+
+    def eo_mcg(self, b, sigma, maxiter=100, tol=1e-14, pce=0):
+        """Multi-shift CG: one FP64 parity field per shift, solving
+        (mtilde^+ mtilde + sigma_k) x_k = b from zero.  Returns (fields, iterations)."""
+        n = len(sigma)
+        xs = [self.eo_spinor(F64) for _ in range(n)]
+        xp = (ctypes.c_void_p * n)(*[x.ptr.value for x in xs])
+        sg = (ctypes.c_double * n)(*[float(v) for v in sigma])
+        it = ctypes.c_int()
+        check(lib().qwsg_eo_mcg(self.h, xp, b.ptr, n, sg, maxiter, float(tol), pce, ctypes.byref(it)))
+        return xs, it.value
 
     def msap_h(self, out, inp, nsap, nm):
         """out = M_SAP inp with FP16-stored SAP iterates and fields (FP32 vectors)."""

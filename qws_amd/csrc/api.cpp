@@ -304,6 +304,21 @@ int qwsg_eo_bicgstab(qwsg_lattice* h, void* x, const void* b, int maxiter, doubl
     if (iters) *iters = it;
   });
 }
+int qwsg_eo_cg(qwsg_lattice* h, void* x, const void* b, int maxiter, double tol, int pce, int* iters) {
+  return guarded([&] {
+    const double k = h->lat->kappa();
+    const int it = h->lat->cg_eo((double*)x, (const double*)b, maxiter, tol, pce, -k * k);
+    if (iters) *iters = it;
+  });
+}
+int qwsg_eo_mcg(qwsg_lattice* h, void* const* xs, const void* b, int nshift, const double* sigma, int maxiter,
+                double tol, int pce, int* iters) {
+  return guarded([&] {
+    const double k = h->lat->kappa();
+    const int it = h->lat->mcg_eo((double* const*)xs, (const double*)b, nshift, sigma, maxiter, tol, pce, -k * k);
+    if (iters) *iters = it;
+  });
+}
 int qwsg_norm2(qwsg_lattice* h, int prec, const void* a, double* out) {
   return guarded([&] { *out = h->lat->norm2(prec, a); });
 }
@@ -356,6 +371,8 @@ struct DropIn {
   int* pco = nullptr;
   void* dev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};  // d0 d1 s0 s1 pre_d pre_s
   void* eo[2][4] = {{nullptr, nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr, nullptr}};  // [P64/P32][in out in0 pre]
+  void* eo_multi = nullptr;  // mcg_vm_'s solutions (one FP64 parity field per shift)
+  size_t eo_multi_bytes = 0;
   std::vector<char> host_tmp[2];
 };
 DropIn g_qws;
@@ -419,6 +436,9 @@ void release_drop_in() {
         hipFree(p);
         p = nullptr;
       }
+  if (g_qws.eo_multi) hipFree(g_qws.eo_multi);
+  g_qws.eo_multi = nullptr;
+  g_qws.eo_multi_bytes = 0;
   delete g_qws.lat;
   g_qws.lat = nullptr;
 }
@@ -451,6 +471,19 @@ void* eobuf(int prec, int i) {
     if (hipMalloc(&p, g_qws.lat->eo_spinor_bytes(prec)) != hipSuccess) throw std::runtime_error("hipMalloc failed");
   }
   return p;
+}
+
+// n consecutive FP64 parity fields (grown on demand, kept until qws_finalize_)
+void* eobuf_n(int prec, int n) {
+  const size_t need = g_qws.lat->eo_spinor_bytes(prec) * (size_t)n;
+  if (g_qws.eo_multi_bytes < need) {
+    if (g_qws.eo_multi) (void)hipFree(g_qws.eo_multi);
+    g_qws.eo_multi = nullptr;
+    g_qws.eo_multi_bytes = 0;
+    if (hipMalloc(&g_qws.eo_multi, need) != hipSuccess) throw std::runtime_error("hipMalloc failed");
+    g_qws.eo_multi_bytes = need;
+  }
+  return g_qws.eo_multi;
 }
 
 // out (parity *pe, one parity field of the host layout) from in (parity 1 - *pe):
@@ -877,6 +910,37 @@ void bicgstab_vm_(scd_t* x, scd_t* b, int* conviter, int* maxiter) {
     const int it = L.bicgstab_eo(xx, bb, *maxiter, 1e-14, *g_qws.pce, -kappa2);
     L.eo_download_spinor(qws::P64, xx, x, 1, QWS_VLEND, 1);
     if (conviter) *conviter = it;
+  });
+}
+// cg.cc:65 -- CG on mtdagmt_vm_ from the initial guess in x, stopping at
+// sqrt(|r|^2/|b|^2) < 1e-14 or after maxiter-1 iterations
+void cg_vm_(scd_t* x, scd_t* b, int* conviter, int* maxiter) {
+  if (!g_qws.pce || (*g_qws.pce != 0 && *g_qws.pce != 1)) fatal(__func__, "pce must point at 0 or 1");
+  drop_in(__func__, [&] {
+    Lattice& L = lat_sync_eo("cg_vm_");
+    double* xx = (double*)eobuf(qws::P64, 0);
+    double* bb = (double*)eobuf(qws::P64, 2);
+    L.eo_upload_spinor(qws::P64, xx, x, 1, QWS_VLEND, 1);
+    L.eo_upload_spinor(qws::P64, bb, b, 1, QWS_VLEND, 1);
+    const int it = L.cg_eo(xx, bb, *maxiter, 1e-14, *g_qws.pce, -kappa2);
+    L.eo_download_spinor(qws::P64, xx, x, 1, QWS_VLEND, 1);
+    if (conviter) *conviter = it;
+  });
+}
+// mcg.cc:65 -- multi-shift CG: xx[k*vold .. (k+1)*vold) solves (mtdagmt + sigma[k]) x = b
+void mcg_vm_(scd_t* xx, scd_t* b, int* n, double* sigma, int* maxiter) {
+  if (!g_qws.pce || (*g_qws.pce != 0 && *g_qws.pce != 1)) fatal(__func__, "pce must point at 0 or 1");
+  if (*n < 1 || *n > 64) fatal(__func__, "the number of shifts must be in 1..64");
+  drop_in(__func__, [&] {
+    Lattice& L = lat_sync_eo("mcg_vm_");
+    const size_t eb = L.eo_spinor_bytes(qws::P64);
+    double* bb = (double*)eobuf(qws::P64, 2);
+    char* base = (char*)eobuf_n(qws::P64, *n);
+    std::vector<double*> xs(*n);
+    for (int k = 0; k < *n; ++k) xs[k] = (double*)(base + k * eb);
+    L.eo_upload_spinor(qws::P64, bb, b, 1, QWS_VLEND, 1);
+    L.mcg_eo(xs.data(), bb, *n, sigma, *maxiter, 1e-14, *g_qws.pce, -kappa2);
+    for (int k = 0; k < *n; ++k) L.eo_download_spinor(qws::P64, xs[k], xx + (size_t)k * vold, 1, QWS_VLEND, 1);
   });
 }
 void mtilde_dag_vm_(scd_t* out, scd_t* in) { eo_mtilde(__func__, qws::P64, 1, -kappa2, out, in); }

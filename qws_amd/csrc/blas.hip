@@ -381,6 +381,17 @@ __global__ void __launch_bounds__(RED_THREADS) k_sub(long long n, R* out, const 
   GRID_STRIDE(i, n) out[i] = a[i] - b[i];
 }
 
+// y = a x + y as one FMA (mode 0), or y = b y + a x with the product a x rounded first
+// (mode 1): the two update forms of QWS's CG solvers (fmadd_d / fnmadd_d, cg.cc, mcg.cc)
+template <typename R>
+__global__ void __launch_bounds__(RED_THREADS) k_lincomb(long long n, R a, const R* x, R b, R* y, int mode) {
+  if (mode == 0) {
+    GRID_STRIDE(i, n) y[i] = fma(a, x[i], y[i]);
+  } else {
+    GRID_STRIDE(i, n) y[i] = fma(b, y[i], a * x[i]);
+  }
+}
+
 // ------------------------------------------------------------------ launchers
 #define LAUNCH_B(kern, nb, ...) \
   hipLaunchKernelGGL(kern, dim3(RED_BLOCKS, (nb)), dim3(RED_THREADS), 0, st, __VA_ARGS__)
@@ -447,6 +458,9 @@ void launch_digest(long long nwords, const void* data, unsigned long long* out, 
 template <typename R> void launch_sub(long long n, R* out, const R* a, const R* b, hipStream_t st) {
   LAUNCH_B((k_sub<R>), 1, n, out, a, b);
 }
+template <typename R> void launch_lincomb(long long n, R a, const R* x, R b, R* y, int mode, hipStream_t st) {
+  LAUNCH_B((k_lincomb<R>), 1, n, a, x, b, y, mode);
+}
 
 // HBM probe: out = the sum of nin input arrays of n4 float4 each (nin reads per write;
 // nin = 1 is a copy), coalesced 128-bit accesses.  Each thread handles U elements a
@@ -498,7 +512,8 @@ void launch_stream(int cfg, long long n4, int nin, const float4* in, float4* out
                                     const R*, double*, hipStream_t);                                                  \
   template void launch_update_p<R>(const Batch&, long long, R*, const R*, const R*, const Scalars*, hipStream_t);     \
   template void launch_norm<R>(const Batch&, long long, const R*, double*, hipStream_t);                              \
-  template void launch_sub<R>(long long, R*, const R*, const R*, hipStream_t);
+  template void launch_sub<R>(long long, R*, const R*, const R*, hipStream_t);                                      \
+  template void launch_lincomb<R>(long long, R, const R*, R, R*, int, hipStream_t);
 INST(float)
 INST(double)
 template void launch_d2s_scaled<float>(const Batch&, long long, long long, long long, float*, const double*,

@@ -98,6 +98,8 @@ void launch_digest(long long nwords, const void* data, unsigned long long* out, 
 void launch_stream(int cfg, long long n4, int nin, const float4* in, float4* out, hipStream_t st);
 // out = a - b (real arrays of length n)
 template <typename R> void launch_sub(long long n, R* out, const R* a, const R* b, hipStream_t st);
+// real arrays of length n: mode 0 y = fma(a, x, y); mode 1 y = fma(b, y, a * x)
+template <typename R> void launch_lincomb(long long n, R a, const R* x, R b, R* y, int mode, hipStream_t st);
 
 // ---- layout conversion (device site order <-> host layouts); host data staged in device memory
 // kind: 0 = spinor (24 reals), 1 = links (4 x 18 reals), 2 = clover (72 reals)

@@ -697,6 +697,97 @@ int Lattice::bicgstab_eo(double* x, const double* b, int maxiter, double tol, in
   return iter;
 }
 
+double Lattice::reduce_real(long long n, const double* a, const double* b) {
+  if (n % 2) throw std::invalid_argument("reduce_real: an even number of reals");
+  if (!sc_eo_) sc_eo_ = dalloc<Scalars>(1);
+  double* part = reduction_partials(1);
+  const Batch B = single();
+  // complex view: |a|^2, or Re <a, b> = sum a_i b_i over the reals
+  if (b) launch_dot<double>(B, n / 2, a, b, part, st_);
+  else launch_norm<double>(B, n / 2, a, part, st_);
+  launch_finalize(B, part, 1, sc_eo_, FIN_RAW, st_);
+  read_scalars(sc_eo_, 1);
+  return sc_host_[0].red[0];
+}
+
+// cg_vm_ (cg.cc:65-172) on the parity fields: q = A x, r = b - q, p = r; per iteration
+// q = A p, alpha = |r|^2 / <p,q>, x += alpha p, r -= alpha q, stop when |r|/|b| < tol,
+// p = r + beta p.  A = mtilde^+ mtilde.
+int Lattice::cg_eo(double* x, const double* b, int maxiter, double tol, int pce, double fac) {
+  const long long nr = g_.V / 2 * 24;  // reals of a parity field
+  const size_t bytes = eo_spinor_bytes(P64);
+  double* q = (double*)work("eoc_q", bytes);
+  double* r = (double*)work("eoc_r", bytes);
+  double* p = (double*)work("eoc_p", bytes);
+  mtilde(P64, pce, 2, fac, q, x);
+  launch_sub<double>(nr, r, b, q, st_);
+  QWS_HIP(hipMemcpyAsync(p, r, bytes, hipMemcpyDeviceToDevice, st_));
+  const double bnorm = reduce_real(nr, b, nullptr);
+  double rho0 = reduce_real(nr, r, nullptr);
+  int iter;
+  for (iter = 1; iter < maxiter; ++iter) {
+    mtilde(P64, pce, 2, fac, q, p);
+    const double alpha = rho0 / reduce_real(nr, p, q);
+    launch_lincomb<double>(nr, alpha, p, 1.0, x, 0, st_);
+    launch_lincomb<double>(nr, -alpha, q, 1.0, r, 0, st_);
+    const double rnorm = reduce_real(nr, r, nullptr);
+    if (std::sqrt(rnorm / bnorm) < tol) break;
+    const double beta = rnorm / rho0;
+    rho0 = rnorm;
+    launch_lincomb<double>(nr, 1.0, r, beta, p, 1, st_);
+  }
+  sync();
+  return iter;
+}
+
+// mcg_vm_ (mcg.cc:65-219): one Krylov space for every shift; with alpha the negated
+// CG step, xi the shift's scale factors (sxi0/1/2) and pp_k its search direction:
+// r += alpha q; xs_k -= alpha xi2/xi1 pp_k; p = r + beta p; pp_k = beta (xi2/xi1)^2 pp_k + xi2 r.
+int Lattice::mcg_eo(double* const* xs, const double* b, int ns, const double* sigma, int maxiter, double tol, int pce,
+                    double fac) {
+  if (ns < 1 || ns > 64) throw std::invalid_argument("mcg: 1..64 shifts");
+  const long long nr = g_.V / 2 * 24;
+  const size_t bytes = eo_spinor_bytes(P64);
+  double* q = (double*)work("eom_q", bytes);
+  double* r = (double*)work("eom_r", bytes);
+  double* p = (double*)work("eom_p", bytes);
+  double* pp = (double*)work("eom_pp", bytes * ns);
+  std::vector<double> x0(ns, 1.0), x1(ns, 1.0), x2(ns, 0.0);
+  double alpha0 = 1, beta = 0;
+  QWS_HIP(hipMemcpyAsync(r, b, bytes, hipMemcpyDeviceToDevice, st_));
+  QWS_HIP(hipMemcpyAsync(p, b, bytes, hipMemcpyDeviceToDevice, st_));
+  const double bnorm = reduce_real(nr, b, nullptr);
+  double rho0 = bnorm;
+  for (int k = 0; k < ns; ++k) {
+    QWS_HIP(hipMemsetAsync(xs[k], 0, bytes, st_));
+    QWS_HIP(hipMemcpyAsync(pp + k * nr, r, bytes, hipMemcpyDeviceToDevice, st_));
+  }
+  int iter;
+  for (iter = 1; iter < maxiter; ++iter) {
+    mtilde(P64, pce, 2, fac, q, p);
+    const double alpha = -rho0 / reduce_real(nr, p, q);
+    launch_lincomb<double>(nr, alpha, q, 1.0, r, 0, st_);
+    const double rnorm = reduce_real(nr, r, nullptr);
+    for (int k = 0; k < ns; ++k) {
+      x2[k] = x1[k] * x0[k] * alpha0 / (alpha * beta * (x0[k] - x1[k]) + x0[k] * alpha0 * (1 - sigma[k] * alpha));
+      launch_lincomb<double>(nr, -(alpha * x2[k] / x1[k]), pp + k * nr, 1.0, xs[k], 0, st_);
+    }
+    if (std::sqrt(rnorm / bnorm) < tol) break;
+    beta = rnorm / rho0;
+    rho0 = rnorm;
+    launch_lincomb<double>(nr, 1.0, r, beta, p, 1, st_);
+    for (int k = 0; k < ns; ++k) {
+      const double c = beta * (x2[k] / x1[k]) * (x2[k] / x1[k]);
+      launch_lincomb<double>(nr, x2[k], r, c, pp + k * nr, 1, st_);
+      x0[k] = x1[k];
+      x1[k] = x2[k];
+    }
+    alpha0 = alpha;
+  }
+  sync();
+  return iter;
+}
+
 // ------------------------------------------------------------------ inner BiCGStab
 // Right-preconditioned BiCGStab for (D M_SAP) x = b, x0 = 0 (bicgstab_precdd_s.cc:85-559),
 // for every RHS k with live[k] != 0.  B.active must point at d_inner_mask_; an RHS

@@ -143,6 +143,8 @@ class Lattice {
   // n independent right-hand sides solved together (x[k] holds the initial guess)
   std::vector<SolveStats> solve_batch(double* const* x, const double* const* b, int n, const SolveParams& p);
   double norm2(int prec, const void* a);
+  // FP64 reductions over n reals to the host: |a|^2 (b == nullptr) or sum a_i b_i
+  double reduce_real(long long n, const double* a, const double* b);
   double true_residual(const double* x, const double* b);
   double time_dslash(int prec, int reps);
   // average device time (ms) of one fused SAP launch: reps times the 2*nsap + 2
@@ -176,6 +178,12 @@ class Lattice {
   // -kappa^2) from the initial guess in x, iterations 1..maxiter-1, stopping when
   // sqrt(|r|^2/|b|^2) < tol (the reference's 1e-14); returns the last iteration
   int bicgstab_eo(double* x, const double* b, int maxiter, double tol, int pce, double fac);
+  // cg.cc:65 -- CG on mtilde^+ mtilde from the initial guess in x; returns the iterations
+  int cg_eo(double* x, const double* b, int maxiter, double tol, int pce, double fac);
+  // mcg.cc:65 -- multi-shift CG: the parity field xs[k] solves
+  // (mtilde^+ mtilde + sigma[k]) xs[k] = b, from zero; returns the iterations
+  int mcg_eo(double* const* xs, const double* b, int ns, const double* sigma, int maxiter, double tol, int pce,
+             double fac);
 
   // ---- batched building blocks (used by OuterBatch)
   void dslash_batch(const Batch& B, double* out, const double* in);

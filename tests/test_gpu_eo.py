@@ -241,3 +241,86 @@ def test_dropin_bicgstab_vm_case0():
     assert K.check_value("bnorm^2", O.norm2(bd), K.expected("CASE0", "bnorm^2", "EO"))
     assert O.norm2(bd - r) < 1e-22
     assert K.check_value("xnorm^2", O.norm2(x), K.expected("CASE0", "xnorm^2", "EO"))
+
+
+def _case0_dropin_setup(L):
+    nloc = [32, 6, 4, 3]
+    _, U, C, b, _ = O.case_inputs(nloc)
+    ci = lambda v: ctypes.byref(ctypes.c_int(v))
+    keep = [(ctypes.c_int * 4)(1, 1, 1, 1), (ctypes.c_int * 4)(1, 1, 1, -1), ctypes.c_int(0), ctypes.c_int(1),
+            (ctypes.c_int * 4)(1, 2, 2, 2)]
+    L.qws_init_(ci(32), ci(6), ci(4), ci(3), keep[0], keep[1], ctypes.byref(keep[2]), ctypes.byref(keep[3]), keep[4])
+    vold = ctypes.c_int.in_dll(L, "vold").value
+    glud = np.ctypeslib.as_array((ctypes.c_double * (vold * 8 * 144)).from_address(
+        ctypes.c_void_p.in_dll(L, "glud").value))
+    clvd = np.ctypeslib.as_array((ctypes.c_double * (vold * 2 * 576)).from_address(
+        ctypes.c_void_p.in_dll(L, "clvd").value))
+    glud[:] = RW.links_to_aosoa(U, nloc, 8).ravel()
+    clvd[:] = RW.clover_to_aosoa(C, nloc, 8).ravel()
+    ctypes.c_double.in_dll(L, "kappa").value = 0.05
+    ctypes.c_double.in_dll(L, "kappa2").value = 0.0025
+    L.qws_mi355x_fields_modified_()
+    bd = np.ascontiguousarray(RW.spinor_to_aosoa(b, nloc, 8).reshape(2 * vold, -1)[:vold])
+    return nloc, U, C, b, bd, keep
+
+
+@pytest.mark.skipif(not RW.available(), reason="oracle/_ref/libqws_ref.so not built")
+def test_dropin_cg_mcg_vm_case0():
+    """cg_vm_ and mcg_vm_ through the QWS ABI on data/CASE0's fields (kappa 0.05) agree
+    with the reference's own cg_vm_/mcg_vm_ (oracle/_ref) and meet the residual
+    (mtdagmt_vm_ + sigma) x = b; a shift whose scale factor underflows ends as NaN in
+    both, as in the reference."""
+    L = Q.lib()
+    nloc, U, C, b, bd, keep = _case0_dropin_setup(L)
+    sigma = [0.0, 0.01, 0.05, 0.5]
+    ci = lambda v: ctypes.byref(ctypes.c_int(v))
+    x = np.zeros_like(bd)
+    it = ctypes.c_int(0)
+    L.cg_vm_(_ptr(x), _ptr(bd), ctypes.byref(it), ci(3000))
+    xx = np.zeros((len(sigma),) + bd.shape)
+    sg = np.array(sigma)
+    L.mcg_vm_(_ptr(xx), _ptr(bd), ci(len(sigma)), sg.ctypes.data_as(ctypes.c_void_p), ci(3000))
+    res = []
+    for v in [x] + [xx[k] for k in range(3)]:
+        r = np.zeros_like(bd)
+        L.mtdagmt_vm_(_ptr(r), _ptr(np.ascontiguousarray(v)))
+        res.append(r)
+    L.qws_finalize_()
+    assert 500 < it.value < 3000
+    assert O.norm2(bd - res[0]) < 1e-24 * O.norm2(bd)
+    for k in range(3):
+        assert O.norm2(bd - res[k + 1] - sigma[k] * xx[k]) < 1e-24 * O.norm2(bd), k
+    assert np.isnan(xx[3]).any()
+    # both stop at |r|/|b| < 1e-14 on an operator of condition ~1e4-1e5, after ~1100
+    # iterations with differently ordered sums: the solutions agree to ~cond x 1e-14
+    ref = RW.call(nloc, 0.05, U, C, b, ["cg_vm", "mcg_vm"], maxiter=3000, sigma=sigma)
+    assert rel(x, ref["cg_vm"]) < 1e-8
+    for k in range(3):
+        assert rel(xx[k], ref["mcg_vm"][k]) < 1e-8, k
+
+
+def test_eo_cg_mcg_device():
+    """The device-resident CG and multi-shift CG on random fields (even extents, fields
+    gathered from the lattice) meet their residuals, and the zero shift equals CG."""
+    L, B = [8, 4, 4, 8], [4, 2, 2, 4]
+    V = int(np.prod(L))
+    rng = np.random.default_rng(31)
+    U, C = K.random_su3_like(V, rng)
+    lat = Q.Lattice(L, B, -1, 18, kappa=0.12)
+    lat.load_gauge(U)
+    lat.load_clover(C)
+    lat.eo_from_lattice()
+    bh = rng.standard_normal((V // 2, 24))
+    b = lat.eo_spinor(Q.F64, bh)
+    x = lat.eo_spinor(Q.F64, np.zeros_like(bh))
+    it = lat.eo_cg(x, b, maxiter=5000, tol=1e-12)
+    assert 1 < it < 5000
+    sigma = [0.0, 0.1, 1.0]
+    xs, it2 = lat.eo_mcg(b, sigma, maxiter=5000, tol=1e-12)
+    assert abs(it2 - it) <= 1
+    q = lat.eo_spinor(Q.F64)
+    for s, xk in zip(sigma, xs):
+        lat.mtilde(q, xk, 0, 2)
+        r = bh - q.download() - s * xk.download()
+        assert O.norm2(r) < 1e-22 * O.norm2(bh), s
+    assert rel(xs[0].download(), x.download()) < 1e-10

@@ -390,3 +390,58 @@ def test_even_odd_family_case0_fp32():
     }
     for name, v in got.items():
         assert K.check_value(name, O.norm2(v), K.expected("CASE0", name)), name
+
+
+# ---------------------------------------------------------------- even/odd Krylov solvers
+def _spd(n, seed, cond=50.0):
+    rng = np.random.default_rng(seed)
+    q, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    return (q * np.geomspace(1.0, cond, n)) @ q.T, rng.standard_normal(n)
+
+
+def test_eo_cg_restatement_solves_spd_systems():
+    """cg reaches the linear-algebra solution from zero and from a guess; a plausible
+    slip (a sign in the residual update, beta from the wrong norms) fails this."""
+    from oracle import eo_solvers as ES
+    A, b = _spd(48, 1)
+    want = np.linalg.solve(A, b)
+    for x0 in (np.zeros(48), np.random.default_rng(2).standard_normal(48)):
+        x, it = ES.cg(lambda v: A @ v, b, x0, 200)
+        assert 1 < it < 200
+        assert np.linalg.norm(x - want) < 1e-11 * np.linalg.norm(want)
+    x, it = ES.cg(lambda v: A @ v, b, np.zeros(48), 3)  # maxiter bounds the iterations
+    assert it == 3
+
+
+def test_eo_mcg_restatement_solves_every_shift():
+    """mcg's x_k solve (A + sigma_k) x = b: the sign and placement of the shift in the
+    xi recursion are fixed by comparing with the dense solutions."""
+    from oracle import eo_solvers as ES
+    A, b = _spd(48, 3)
+    sigma = [0.0, 0.05, 0.7, 3.0]
+    xs, it = ES.mcg(lambda v: A @ v, b, sigma, 300)
+    assert 1 < it < 300
+    for s, x in zip(sigma, xs):
+        want = np.linalg.solve(A + s * np.eye(48), b)
+        assert np.linalg.norm(x - want) < 1e-10 * np.linalg.norm(want), s
+    x0, _ = ES.cg(lambda v: A @ v, b, np.zeros(48), 300)
+    assert np.linalg.norm(xs[0] - x0) < 1e-11 * np.linalg.norm(x0)
+
+
+@needs_ref
+def test_eo_solvers_match_reference_library():
+    """The reference's own cg_vm_ and mcg_vm_ (oracle/_ref) on data/CASE0's fields
+    (kappa 0.05, x-domain 0 as the parity field) agree with the restatement run on the
+    reference's mtdagmt_vm_.  The normal equations take ~1100 iterations here.  A
+    shift whose scale factor xi underflows before the unshifted residual converges
+    ends as NaN in the reference (mcg.cc:160 divides by it), and in the restatement."""
+    nloc = [32, 6, 4, 3]
+    _, U, C, b, _ = O.case_inputs(nloc)
+    res = RW.call(nloc, 0.05, U, C, b, ["cg_vm", "mcg_vm"], maxiter=3000, sigma=[0.0, 0.01, 0.05, 0.5])
+    rel = lambda a, w: np.linalg.norm(a - w) / np.linalg.norm(w)
+    assert rel(res["cg_vm_oracle"], res["cg_vm"]) < 1e-12
+    assert 500 < int(res["cg_vm_oracle_iter"]) < 3000
+    for k in range(3):
+        assert rel(res["mcg_vm_oracle"][k], res["mcg_vm"][k]) < 1e-12, k
+    assert rel(res["mcg_vm"][0], res["cg_vm"]) < 1e-12
+    assert np.isnan(res["mcg_vm"][3]).any() and np.isnan(res["mcg_vm_oracle"][3]).any()
