@@ -218,6 +218,11 @@ int qwsg_set_kappa(qwsg_lattice* lat, double kappa);
 /* SAP kernels: 1 (default) = one fused launch per half-sweep where the geometry
  * allows it (proper colouring, 256-site blocks), 0 = one launch per operator step */
 int qwsg_set_sap_fused(qwsg_lattice* lat, int on);
+/* FP32 fused SAP: 1 (default) = each face update reads a face buffer that the
+ * previous half-sweep wrote (projected, for backward hops link-multiplied, face
+ * spinors of the other colour); 0 = it gathers the other colour's spinors and links.
+ * Same results bit for bit.  No counterpart in the reference. */
+int qwsg_set_sap_face_buffer(qwsg_lattice* lat, int on);
 int qwsg_volume(qwsg_lattice* lat, long long* V);
 /* host plain layout: links [4][V][3][3][2] (QWS c[i][j] of the link x -> x+mu),
  * clover inverse [V][2][36] (QWS packing), site x + Lx*(y + Ly*(z + Lz*t)) */

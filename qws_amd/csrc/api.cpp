@@ -67,6 +67,9 @@ int qwsg_destroy(qwsg_lattice* h) {
 int qwsg_set_sap_fused(qwsg_lattice* h, int on) {
   return guarded([&] { h->lat->set_fused_sap(on != 0); });
 }
+int qwsg_set_sap_face_buffer(qwsg_lattice* h, int on) {
+  return guarded([&] { h->lat->set_sap_face_buffer(on != 0); });
+}
 int qwsg_set_kappa(qwsg_lattice* h, double k) {
   return guarded([&] { h->lat->set_kappa(k); });
 }

@@ -62,6 +62,8 @@ Lattice::Lattice(const int L[4], const int B[4], int tbc, int links32) {
   g_.xcd_remap = (xr && xr[0] == '0') ? 0 : 1;
   // full-operator kernel (FP32, 256-site blocks): LDS-staged, compiled for 4 waves per
   // SIMD, unless QWS_MI355X_DSLASH names another (0, 3, 4); autotune_dslash() times them
+  const char* fbv = std::getenv("QWS_MI355X_SAP_FB");
+  use_fb_ = !(fbv && fbv[0] == '0');
   const char* dk = std::getenv("QWS_MI355X_DSLASH");
   g_.dslash_kernel = dk ? std::atoi(dk) : 4;
   if (g_.dslash_kernel != 0 && g_.dslash_kernel != 3) g_.dslash_kernel = 4;
@@ -411,15 +413,36 @@ template <typename S> bool Lattice::fused_ok() const {
 // the nsap sweeps of qws.cc:809-841 with the face update of each half-sweep
 // done at the start of the next one
 template <typename S>
-void Lattice::sap_sweeps_fused(const Batch& B, const S* in, S* s, S* q, int nsap, int nm) {
+void Lattice::sap_sweeps_fused(const Batch& B, const S* in, S* s, S* q, int nsap, int nm, float* fb, long long fbs) {
   const FieldSet<S> f = fields<S>();
   for (int isap = 0; isap < nsap; ++isap)
     for (int c = 0; c < 2; ++c) {
       int flags = SAP_EPI_SUPDATE;
       if (isap == 0) flags |= SAP_SRC_IN;
       if (isap > 0 || c == 1) flags |= SAP_FACE_PRO;
-      launch_sap_block<S>(g_, sel(c), B, f.nl, (float)kappa_, nm, flags, s, in, q, q, nullptr, f.U, f.C, st_);
+      launch_sap_block<S>(g_, sel(c), B, f.nl, (float)kappa_, nm, fb_flags(flags, fb, true), s, in, q, q, nullptr, f.U,
+                          f.C, fb, fbs, st_);
     }
+}
+
+// The face buffer of a fused sequence (FP32): every launch but the last writes it,
+// every face update reads it instead of gathering the other colour's q.
+template <typename S> float* Lattice::sap_face_buffer(const Batch& B, long long& fbs) {
+  fbs = 0;
+  if constexpr (!std::is_same<S, float>::value) return nullptr;
+  fbs = sap_face_buffer_floats(g_);
+  if (!fbs || !use_fb_) {
+    fbs = 0;
+    return nullptr;
+  }
+  return (float*)work("sap_fb", sizeof(float) * (size_t)fbs * B.n);
+}
+
+int Lattice::fb_flags(int flags, const float* fb, bool produce) {
+  if (!fb) return flags;
+  if (produce) flags |= SAP_FACE_OUT;
+  if (flags & SAP_FACE_PRO) flags |= SAP_FACE_FB;
+  return flags;
 }
 
 // prec_ddd_s_ (qws.cc:776-926): out = D M_SAP in, with the final sequence of the
@@ -435,13 +458,17 @@ void Lattice::prec_ddd_t(const Batch& B, S* out, const S* in, int nsap, int nm) 
     if (fused_ok<S>()) {
       const FieldSet<float> f = fields<float>();
       const int src = nsap == 0 ? SAP_SRC_IN : 0;
-      sap_sweeps_fused<float>(B, in, s, q, nsap, nm);
+      long long fbs;
+      float* fb = sap_face_buffer<float>(B, fbs);
+      sap_sweeps_fused<float>(B, in, s, q, nsap, nm, fb, fbs);
       // final e: q_e = A_e s_e, out_e = D_in q_e
-      launch_sap_block<float>(g_, sel(0), B, f.nl, (float)kappa_, nm, SAP_EPI_OUT | src | (nsap > 0 ? SAP_FACE_PRO : 0), s,
-                       in, q, q, out, f.U, f.C, st_);
+      launch_sap_block<float>(g_, sel(0), B, f.nl, (float)kappa_, nm,
+                              fb_flags(SAP_EPI_OUT | src | (nsap > 0 ? SAP_FACE_PRO : 0), fb, true), s, in, q, q, out,
+                              f.U, f.C, fb, fbs, st_);
       // final o: s_o += kappa T^-1 H_face q, q_o = A_o s_o, out_o = D_oe q_e + D_in q_o
-      launch_sap_block<float>(g_, sel(1), B, f.nl, (float)kappa_, nm, SAP_EPI_OUT_FACE | src | SAP_FACE_PRO, s, in, q, q,
-                       out, f.U, f.C, st_);
+      launch_sap_block<float>(g_, sel(1), B, f.nl, (float)kappa_, nm,
+                              fb_flags(SAP_EPI_OUT_FACE | src | SAP_FACE_PRO, fb, false), s, in, q, q, out, f.U, f.C, fb,
+                              fbs, st_);
       hop<S>(sel(0), B, HOPS_FACE, -k, out, out, nullptr, nullptr, q);  // out_e += D_eo q_o
       return;
     }
@@ -477,11 +504,14 @@ void Lattice::prec_s_t(const Batch& B, S* out, const S* in, int nsap, int nm) {
     if (fused_ok<S>()) {
       const FieldSet<S> f = fields<S>();
       const int src = nsap == 0 ? SAP_SRC_IN : 0;
-      sap_sweeps_fused<S>(B, in, s, q, nsap, nm);
-      launch_sap_block<S>(g_, sel(0), B, f.nl, (float)kappa_, nm, SAP_EPI_NONE | src | (nsap > 0 ? SAP_FACE_PRO : 0),
-                          s, in, q, out, nullptr, f.U, f.C, st_);
-      launch_sap_block<S>(g_, sel(1), B, f.nl, (float)kappa_, nm, SAP_EPI_NONE | src | SAP_FACE_PRO, s, in, out, out,
-                          nullptr, f.U, f.C, st_);
+      long long fbs;
+      float* fb = sap_face_buffer<S>(B, fbs);
+      sap_sweeps_fused<S>(B, in, s, q, nsap, nm, fb, fbs);
+      launch_sap_block<S>(g_, sel(0), B, f.nl, (float)kappa_, nm,
+                          fb_flags(SAP_EPI_NONE | src | (nsap > 0 ? SAP_FACE_PRO : 0), fb, true), s, in, q, out, nullptr,
+                          f.U, f.C, fb, fbs, st_);
+      launch_sap_block<S>(g_, sel(1), B, f.nl, (float)kappa_, nm, fb_flags(SAP_EPI_NONE | src | SAP_FACE_PRO, fb, false),
+                          s, in, out, out, nullptr, f.U, f.C, fb, fbs, st_);
       return;
     }
   }
@@ -1151,12 +1181,14 @@ double Lattice::time_sap_t(int reps, int nsap, int nm) {
   }
   const Batch B = single();
   const FieldSet<S> f = fields<S>();
+  long long fbs;
+  float* fb = sap_face_buffer<S>(B, fbs);
   auto apply = [&] {
-    sap_sweeps_fused<S>(B, in, s, q, nsap, nm);
-    launch_sap_block<S>(g_, sel(0), B, f.nl, (float)kappa_, nm, SAP_EPI_OUT | SAP_FACE_PRO, s, in, q, q, out, f.U, f.C,
-                        st_);
-    launch_sap_block<S>(g_, sel(1), B, f.nl, (float)kappa_, nm, SAP_EPI_OUT_FACE | SAP_FACE_PRO, s, in, q, q, out, f.U,
-                        f.C, st_);
+    sap_sweeps_fused<S>(B, in, s, q, nsap, nm, fb, fbs);
+    launch_sap_block<S>(g_, sel(0), B, f.nl, (float)kappa_, nm, fb_flags(SAP_EPI_OUT | SAP_FACE_PRO, fb, true), s, in, q,
+                        q, out, f.U, f.C, fb, fbs, st_);
+    launch_sap_block<S>(g_, sel(1), B, f.nl, (float)kappa_, nm, fb_flags(SAP_EPI_OUT_FACE | SAP_FACE_PRO, fb, false), s,
+                        in, q, q, out, f.U, f.C, fb, fbs, st_);
   };
   apply();  // warm up
   hipEvent_t e0, e1;

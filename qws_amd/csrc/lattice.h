@@ -153,6 +153,9 @@ class Lattice {
   // FP32 full-operator kernel on 256-site blocks (0 global gathers, 3/4 LDS-staged);
   // the variants give the same results, autotune_dslash keeps the fastest on this
   // lattice (timed over `reps` launches each) and returns it
+  // FP32 fused SAP: face updates through the face buffer (default) or by gathering
+  // the other colour's spinors; both give the same results bit for bit
+  void set_sap_face_buffer(bool on) { use_fb_ = on; }
   void set_dslash_kernel(int k);
   int dslash_kernel() const { return g_.dslash_kernel; }
   int autotune_dslash(int reps);
@@ -205,7 +208,11 @@ class Lattice {
   template <typename S> void sap_sweeps(const Batch& B, const S* in, S* s, S* q, S* tmp, int nsap, int nm);
   template <typename S> bool fused_ok() const;
   template <typename S> double time_sap_t(int reps, int nsap, int nm);
-  template <typename S> void sap_sweeps_fused(const Batch& B, const S* in, S* s, S* q, int nsap, int nm);
+  template <typename S>
+  void sap_sweeps_fused(const Batch& B, const S* in, S* s, S* q, int nsap, int nm, float* fb, long long fbs);
+  template <typename S> float* sap_face_buffer(const Batch& B, long long& fbs);
+  static int fb_flags(int flags, const float* fb, bool produce);
+  bool use_fb_ = true;
   template <typename S> void prec_ddd_t(const Batch& B, S* out, const S* in, int nsap, int nm);
   template <typename S> void prec_s_t(const Batch& B, S* out, const S* in, int nsap, int nm);
   void msap_h_t(const Batch& B, float* out, const float* in, int nsap, int nm);

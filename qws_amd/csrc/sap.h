@@ -14,11 +14,17 @@ enum SapFlags {
   SAP_EPI_MASK = 3,
   SAP_FACE_PRO = 4,      // prologue: s_c += kappa T^-1 H_face qface
   SAP_SRC_IN = 8,        // take s_c from `in` (first sweep: s = in)
+  SAP_FACE_OUT = 16,     // FP32: also write the face buffer for the next launch's prologue
+  SAP_FACE_FB = 32,      // FP32: the prologue reads the face buffer instead of gathering qface
 };
 
 bool sap_fusable(const Geo& g);
+// floats per RHS of the face buffer, or 0 when the block shape does not allow it
+// (its slots must fit the iterate's LDS: 2 sum_mu Vb / B_mu <= 2 Vb)
+long long sap_face_buffer_floats(const Geo& g);
 template <typename S>
 void launch_sap_block(const Geo& g, BlockSel sel, const Batch& B, int nl, float kappa, int nm, int flags, S* s,
-                      const S* in, const S* qface, S* qout, S* out, const S* U, const S* C, hipStream_t st);
+                      const S* in, const S* qface, S* qout, S* out, const S* U, const S* C, float* fb,
+                      long long fbs, hipStream_t st);
 
 }  // namespace qws

@@ -31,19 +31,30 @@ constexpr int VB = 512;  // e.g. 8x4x4x4 blocks (FP32: 144 KB of LDS per workgro
 
 bool sap_fusable(const Geo& g) { return g.Vb == 256 || g.Vb == 512; }
 
+long long sap_face_buffer_floats(const Geo& g) {
+  if (!sap_fusable(g)) return 0;
+  const int S = g.Vb == 512 ? sap512::face_slots(g) : sap256::face_slots(g);
+  if (S > 2 * g.Vb) return 0;
+  return (long long)g.nblk * S * 12;
+}
+
 template <typename S>
 void launch_sap_block(const Geo& g, BlockSel sel, const Batch& B, int nl, float kappa, int nm, int flags, S* s,
-                      const S* in, const S* qface, S* qout, S* out, const S* U, const S* C, hipStream_t st) {
+                      const S* in, const S* qface, S* qout, S* out, const S* U, const S* C, float* fb,
+                      long long fbs, hipStream_t st) {
   if (sel.n == 0) return;
   if (nm < 1) nm = 1;  // the first application is unconditional (static_solver.cc:77-101)
-  if (g.Vb == 512) sap512::launch<S>(g, sel, B, nl, kappa, nm, flags, s, in, qface, qout, out, U, C, st);
-  else sap256::launch<S>(g, sel, B, nl, kappa, nm, flags, s, in, qface, qout, out, U, C, st);
+  if ((flags & (SAP_FACE_OUT | SAP_FACE_FB)) && (!fb || !std::is_same<S, float>::value))
+    throw std::invalid_argument("fused SAP: the face buffer is FP32 only and needs a buffer");
+  if (g.Vb == 512) sap512::launch<S>(g, sel, B, nl, kappa, nm, flags, s, in, qface, qout, out, U, C, fb, fbs, st);
+  else sap256::launch<S>(g, sel, B, nl, kappa, nm, flags, s, in, qface, qout, out, U, C, fb, fbs, st);
 }
 
 template void launch_sap_block<float>(const Geo&, BlockSel, const Batch&, int, float, int, int, float*, const float*,
-                                      const float*, float*, float*, const float*, const float*, hipStream_t);
+                                      const float*, float*, float*, const float*, const float*, float*, long long,
+                                      hipStream_t);
 template void launch_sap_block<__half>(const Geo&, BlockSel, const Batch&, int, float, int, int, __half*,
                                        const __half*, const __half*, __half*, __half*, const __half*, const __half*,
-                                       hipStream_t);
+                                       float*, long long, hipStream_t);
 
 }  // namespace qws
