@@ -1109,8 +1109,19 @@ std::vector<SolveStats> Lattice::solve_batch(double* const* x, const double* con
   }
   OuterBatch& ob = *batch_;
   ob.set_params(prm);
-  for (int k = 0; k < nrhs; ++k) ob.admit(k, b[k], x[k]);
+  // a zero source has the exact solution x = 0 (the relative residual is undefined
+  // there, and the normalised inner solve would divide by zero): it takes no slot
   std::vector<int> open(nrhs, 1);
+  for (int k = 0; k < nrhs; ++k) {
+    if (norm2(P64, b[k]) == 0.0) {
+      open[k] = 0;
+      out[k].true_residual = 0;
+      QWS_HIP(hipMemsetAsync(x[k], 0, spinor_bytes(P64), st_));
+      ob.release(k);
+    } else {
+      ob.admit(k, b[k], x[k]);
+    }
+  }
   for (;;) {
     while (ob.step() > 0) {
     }
@@ -1132,7 +1143,8 @@ std::vector<SolveStats> Lattice::solve_batch(double* const* x, const double* con
     if (!again) break;
   }
   for (int k = 0; k < nrhs; ++k)
-    QWS_HIP(hipMemcpyAsync(x[k], ob.x(k), spinor_bytes(P64), hipMemcpyDeviceToDevice, st_));
+    if (ob.running(k) || ob.finished(k))
+      QWS_HIP(hipMemcpyAsync(x[k], ob.x(k), spinor_bytes(P64), hipMemcpyDeviceToDevice, st_));
   sync();
   const double sec = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   for (auto& s : out) s.seconds = sec;

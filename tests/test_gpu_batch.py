@@ -67,3 +67,19 @@ def test_twelve_point_sources(lat):
     # gamma_5 hermiticity is not assumed; only that distinct sources give distinct solutions
     sols = [x.download() for x in xs]
     assert not np.allclose(sols[0], sols[1])
+
+
+def test_zero_source_in_a_batch(lat):
+    """A zero source has the exact solution 0: it takes no iterations and leaves the
+    other right-hand sides of the batch as they are alone (bit for bit)."""
+    bs = _sources(lat, 3)
+    zero = lat.spinor(Q.F64, np.zeros((lat.V, 3, 4, 2)))
+    xs = [lat.spinor(Q.F64) for _ in range(4)]
+    xs[1] = lat.spinor(Q.F64, np.ones((lat.V, 3, 4, 2)))  # content that must be replaced by 0
+    st = lat.solve_batch(xs, [bs[0], zero, bs[1], bs[2]], **PRM)
+    assert not np.any(xs[1].download())
+    assert st[1]["outer_iters"] == 0 and st[1]["inner_iters"] == 0 and st[1]["true_residual"] == 0
+    for k, kb in ((0, 0), (2, 1), (3, 2)):
+        x1 = lat.spinor(Q.F64)
+        lat.solve(x1, bs[kb], **PRM)
+        assert np.array_equal(x1.download(), xs[k].download()), k
