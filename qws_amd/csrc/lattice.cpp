@@ -429,7 +429,7 @@ void Lattice::sap_sweeps_fused(const Batch& B, const S* in, S* s, S* q, int nsap
 // every face update reads it instead of gathering the other colour's q.
 template <typename S> float* Lattice::sap_face_buffer(const Batch& B, long long& fbs) {
   fbs = 0;
-  if constexpr (!std::is_same<S, float>::value) return nullptr;
+  if constexpr (std::is_same<S, double>::value) return nullptr;
   fbs = sap_face_buffer_floats(g_);
   if (!fbs || !use_fb_) {
     fbs = 0;

@@ -14,8 +14,8 @@ enum SapFlags {
   SAP_EPI_MASK = 3,
   SAP_FACE_PRO = 4,      // prologue: s_c += kappa T^-1 H_face qface
   SAP_SRC_IN = 8,        // take s_c from `in` (first sweep: s = in)
-  SAP_FACE_OUT = 16,     // FP32: also write the face buffer for the next launch's prologue
-  SAP_FACE_FB = 32,      // FP32: the prologue reads the face buffer instead of gathering qface
+  SAP_FACE_OUT = 16,     // also write the face buffer (FP32 slots) for the next launch's prologue
+  SAP_FACE_FB = 32,      // the prologue reads the face buffer instead of gathering qface
 };
 
 bool sap_fusable(const Geo& g);

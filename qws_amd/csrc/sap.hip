@@ -44,8 +44,8 @@ void launch_sap_block(const Geo& g, BlockSel sel, const Batch& B, int nl, float 
                       long long fbs, hipStream_t st) {
   if (sel.n == 0) return;
   if (nm < 1) nm = 1;  // the first application is unconditional (static_solver.cc:77-101)
-  if ((flags & (SAP_FACE_OUT | SAP_FACE_FB)) && (!fb || !std::is_same<S, float>::value))
-    throw std::invalid_argument("fused SAP: the face buffer is FP32 only and needs a buffer");
+  if ((flags & (SAP_FACE_OUT | SAP_FACE_FB)) && (!fb || sap_face_buffer_floats(g) == 0))
+    throw std::invalid_argument("fused SAP: face-buffer flags need a buffer and a block shape that allows one");
   if (g.Vb == 512) sap512::launch<S>(g, sel, B, nl, kappa, nm, flags, s, in, qface, qout, out, U, C, fb, fbs, st);
   else sap256::launch<S>(g, sel, B, nl, kappa, nm, flags, s, in, qface, qout, out, U, C, fb, fbs, st);
 }

@@ -58,6 +58,24 @@ def test_msap_h_matches_oracle(prob, nsap, nm):
         assert rel(got, p.orc.prec_s(b, nsap, nm)) > 5 * TOL_H
 
 
+@pytest.mark.parametrize("L,B,su3", [([16, 8, 8, 8], [4, 4, 4, 4], True), ([16, 8, 8, 8], [4, 4, 4, 4], False),
+                                     ([16, 8, 8, 8], [8, 4, 4, 4], True)])
+def test_msap_h_face_buffer_is_bitwise_the_gather(L, B, su3):
+    """The FP16 SAP's face updates through the face buffer equal the gathered ones
+    bit for bit (the same FP32 hop arithmetic on the same FP16 fields)."""
+    p = Problem(L, B, su3=su3)
+    b = p.b.astype(np.float32)
+    res = {}
+    for fb in (True, False):
+        p.lat.set_sap_face_buffer(fb)
+        out = p.lat.spinor(Q.F32)
+        p.lat.msap_h(out, p.lat.spinor(Q.F32, b), 3, 2)
+        res[fb] = out.download()
+    p.lat.set_sap_face_buffer(True)
+    assert np.array_equal(res[True], res[False])
+    assert rel(res[True], p.orc.msap_h(b, 3, 2)) < 2 * TOL_H
+
+
 def test_msap_h_links12():
     """12-real links: the third row is rebuilt in FP32 from the FP16 rows 0 and 1,
     so the oracle (which rounds all 18 reals) agrees to a few FP16 roundings."""
