@@ -466,10 +466,13 @@ void Lattice::prec_ddd_t(const Batch& B, S* out, const S* in, int nsap, int nm) 
                               fb_flags(SAP_EPI_OUT | src | (nsap > 0 ? SAP_FACE_PRO : 0), fb, true), s, in, q, q, out,
                               f.U, f.C, fb, fbs, st_);
       // final o: s_o += kappa T^-1 H_face q, q_o = A_o s_o, out_o = D_oe q_e + D_in q_o
+      // (and, with the face buffer, the face data of q_o for the last step)
       launch_sap_block<float>(g_, sel(1), B, f.nl, (float)kappa_, nm,
-                              fb_flags(SAP_EPI_OUT_FACE | src | SAP_FACE_PRO, fb, false), s, in, q, q, out, f.U, f.C, fb,
+                              fb_flags(SAP_EPI_OUT_FACE | src | SAP_FACE_PRO, fb, true), s, in, q, q, out, f.U, f.C, fb,
                               fbs, st_);
-      hop<S>(sel(0), B, HOPS_FACE, -k, out, out, nullptr, nullptr, q);  // out_e += D_eo q_o
+      // out_e += D_eo q_o
+      if (fb) launch_sap_face(g_, sel(0), B, f.nl, -k, out, fb, fbs, f.U, f.C, st_);
+      else hop<S>(sel(0), B, HOPS_FACE, -k, out, out, nullptr, nullptr, q);
       return;
     }
   }

@@ -22,6 +22,10 @@ bool sap_fusable(const Geo& g);
 // floats per RHS of the face buffer, or 0 when the block shape does not allow it
 // (its slots must fit the iterate's LDS: 2 sum_mu Vb / B_mu <= 2 Vb)
 long long sap_face_buffer_floats(const Geo& g);
+// out += gamma T^-1 H_face q on the blocks of `sel` (FP32), H_face q from the face
+// buffer that the preceding SAP launch of the other colour wrote (SAP_FACE_OUT)
+void launch_sap_face(const Geo& g, BlockSel sel, const Batch& B, int nl, float gamma, float* out, const float* fb,
+                     long long fbs, const float* U, const float* C, hipStream_t st);
 template <typename S>
 void launch_sap_block(const Geo& g, BlockSel sel, const Batch& B, int nl, float kappa, int nm, int flags, S* s,
                       const S* in, const S* qface, S* qout, S* out, const S* U, const S* C, float* fb,

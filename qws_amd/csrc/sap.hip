@@ -50,6 +50,14 @@ void launch_sap_block(const Geo& g, BlockSel sel, const Batch& B, int nl, float 
   else sap256::launch<S>(g, sel, B, nl, kappa, nm, flags, s, in, qface, qout, out, U, C, fb, fbs, st);
 }
 
+void launch_sap_face(const Geo& g, BlockSel sel, const Batch& B, int nl, float gamma, float* out, const float* fb,
+                     long long fbs, const float* U, const float* C, hipStream_t st) {
+  if (sel.n == 0) return;
+  if (!fb || sap_face_buffer_floats(g) == 0) throw std::invalid_argument("face term: no face buffer for this lattice");
+  if (g.Vb == 512) sap512::launch_face(g, sel, B, nl, gamma, out, fb, fbs, U, C, st);
+  else sap256::launch_face(g, sel, B, nl, gamma, out, fb, fbs, U, C, st);
+}
+
 template void launch_sap_block<float>(const Geo&, BlockSel, const Batch&, int, float, int, int, float*, const float*,
                                       const float*, float*, float*, const float*, const float*, float*, long long,
                                       hipStream_t);
