@@ -64,7 +64,7 @@ k_hop(Geo g, const int* __restrict__ list, int wg_per_blk, int nrhs, long long s
       for (int k = 0; k < 12; ++k) v[k] = csub(v[k], w[k]);
     }
 #pragma unroll
-    for (int k = 0; k < 12; ++k) v[k] = cadd(v[k], cscale(gamma, acc[k]));
+    for (int k = 0; k < 12; ++k) v[k] = {fmar(gamma, acc[k].re, v[k].re), fmar(gamma, acc[k].im, v[k].im)};
     SpinorIO<S>::store(out + off, s, g.V, v);
   }
 }
@@ -96,7 +96,7 @@ k_dslash(Geo g, const int* __restrict__ order, int wg_per_blk, const int* __rest
   Cx<R> v[12];
   SpinorIO<S>::load(in, s, g.V, v);
 #pragma unroll
-  for (int k = 0; k < 12; ++k) v[k] = cadd(v[k], cscale(gamma, acc[k]));
+  for (int k = 0; k < 12; ++k) v[k] = {fmar(gamma, acc[k].re, v[k].re), fmar(gamma, acc[k].im, v[k].im)};
   SpinorIO<S>::store(out, s, g.V, v);
 }
 
@@ -139,6 +139,7 @@ __device__ __forceinline__ void hop_staged(const Geo& g, long long s, const Site
 // algorithmic HBM bytes against 1.48x for k_dslash, 6% faster).  W is the number of
 // waves per SIMD the compiler schedules for (amdgpu_waves_per_eu); at about 104
 // VGPRs 4 waves are resident either way.
+// (Nontemporal link and clover loads were measured slower here, with more L2 misses.)
 template <int NL, int W>
 __global__ void __launch_bounds__(HOP_THREADS) __attribute__((amdgpu_waves_per_eu(1, W)))
 k_dslash_lds(Geo g, const int* __restrict__ order, const int* __restrict__ active, float gamma, float* __restrict__ out,

@@ -20,6 +20,8 @@
 // Every kernel is written for 64-wide wavefronts; one thread owns one site.
 #pragma once
 
+#include <type_traits>
+
 #include <hip/hip_fp16.h>
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -121,12 +123,17 @@ template <typename R> struct Cx { R re, im; };
 
 template <typename R> __device__ __forceinline__ Cx<R> cadd(Cx<R> a, Cx<R> b) { return {a.re + b.re, a.im + b.im}; }
 template <typename R> __device__ __forceinline__ Cx<R> csub(Cx<R> a, Cx<R> b) { return {a.re - b.re, a.im - b.im}; }
+// a b and conj(a) b with the fused multiply-add written out, so that every kernel
+// rounds them the same way (left to the compiler, which product it fuses depends on
+// the surrounding code)
+__device__ __forceinline__ float fmar(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ __forceinline__ double fmar(double a, double b, double c) { return __builtin_fma(a, b, c); }
 template <typename R> __device__ __forceinline__ Cx<R> cmul(Cx<R> a, Cx<R> b) {
-  return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+  return {fmar(a.re, b.re, -(a.im * b.im)), fmar(a.re, b.im, a.im * b.re)};
 }
 // conj(a) * b
 template <typename R> __device__ __forceinline__ Cx<R> cmulc(Cx<R> a, Cx<R> b) {
-  return {a.re * b.re + a.im * b.im, a.re * b.im - a.im * b.re};
+  return {fmar(a.re, b.re, a.im * b.im), fmar(a.re, b.im, -(a.im * b.re))};
 }
 template <typename R> __device__ __forceinline__ Cx<R> cconj(Cx<R> a) { return {a.re, -a.im}; }
 template <typename R> __device__ __forceinline__ Cx<R> cimul(Cx<R> a) { return {-a.im, a.re}; }   // i*a
@@ -136,8 +143,6 @@ template <typename R> __device__ __forceinline__ Cx<R> cscale(R s, Cx<R> a) { re
 // Complex multiply-accumulate as chains of fused multiply-adds (4 FMAs; written as
 // c + a*b through cmul and cadd it costs 2 multiplies, 2 FMAs and 2 adds, since the
 // compiler may not reassociate the sum).
-__device__ __forceinline__ float fmar(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
-__device__ __forceinline__ double fmar(double a, double b, double c) { return __builtin_fma(a, b, c); }
 // c + a b
 template <typename R> __device__ __forceinline__ Cx<R> cfma(Cx<R> a, Cx<R> b, Cx<R> c) {
   return {fmar(-a.im, b.im, fmar(a.re, b.re, c.re)), fmar(a.im, b.re, fmar(a.re, b.im, c.im))};
@@ -336,8 +341,9 @@ __device__ __forceinline__ void reconstruct(const Cx<R> b[6], R tf, Cx<R> o[12])
       if (DIR > 0) { s[2] = cadd(s[2], cimul(b0)); s[3] = csub(s[3], cimul(b1)); }
       else         { s[2] = csub(s[2], cimul(b0)); s[3] = cadd(s[3], cimul(b1)); }
     } else {
-      if (DIR > 0) { s[2] = cadd(s[2], cscale(tf, b0)); s[3] = cadd(s[3], cscale(tf, b1)); }
-      else         { s[0] = cadd(s[0], cscale(tf, b0)); s[1] = cadd(s[1], cscale(tf, b1)); }
+      auto ax = [tf](Cx<R> y, Cx<R> b) { return Cx<R>{fmar(tf, b.re, y.re), fmar(tf, b.im, y.im)}; };
+      if (DIR > 0) { s[2] = ax(s[2], b0); s[3] = ax(s[3], b1); }
+      else         { s[0] = ax(s[0], b0); s[1] = ax(s[1], b1); }
     }
   }
 }

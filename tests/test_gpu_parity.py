@@ -90,8 +90,8 @@ def test_dslash_fp64_fp32(prob, hops, colour):
 @pytest.mark.parametrize("su3", [True, False])
 def test_dslash_kernel_variants(su3):
     """The FP32 full-operator kernels on 256-site blocks (global gathers, LDS-staged
-    for 3 or 4 waves) agree with the oracle and with each other bit for bit (same
-    hop order and arithmetic); the autotuner keeps one of them."""
+    for 3 or 4 waves) agree with the oracle and with each other bit for bit (same hop
+    order and arithmetic); the autotuner keeps one of them."""
     p = Problem([16, 8, 8, 8], [4, 4, 4, 4], kappa=0.125, seed=12, su3=su3)
     src = p.b.astype(np.float32)
     inp = p.lat.spinor(Q.F32, src)
@@ -103,7 +103,7 @@ def test_dslash_kernel_variants(su3):
         p.lat.dslash(out, inp, Q.HOPS_ALL, -1)
         outs[k] = out.download()
         assert rel(outs[k], want) < TOL32, k
-    assert np.array_equal(outs[0], outs[3]) and np.array_equal(outs[0], outs[4])
+    assert all(np.array_equal(outs[0], outs[k]) for k in (3, 4))
     assert p.lat.autotune_dslash(3) in (0, 3, 4)
     with pytest.raises(Q.QwsError):
         p.lat.set_dslash_kernel(7)
