@@ -58,7 +58,9 @@ void qws_init_(int* lx, int* ly, int* lz, int* lt, int* npe_f, int* fbc_f, int* 
 /* qws.cc:319 */
 void qws_finalize_(void);
 
-/* qws_bqcd.cc:125 -- BQCD-ordered links (needs BQCD's std_xyzt2i_/e_o_); sets kappa */
+/* qws_bqcd.cc:125 -- BQCD-ordered links; sets kappa.  BQCD's std_xyzt2i_ and e_o_
+ * (site index within its parity, 1-based, and the parity) must be linked or loaded
+ * globally before any qws_load / qws_store call. */
 void qws_loadg_dd_(double* u, int* volh_tot, double* in_kappa);
 /* qws_bqcd.cc:284 -- BQCD-ordered clover inverse (72 reals per site) */
 void qws_loadc_dd_(double* c, int* volh);
@@ -67,6 +69,15 @@ void qws_loadfd_dd_(scd_t* out, double* ine, double* ino);
 void qws_storefd_dd_(double* oute, double* outo, scd_t* in);
 void qws_loadfs_dd_(scs_t* out, double* ine, double* ino);
 void qws_storefs_dd_(double* oute, double* outo, scs_t* in);
+/* qws_bqcd.cc:66,184,384,413,442,471 -- the same BQCD arrays into the even/odd reading
+ * of the host arrays (x-compressed sites, parity slot = e_o_): links and clover inverse
+ * of both parities; spinors of the even sites only (a field of vold / vols vectors). */
+void qws_loadg_bqcd_(double* u, int* volh_tot, double* in_kappa);
+void qws_loadc_bqcd_(double* c, int* volh);
+void qws_loadfd_bqcd_(scd_t* out, double* in);
+void qws_loadfs_bqcd_(scs_t* out, double* in);
+void qws_storefd_bqcd_(double* out, scd_t* in);
+void qws_storefs_bqcd_(double* out, scs_t* in);
 
 /* bicgstab_dd_mix.cc:105 -- FP64 outer BiCGStab with FP32 SAP-preconditioned inner solve.
  * x: initial guess in, solution out.  tol < 0: run maxiter-1 outer iterations. */
@@ -85,6 +96,8 @@ void prec_s_(scs_t* out, scs_t* in, int* nsap, int* nm);
  * FP32 D; bicgstab_precdd_s_h_ is the inner BiCGStab whose x accumulates M_SAP p
  * directly; bicgstab_dd_mix2_hf_ is the FP64 outer solve on top of it. */
 void qws_h_init_(const glus_t* u, const clvs_t* c);
+/* qws_bqcd.cc:647 -- qws_h_init_(glus, clvs) */
+void bqcd_qws_h_init_(void);
 void assign_mult_mSAP_s_h_(scs_t* xs, const scs_t* bs, const int* nsap, const int* nm);
 void assign_mult_wd_s_(scs_t* x, const scs_t* b);
 void bicgstab_precdd_s_h_(scs_t* x, const scs_t* b, const double* tol, int* conviter, const int* maxiter,

@@ -2,6 +2,8 @@
 // interface and the QWS drop-in entry points with their host mirrors.
 #include <hip/hip_runtime.h>
 
+#include <dlfcn.h>
+
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -1032,13 +1034,34 @@ void fermi_reorder_s2d_dd_renorm_(scd_t* out, const scs_t* in, const double* dno
         }
 }
 
-// ---- BQCD loaders (qws_bqcd.cc:125-642); site order and parity come from BQCD.
-static void need_bqcd(const char* where) {
-  if (!std_xyzt2i_ || !e_o_) fatal(where, "BQCD's std_xyzt2i_/e_o_ are not linked");
+}  // extern "C"
+
+// ---- BQCD loaders (qws_bqcd.cc:66-647).  BQCD numbers the sites of each parity
+// itself: std_xyzt2i_ (1-based index within the parity) and e_o_ (the parity) come
+// from the caller.  They are found as weak references at load time, or else in the
+// process's global symbol scope at call time (a caller that loads them later).
+namespace {
+
+using bqcd_site_fn = int (*)(int*);
+
+template <class F>
+void for_each_bqcd_site(const char* where, F&& f) {
+  bqcd_site_fn idx = std_xyzt2i_ ? std_xyzt2i_ : (bqcd_site_fn)dlsym(RTLD_DEFAULT, "std_xyzt2i_");
+  bqcd_site_fn par = e_o_ ? e_o_ : (bqcd_site_fn)dlsym(RTLD_DEFAULT, "e_o_");
+  if (!idx || !par) fatal(where, "BQCD's std_xyzt2i_/e_o_ are not linked");
+  for (int t = 0; t < nt; ++t)
+    for (int z = 0; z < nz; ++z)
+      for (int y = 0; y < ny; ++y)
+        for (int x = 0; x < nx; ++x) {
+          int j[4] = {x, y, z, t};
+          const int i = idx(j);
+          const int eo = par(j);
+          f(x, y, z, t, i, eo);
+        }
 }
 
 // QWS clover index k <- BQCD index (qws_bqcd.cc:301-337)
-static int bqcd_clover_index(int k) {
+int bqcd_clover_index(int k) {
   if (k < 2) return k;
   if (k < 4) return 20 + (k - 2);
   if (k < 6) return 32 + (k - 4);
@@ -1047,104 +1070,123 @@ static int bqcd_clover_index(int k) {
   return k;
 }
 
-void qws_loadg_dd_(double* u, int* volh_tot, double* in_kappa) {
-  need_bqcd(__func__);
-  kappa = *in_kappa;
-  mkappa = -kappa;
-  kappa2 = kappa * kappa;
-  for (int t = 0; t < nt; ++t)
-    for (int z = 0; z < nz; ++z)
-      for (int y = 0; y < ny; ++y)
-        for (int x = 0; x < nx; ++x) {
-          int j[4] = {x, y, z, t};
-          const int i = std_xyzt2i_(j);
-          const int eo = e_o_(j);
-          const long ihh = (long)(i - 1) * 18 + (long)eo * (*volh_tot) * 18;
-          const int iwsd = (x % nxh) / QWS_VLEND + nxd * y + nxd * ny * z + nxd * ny * nz * t + 4 * vold * (x / nxh);
-          const int ixxd = (x % nxh) % QWS_VLEND;
-          const int iwss = (x % nxh) / QWS_VLENS + nxs * y + nxs * ny * z + nxs * ny * nz * t + 4 * vols * (x / nxh);
-          const int ixxs = (x % nxh) % QWS_VLENS;
-          for (int mu = 0; mu < 4; ++mu)
-            for (int c1 = 0; c1 < 3; ++c1)
-              for (int c2 = 0; c2 < 3; ++c2)
-                for (int ri = 0; ri < 2; ++ri) {
-                  const double v = u[ri + 2 * c1 + 6 * c2 + ihh + (long)mu * (*volh_tot) * 36];
-                  glud[iwsd + vold * mu].c[c2][c1][ri][ixxd] = v;
-                  glus[iwss + vols * mu].c[c2][c1][ri][ixxs] = (float)v;
-                }
-        }
+void set_kappa(double k) {
+  kappa = k;
+  mkappa = -k;
+  kappa2 = k * k;
+}
+
+// Host AoSoA position of a site: vector index without the domain/parity offset, and
+// lane.  The DD layout splits x into two domains (xx = x % nxh, slot = x / nxh); the
+// even/odd layout compresses x (xx = x / 2, slot = parity) (qws_bqcd.cc:87-90,144-147).
+struct HostPos {
+  int iv, lane;
+};
+HostPos host_pos(int xx, int y, int z, int t, int vlen) {
+  const int nxv = nxh / vlen;
+  return {xx / vlen + nxv * (y + ny * (z + nz * t)), xx % vlen};
+}
+
+// Links: u[ri + 2*col1 + 6*col2 + 18*(i-1) + 18*volh_tot*eo + 36*volh_tot*mu] ->
+// glu[iv + vol*mu + 4*vol*slot].c[col2][col1][ri] (qws_bqcd.cc:86-112,140-151).
+void load_bqcd_links(const char* where, const double* u, long volh_tot, bool eo_layout) {
+  for_each_bqcd_site(where, [&](int x, int y, int z, int t, int i, int eo) {
+    const long ihh = (long)(i - 1) * 18 + (long)eo * volh_tot * 18;
+    const int xx = eo_layout ? x / 2 : x % nxh;
+    const int slot = eo_layout ? eo : x / nxh;
+    const HostPos d = host_pos(xx, y, z, t, QWS_VLEND), s = host_pos(xx, y, z, t, QWS_VLENS);
+    for (int mu = 0; mu < 4; ++mu)
+      for (int c1 = 0; c1 < 3; ++c1)
+        for (int c2 = 0; c2 < 3; ++c2)
+          for (int ri = 0; ri < 2; ++ri) {
+            const double v = u[ri + 2 * c1 + 6 * c2 + ihh + (long)mu * volh_tot * 36];
+            glud[d.iv + vold * mu + 4 * vold * slot].c[c2][c1][ri][d.lane] = v;
+            glus[s.iv + vols * mu + 4 * vols * slot].c[c2][c1][ri][s.lane] = (float)v;
+          }
+  });
   g_qws.dirty = true;
   g_qws.eo_dirty = true;
 }
 
-void qws_loadc_dd_(double* c, int* volh) {
-  need_bqcd(__func__);
-  for (int t = 0; t < nt; ++t)
-    for (int z = 0; z < nz; ++z)
-      for (int y = 0; y < ny; ++y)
-        for (int x = 0; x < nx; ++x) {
-          int j[4] = {x, y, z, t};
-          const int i = std_xyzt2i_(j);
-          const int eo = e_o_(j);
-          const long ihh = (long)(i - 1) * 72 + (long)eo * (*volh) * 72;
-          const int iwsd = (x % nxh) / QWS_VLEND + nxd * y + nxd * ny * z + nxd * ny * nz * t + vold * (x / nxh);
-          const int ixxd = (x % nxh) % QWS_VLEND;
-          const int iwss = (x % nxh) / QWS_VLENS + nxs * y + nxs * ny * z + nxs * ny * nz * t + vols * (x / nxh);
-          const int ixxs = (x % nxh) % QWS_VLENS;
-          for (int ud = 0; ud < 2; ++ud)
-            for (int k = 0; k < 36; ++k) {
-              const double v = c[bqcd_clover_index(k) + ud * 36 + ihh];
-              clvd[iwsd].c[ud][k][ixxd] = v;
-              clvs[iwss].c[ud][k][ixxs] = (float)v;
-            }
-        }
+// Clover inverse: 72 reals per site, two chiralities of 36 in BQCD's order, permuted
+// into QWS's (qws_bqcd.cc:203-265,301-337).
+void load_bqcd_clover(const char* where, const double* c, long volh, bool eo_layout) {
+  for_each_bqcd_site(where, [&](int x, int y, int z, int t, int i, int eo) {
+    const long ihh = (long)(i - 1) * 72 + (long)eo * volh * 72;
+    const int xx = eo_layout ? x / 2 : x % nxh;
+    const int slot = eo_layout ? eo : x / nxh;
+    const HostPos d = host_pos(xx, y, z, t, QWS_VLEND), s = host_pos(xx, y, z, t, QWS_VLENS);
+    for (int ud = 0; ud < 2; ++ud)
+      for (int k = 0; k < 36; ++k) {
+        const double v = c[bqcd_clover_index(k) + ud * 36 + ihh];
+        clvd[d.iv + vold * slot].c[ud][k][d.lane] = v;
+        clvs[s.iv + vols * slot].c[ud][k][s.lane] = (float)v;
+      }
+  });
   g_qws.dirty = true;
   g_qws.eo_dirty = true;
 }
 
-}  // extern "C"
-
+// Spinors: h[ri + 2*s + 8*c + 24*(i-1)].  The DD form takes both parities (ine, ino)
+// into the full field (qws_bqcd.cc:500-642); the even/odd form moves the even sites
+// only, into one parity field of x-compressed sites (qws_bqcd.cc:384-498).
 template <typename T, int VL>
-static void bqcd_spinor(T* dd, double* ine, double* ino, bool load) {
-  for (int t = 0; t < nt; ++t)
-    for (int z = 0; z < nz; ++z)
-      for (int y = 0; y < ny; ++y)
-        for (int x = 0; x < nx; ++x) {
-          int j[4] = {x, y, z, t};
-          const int i = std_xyzt2i_(j);
-          const int eo = e_o_(j);
-          const long ihh = (long)(i - 1) * 24;
-          const int nxv = nxh / VL;
-          const int vv = nxv * ny * nz * nt;
-          const int iws = (x % nxh) / VL + nxv * y + nxv * ny * z + nxv * ny * nz * t + vv * (x / nxh);
-          const int ixx = (x % nxh) % VL;
-          double* h = eo == 0 ? ine : ino;
-          for (int c = 0; c < 3; ++c)
-            for (int s = 0; s < 4; ++s)
-              for (int ri = 0; ri < 2; ++ri) {
-                if (load) dd[iws].c[c][s][ri][ixx] = h[ri + 2 * s + 8 * c + ihh];
-                else h[ri + 2 * s + 8 * c + ihh] = (double)dd[iws].c[c][s][ri][ixx];
-              }
+void bqcd_spinor(const char* where, T* dd, double* he, double* ho, bool load, bool eo_layout) {
+  for_each_bqcd_site(where, [&](int x, int y, int z, int t, int i, int eo) {
+    if (eo_layout && eo != 0) return;
+    const long ihh = (long)(i - 1) * 24;
+    const int nxv = nxh / VL;
+    const HostPos p = host_pos(eo_layout ? x / 2 : x % nxh, y, z, t, VL);
+    const int iws = p.iv + (eo_layout ? 0 : nxv * ny * nz * nt * (x / nxh));
+    double* h = eo == 0 ? he : ho;
+    for (int c = 0; c < 3; ++c)
+      for (int s = 0; s < 4; ++s)
+        for (int ri = 0; ri < 2; ++ri) {
+          if (load) dd[iws].c[c][s][ri][p.lane] = h[ri + 2 * s + 8 * c + ihh];
+          else h[ri + 2 * s + 8 * c + ihh] = (double)dd[iws].c[c][s][ri][p.lane];
         }
+  });
 }
+
+}  // namespace
 
 extern "C" {
 
+void qws_loadg_dd_(double* u, int* volh_tot, double* in_kappa) {
+  set_kappa(*in_kappa);
+  load_bqcd_links(__func__, u, *volh_tot, false);
+}
+void qws_loadc_dd_(double* c, int* volh) { load_bqcd_clover(__func__, c, *volh, false); }
 void qws_loadfd_dd_(scd_t* out, double* ine, double* ino) {
-  need_bqcd(__func__);
-  bqcd_spinor<scd_t, QWS_VLEND>(out, ine, ino, true);
+  bqcd_spinor<scd_t, QWS_VLEND>(__func__, out, ine, ino, true, false);
 }
 void qws_storefd_dd_(double* oute, double* outo, scd_t* in) {
-  need_bqcd(__func__);
-  bqcd_spinor<scd_t, QWS_VLEND>(in, oute, outo, false);
+  bqcd_spinor<scd_t, QWS_VLEND>(__func__, in, oute, outo, false, false);
 }
 void qws_loadfs_dd_(scs_t* out, double* ine, double* ino) {
-  need_bqcd(__func__);
-  bqcd_spinor<scs_t, QWS_VLENS>(out, ine, ino, true);
+  bqcd_spinor<scs_t, QWS_VLENS>(__func__, out, ine, ino, true, false);
 }
 void qws_storefs_dd_(double* oute, double* outo, scs_t* in) {
-  need_bqcd(__func__);
-  bqcd_spinor<scs_t, QWS_VLENS>(in, oute, outo, false);
+  bqcd_spinor<scs_t, QWS_VLENS>(__func__, in, oute, outo, false, false);
 }
+
+void qws_loadg_bqcd_(double* u, int* volh_tot, double* in_kappa) {
+  set_kappa(*in_kappa);
+  load_bqcd_links(__func__, u, *volh_tot, true);
+}
+void qws_loadc_bqcd_(double* c, int* volh) { load_bqcd_clover(__func__, c, *volh, true); }
+void qws_loadfd_bqcd_(scd_t* out, double* in) {
+  bqcd_spinor<scd_t, QWS_VLEND>(__func__, out, in, nullptr, true, true);
+}
+void qws_loadfs_bqcd_(scs_t* out, double* in) {
+  bqcd_spinor<scs_t, QWS_VLENS>(__func__, out, in, nullptr, true, true);
+}
+void qws_storefd_bqcd_(double* out, scd_t* in) {
+  bqcd_spinor<scd_t, QWS_VLEND>(__func__, in, out, nullptr, false, true);
+}
+void qws_storefs_bqcd_(double* out, scs_t* in) {
+  bqcd_spinor<scs_t, QWS_VLENS>(__func__, in, out, nullptr, false, true);
+}
+void bqcd_qws_h_init_(void) { qws_h_init_(glus, clvs); }
 
 }  // extern "C"

@@ -12,6 +12,7 @@ The oracle is pinned three ways:
 import numpy as np
 import pytest
 
+from oracle import bqcd as BQ
 from oracle import oracle as O
 from oracle import ref_worker as RW
 from tests import casekit as K
@@ -445,3 +446,37 @@ def test_eo_solvers_match_reference_library():
         assert rel(res["mcg_vm_oracle"][k], res["mcg_vm"][k]) < 1e-12, k
     assert rel(res["mcg_vm"][0], res["cg_vm"]) < 1e-12
     assert np.isnan(res["mcg_vm"][3]).any() and np.isnan(res["mcg_vm_oracle"][3]).any()
+
+
+def test_bqcd_layouts():
+    """oracle/bqcd.py against the loaders' own statements.  The clover is packed from
+    the dense Hermitian blocks in row pairs; QWS's loader takes BQCD's packing apart
+    with an index table (qws_bqcd.cc:301-337): QWS[0,1] <- B[0,1], QWS[2,3] <-
+    B[20,21], QWS[4,5] <- B[32,33], QWS[6..23] <- B[2..19], QWS[24..33] <- B[22..31],
+    QWS[34,35] <- B[34,35]; both must give the same numbers.  Links follow
+    u[ri + 2 col1 + 6 col2 + 18 (i-1) + 18 volh eo + 36 volh mu] -> c[col2][col1][ri]
+    (qws_bqcd.cc:140-151), spinors h[ri + 2 s + 8 c + 24 (i-1)] (qws_bqcd.cc:512-528);
+    the site numbering is a bijection onto 1..volh within each parity."""
+    nloc = [4, 2, 2, 6]
+    V = int(np.prod(nloc))
+    rng = np.random.default_rng(3)
+    i, eo = BQ.site_numbers(nloc)
+    for p in (0, 1):
+        assert sorted(i[eo == p]) == list(range(1, V // 2 + 1))
+    C = rng.standard_normal((V, 2, 36))
+    B = BQ.clover(C, nloc).reshape(2, V // 2, 2, 36)[eo, i - 1]
+    perm = [0, 1, 20, 21, 32, 33] + list(range(2, 20)) + list(range(22, 32)) + [34, 35]
+    assert np.array_equal(B[..., perm], C)
+    U = rng.standard_normal((4, V, 3, 3, 2))
+    u = BQ.links(U, nloc)
+    for _ in range(50):
+        mu, s, c1, c2, ri = (int(rng.integers(n)) for n in (4, V, 3, 3, 2))
+        k = ri + 2 * c1 + 6 * c2 + 18 * (i[s] - 1) + 18 * (V // 2) * eo[s] + 36 * (V // 2) * mu
+        assert u[k] == U[mu, s, c2, c1, ri]
+    psi = rng.standard_normal((V, 3, 4, 2))
+    he, ho = BQ.spinor(psi, nloc)
+    for _ in range(50):
+        s, c, sp, ri = (int(rng.integers(n)) for n in (V, 3, 4, 2))
+        h = he if eo[s] == 0 else ho
+        assert h[ri + 2 * sp + 8 * c + 24 * (i[s] - 1)] == psi[s, c, sp, ri]
+    assert np.array_equal(BQ.spinor_from(he, ho, nloc), psi)
