@@ -83,6 +83,14 @@ void qws_storefs_bqcd_(double* out, scs_t* in);
  * x: initial guess in, solution out.  tol < 0: run maxiter-1 outer iterations. */
 void bicgstab_dd_mix_(scd_t* x, scd_t* b, double* tol, int* conviter, int* maxiter, double* tol_s, int* maxiter_s,
                       int* nsap, int* nm);
+/* Not in the reference: bicgstab_dd_mix_ for callers that need to know the outcome.
+ * Same arguments and results, plus true_residual = |b - D x| / |b| recomputed in FP64 on
+ * the GPU (-1 if the solve did not run).  Returns 0 when the recursive residual met tol
+ * (or tol < 0), 1 when maxiter ran out, 2 when the residual is not finite (breakdown),
+ * -1 before qws_init_, -2 for invalid arguments, -3 for a device or runtime failure;
+ * errors are reported on stderr and never end the process. */
+int bicgstab_dd_mix_status_(scd_t* x, scd_t* b, double* tol, int* conviter, int* maxiter, double* tol_s,
+                            int* maxiter_s, int* nsap, int* nm, double* true_residual);
 /* bicgstab_precdd_s.cc:85 -- FP32 right-preconditioned BiCGStab on D M_SAP, x0 = 0 */
 void bicgstab_precdd_s_(scs_t* x, scs_t* b, double* tol, int* conviter, int* maxiter, int* nsap, int* nm);
 /* qws.cc:776 -- out = D M_SAP in */

@@ -752,10 +752,12 @@ void bicgstab_precdd_s_(scs_t* x, scs_t* b, double* tol, int* conviter, int* max
   });
 }
 
-static void dd_mix_generic(const char* where, scd_t* x, const scd_t* b, const double* tol, int* conviter,
-                           const int* maxiter, const double* tol_s, const int* maxiter_s, const int* nsap, const int* nm,
-                           int sap_prec) {
-  drop_in(where, [&] {
+// The outer solve on the host arrays; returns |b - D x| / |b| (FP64, on the GPU) when
+// true_res is set.  Throws on any failure; the callers decide whether that ends the process.
+static void dd_mix_body(const char* where, scd_t* x, const scd_t* b, const double* tol, int* conviter,
+                        const int* maxiter, const double* tol_s, const int* maxiter_s, const int* nsap, const int* nm,
+                        int sap_prec, double* true_res) {
+    if (!g_qws.lat) throw std::runtime_error("qws_init_ has not been called");
     Lattice& L = lat_sync(where);
     double* xx = (double*)dbuf(0, qws::P64);
     double* bb = (double*)dbuf(1, qws::P64);
@@ -774,18 +776,55 @@ static void dd_mix_generic(const char* where, scd_t* x, const scd_t* b, const do
     if (sap_prec == qws::P16) p.maxiter = *maxiter + 1;
     int inner = 0;
     int it = L.bicgstab_dd_mix(xx, bb, p, &inner);
+    if (true_res) *true_res = L.true_residual(xx, bb);
     L.download_spinor(qws::P64, xx, x, 1, 1);
     if (sap_prec != qws::P16) {
       if (conviter) *conviter = it;
     } else if (it < p.maxiter && conviter) {
       *conviter = it - 1;
     }
-  });
+}
+
+static void dd_mix_generic(const char* where, scd_t* x, const scd_t* b, const double* tol, int* conviter,
+                           const int* maxiter, const double* tol_s, const int* maxiter_s, const int* nsap, const int* nm,
+                           int sap_prec) {
+  drop_in(where, [&] { dd_mix_body(where, x, b, tol, conviter, maxiter, tol_s, maxiter_s, nsap, nm, sap_prec, nullptr); });
 }
 
 void bicgstab_dd_mix_(scd_t* x, scd_t* b, double* tol, int* conviter, int* maxiter, double* tol_s, int* maxiter_s,
                       int* nsap, int* nm) {
   dd_mix_generic("bicgstab_dd_mix_", x, b, tol, conviter, maxiter, tol_s, maxiter_s, nsap, nm, qws::P32);
+}
+
+int bicgstab_dd_mix_status_(scd_t* x, scd_t* b, double* tol, int* conviter, int* maxiter, double* tol_s,
+                            int* maxiter_s, int* nsap, int* nm, double* true_residual) {
+  if (true_residual) *true_residual = -1;
+  if (!g_qws.lat) {
+    std::fprintf(stderr, "ERROR at %s : qws_init_ has not been called\n", __func__);
+    return -1;
+  }
+  if (!x || !b || !tol || !maxiter || !tol_s || !maxiter_s || !nsap || !nm) {
+    std::fprintf(stderr, "ERROR at %s : null argument\n", __func__);
+    return -2;
+  }
+  if (*nsap < 1 || *nsap > 10 || *nm < 1 || *nm > 10) {
+    std::fprintf(stderr, "ERROR at %s : nsap and nm must be in 1..10\n", __func__);
+    return -2;
+  }
+  double res = -1;
+  int it = 0;
+  try {
+    dd_mix_body(__func__, x, b, tol, &it, maxiter, tol_s, maxiter_s, nsap, nm, qws::P32, &res);
+  } catch (const std::exception& e) {
+    std::fprintf(stderr, "ERROR at %s : %s\n", __func__, e.what());
+    return -3;
+  }
+  if (conviter) *conviter = it;
+  if (true_residual) *true_residual = res;
+  if (!std::isfinite(res)) return 2;
+  // the recursive residual met tol within the iterations run (bicgstab_dd_mix.cc:249,336);
+  // a solve that ran out reports conviter = maxiter
+  return (*tol < 0 || it < *maxiter) ? 0 : 1;
 }
 
 // ---- half-precision variant (qws_h.cc, prec_ddd_s_h.h, bicgstab_precdd_s_h.cc, bicgstab_dd_mix2_hf.cc)

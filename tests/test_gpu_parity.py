@@ -343,4 +343,23 @@ def test_dropin_abi_case0():
     L.ddd_d_(_ptr(r), _ptr(x))
     assert O.norm2(bd - r) < 1e-15
     assert K.check_value("xnorm^2", O.norm2(x), K.expected("CASE0", "xnorm^2", "DD"))
+    # the status variant: same solution bit for bit, plus the outcome and the true residual
+    L.bicgstab_dd_mix_status_.restype = ctypes.c_int
+    xs = RW.spinor_to_aosoa(x0, nloc, 8)
+    its, tr = ctypes.c_int(0), ctypes.c_double(0)
+    rc = L.bicgstab_dd_mix_status_(_ptr(xs), _ptr(bd), cd(-1.0), ctypes.byref(its), ci(6), cd(-1.0), ci(50), ci(4),
+                                   ci(2), ctypes.byref(tr))
+    assert rc == 0 and its.value == 6 and np.array_equal(xs, x)
+    assert 0 <= tr.value < 1e-9 and abs(tr.value - np.sqrt(O.norm2(bd - r) / O.norm2(bd))) < 1e-12
+    # a tolerance 2 outer iterations do not meet: 1, conviter = maxiter
+    xs = RW.spinor_to_aosoa(x0, nloc, 8)
+    rc = L.bicgstab_dd_mix_status_(_ptr(xs), _ptr(bd), cd(1e-14), ctypes.byref(its), ci(2), cd(1e-6), ci(50), ci(4),
+                                   ci(2), ctypes.byref(tr))
+    assert rc == 1 and its.value == 2 and tr.value > 1e-14
+    rc = L.bicgstab_dd_mix_status_(_ptr(xs), _ptr(bd), cd(1e-14), ctypes.byref(its), ci(2), cd(1e-6), ci(50), ci(11),
+                                   ci(2), ctypes.byref(tr))
+    assert rc == -2 and tr.value == -1
     L.qws_finalize_()
+    rc = L.bicgstab_dd_mix_status_(_ptr(xs), _ptr(bd), cd(1e-14), ctypes.byref(its), ci(2), cd(1e-6), ci(50), ci(4),
+                                   ci(2), ctypes.byref(tr))
+    assert rc == -1
