@@ -244,6 +244,11 @@ int qwsg_set_sap_fused(qwsg_lattice* lat, int on);
  * spinors of the other colour); 0 = it gathers the other colour's spinors and links.
  * Same results bit for bit.  No counterpart in the reference. */
 int qwsg_set_sap_face_buffer(qwsg_lattice* lat, int on);
+/* FP32 inner BiCGStab with the fused SAP: 1 (default) = its direction update
+ * p = r + beta (p - omega q) runs inside the first SAP sweep of the next preconditioner
+ * application; 0 = a separate pass.  Same results bit for bit.  No counterpart in the
+ * reference. */
+int qwsg_set_sap_source_update(qwsg_lattice* lat, int on);
 int qwsg_volume(qwsg_lattice* lat, long long* V);
 /* host plain layout: links [4][V][3][3][2] (QWS c[i][j] of the link x -> x+mu),
  * clover inverse [V][2][36] (QWS packing), site x + Lx*(y + Ly*(z + Lz*t)) */

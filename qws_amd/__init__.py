@@ -50,6 +50,7 @@ def lib():
         L.qwsg_set_kappa.argtypes = [vp, ctypes.c_double]
         L.qwsg_set_sap_fused.argtypes = [vp, ctypes.c_int]
         L.qwsg_set_sap_face_buffer.argtypes = [vp, ctypes.c_int]
+        L.qwsg_set_sap_source_update.argtypes = [vp, ctypes.c_int]
         L.qwsg_volume.argtypes = [vp, ctypes.POINTER(ctypes.c_longlong)]
         for n in ("qwsg_load_gauge", "qwsg_load_clover", "qwsg_load_gauge_f32", "qwsg_load_clover_f32"):
             getattr(L, n).argtypes = [vp, vp]
@@ -281,6 +282,10 @@ class Lattice:
     def set_sap_face_buffer(self, on):
         """FP32 fused SAP face updates through the face buffer (True) or by gathers."""
         check(lib().qwsg_set_sap_face_buffer(self.h, int(bool(on))))
+
+    def set_sap_source_update(self, on):
+        """Inner BiCGStab direction update inside the first SAP sweep (True) or a separate pass."""
+        check(lib().qwsg_set_sap_source_update(self.h, int(bool(on))))
 
     def set_kappa(self, k):
         self.kappa = float(k)

@@ -72,6 +72,9 @@ int qwsg_set_sap_fused(qwsg_lattice* h, int on) {
 int qwsg_set_sap_face_buffer(qwsg_lattice* h, int on) {
   return guarded([&] { h->lat->set_sap_face_buffer(on != 0); });
 }
+int qwsg_set_sap_source_update(qwsg_lattice* h, int on) {
+  return guarded([&] { h->lat->set_sap_source_update(on != 0); });
+}
 int qwsg_set_kappa(qwsg_lattice* h, double k) {
   return guarded([&] { h->lat->set_kappa(k); });
 }

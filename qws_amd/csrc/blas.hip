@@ -181,11 +181,9 @@ k_update_p(long long n, long long stride, const int* active, R* p, const R* r, c
   T* P = RHS(p, T);
   const T *Qv = RHS(q, const T), *Rr = RHS(r, const T);
   GRID_STRIDE(i, n) {
-    T pv = P[i], qv = Qv[i], rv = Rr[i];
-    R cr = pv.x - ar * qv.x + ai * qv.y;
-    R ci = pv.y - ar * qv.y - ai * qv.x;
-    pv.x = br * cr - bi * ci + rv.x;
-    pv.y = br * ci + bi * cr + rv.y;
+    T pv = P[i];
+    const T qv = Qv[i], rv = Rr[i];
+    bicg_p_update<R>(pv.x, pv.y, qv.x, qv.y, rv.x, rv.y, ar, ai, br, bi);
     P[i] = pv;
   }
 }

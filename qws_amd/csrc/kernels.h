@@ -34,6 +34,17 @@ struct Scalars {
   double red[4];
 };
 
+// BiCGStab's direction update of one complex component, p = r + beta (p - omega q)
+// (bicgstab_precdd_s.cc:518-545), with the multiply-adds spelled out so that every
+// kernel that applies it rounds alike.
+template <typename R>
+__device__ __forceinline__ void bicg_p_update(R& px, R& py, R qx, R qy, R rx, R ry, R wr, R wi, R br, R bi) {
+  const R cr = fmar(wi, qy, fmar(-wr, qx, px));
+  const R ci = fmar(-wi, qx, fmar(-wr, qy, py));
+  px = fmar(br, cr, fmar(-bi, ci, rx));
+  py = fmar(br, ci, fmar(bi, cr, ry));
+}
+
 constexpr int RED_BLOCKS = 1024;  // partial sums per reduction (fixed => deterministic)
 constexpr int RED_THREADS = 256;
 

@@ -64,6 +64,9 @@ Lattice::Lattice(const int L[4], const int B[4], int tbc, int links32) {
   // SIMD, unless QWS_MI355X_DSLASH names another (0, 3, 4); autotune_dslash() times them
   const char* fbv = std::getenv("QWS_MI355X_SAP_FB");
   use_fb_ = !(fbv && fbv[0] == '0');
+  // the inner BiCGStab's direction update inside the first SAP sweep unless QWS_MI355X_SAP_PUPD=0
+  const char* pu = std::getenv("QWS_MI355X_SAP_PUPD");
+  fold_pupd_ = !(pu && pu[0] == '0');
   const char* dk = std::getenv("QWS_MI355X_DSLASH");
   g_.dslash_kernel = dk ? std::atoi(dk) : 4;
   if (g_.dslash_kernel != 0 && g_.dslash_kernel != 3) g_.dslash_kernel = 4;
@@ -413,15 +416,20 @@ template <typename S> bool Lattice::fused_ok() const {
 // the nsap sweeps of qws.cc:809-841 with the face update of each half-sweep
 // done at the start of the next one
 template <typename S>
-void Lattice::sap_sweeps_fused(const Batch& B, const S* in, S* s, S* q, int nsap, int nm, float* fb, long long fbs) {
+void Lattice::sap_sweeps_fused(const Batch& B, const S* in, S* s, S* q, int nsap, int nm, float* fb, long long fbs,
+                               const SapSrcUpdate* upd) {
   const FieldSet<S> f = fields<S>();
+  // the first sweep's two launches are the first readers of in_c for their colour c:
+  // with upd, each applies the direction update to its own sites and stores it
+  const SapSrcUpdate none;
   for (int isap = 0; isap < nsap; ++isap)
     for (int c = 0; c < 2; ++c) {
       int flags = SAP_EPI_SUPDATE;
       if (isap == 0) flags |= SAP_SRC_IN;
+      if (isap == 0 && upd) flags |= SAP_SRC_PUPD;
       if (isap > 0 || c == 1) flags |= SAP_FACE_PRO;
       launch_sap_block<S>(g_, sel(c), B, f.nl, (float)kappa_, nm, fb_flags(flags, fb, true), s, in, q, q, nullptr, f.U,
-                          f.C, fb, fbs, st_);
+                          f.C, fb, fbs, st_, (flags & SAP_SRC_PUPD) ? *upd : none);
     }
 }
 
@@ -448,19 +456,23 @@ int Lattice::fb_flags(int flags, const float* fb, bool produce) {
 // prec_ddd_s_ (qws.cc:776-926): out = D M_SAP in, with the final sequence of the
 // improper (npe[mu]==1) and of the proper colouring.
 template <typename S>
-void Lattice::prec_ddd_t(const Batch& B, S* out, const S* in, int nsap, int nm) {
+void Lattice::prec_ddd_t(const Batch& B, S* out, const S* in, int nsap, int nm, const SapSrcUpdate* upd) {
   using R = typename Store<S>::R;
   const int pr = sizeof(S) == 8 ? P64 : sizeof(S) == 4 ? P32 : P16;
   S* s = (S*)work("sap_s", spinor_bytes(pr) * B.n);
   S* q = (S*)work("sap_q", spinor_bytes(pr) * B.n);
   const R k = (R)kappa_;
   if constexpr (std::is_same<S, float>::value) {
+    if (upd && (const void*)upd->p != (const void*)in) throw std::logic_error("prec_ddd_t: the update must be of `in`");
+    // the first sweep applies the update when there is one; otherwise a separate pass
+    const bool fold = upd && fused_ok<S>() && nsap > 0;
+    if (upd && !fold) launch_update_p<float>(B, g_.V * 12, upd->p, upd->r, upd->q, upd->sc, st_);
     if (fused_ok<S>()) {
       const FieldSet<float> f = fields<float>();
       const int src = nsap == 0 ? SAP_SRC_IN : 0;
       long long fbs;
       float* fb = sap_face_buffer<float>(B, fbs);
-      sap_sweeps_fused<float>(B, in, s, q, nsap, nm, fb, fbs);
+      sap_sweeps_fused<float>(B, in, s, q, nsap, nm, fb, fbs, fold ? upd : nullptr);
       // final e: q_e = A_e s_e, out_e = D_in q_e
       launch_sap_block<float>(g_, sel(0), B, f.nl, (float)kappa_, nm,
                               fb_flags(SAP_EPI_OUT | src | (nsap > 0 ? SAP_FACE_PRO : 0), fb, true), s, in, q, q, out,
@@ -476,6 +488,7 @@ void Lattice::prec_ddd_t(const Batch& B, S* out, const S* in, int nsap, int nm) 
       return;
     }
   }
+  if (upd && !std::is_same<S, float>::value) throw std::logic_error("prec_ddd_t: the direction update is FP32");
   S* tmp = (S*)work("sap_tmp", spinor_bytes(pr) * B.n);
   sap_sweeps<S>(B, in, s, q, tmp, nsap, nm);
   if (improper_) launch_zero_blocks<S>(g_, sel(1), B, q, st_);
@@ -887,13 +900,20 @@ void Lattice::bicg_t(const Batch& Bin, S* x, const S* b, double tol, int maxiter
       if (v) return true;
     return false;
   };
+  SapSrcUpdate pupd;
+  pupd.p = reinterpret_cast<float*>(p);
+  pupd.r = reinterpret_cast<const float*>(r);
+  pupd.q = reinterpret_cast<const float*>(q);
+  pupd.sc = sc;
+  bool pupd_pending = false;
   for (int iter = 0; iter < maxiter && any_live(); ++iter) {
     for (int k = 0; k < nb; ++k) iters[k] += live[k];
     if (hp) {
       msap_h_t(B, w, p, nsap, nm);
       hop<S>(sel(-1), B, HOPS_ALL, mk, q, w, nullptr, nullptr, w);
     } else {
-      prec_ddd_t<S>(B, q, p, nsap, nm);
+      prec_ddd_t<S>(B, q, p, nsap, nm, pupd_pending ? &pupd : nullptr);
+      pupd_pending = false;
     }
     launch_dot<S>(B, n, r0, q, part, st_);
     launch_finalize(B, part, 2, sc, FIN_ALPHA, st_);
@@ -911,7 +931,9 @@ void Lattice::bicg_t(const Batch& Bin, S* x, const S* b, double tol, int maxiter
     launch_update_xr<S>(B, n, x, hp ? w : r, r, t, sc, 1, r0, part, st_);
     launch_finalize(B, part, 3, sc, FIN_RHO, st_);
     if (!converge()) break;
-    launch_update_p<S>(B, n, p, r, q, sc, st_);
+    // p = r + beta (p - omega q): done by the next q = D M p (bicgstab_precdd_s.cc:518-545)
+    if (!hp && fold_pupd_) pupd_pending = true;
+    else launch_update_p<S>(B, n, p, r, q, sc, st_);
   }
 }
 

@@ -12,6 +12,10 @@
 #include "kernels.h"
 
 namespace qws {
+struct SapSrcUpdate;
+}
+
+namespace qws {
 
 // GB/s (reads + writes) of a streaming kernel summing nin arrays of `bytes` into one
 // (nin = 1: copy): the fastest of three launch shapes, each averaged over reps
@@ -156,6 +160,10 @@ class Lattice {
   // FP32 fused SAP: face updates through the face buffer (default) or by gathering
   // the other colour's spinors; both give the same results bit for bit
   void set_sap_face_buffer(bool on) { use_fb_ = on; }
+  // FP32 inner BiCGStab with the fused SAP: its direction update p = r + beta (p - omega q)
+  // runs inside the first SAP sweep of the next preconditioner application (default)
+  // or as a separate pass; both give the same results bit for bit
+  void set_sap_source_update(bool on) { fold_pupd_ = on; }
   void set_dslash_kernel(int k);
   int dslash_kernel() const { return g_.dslash_kernel; }
   int autotune_dslash(int reps);
@@ -209,11 +217,15 @@ class Lattice {
   template <typename S> bool fused_ok() const;
   template <typename S> double time_sap_t(int reps, int nsap, int nm);
   template <typename S>
-  void sap_sweeps_fused(const Batch& B, const S* in, S* s, S* q, int nsap, int nm, float* fb, long long fbs);
+  void sap_sweeps_fused(const Batch& B, const S* in, S* s, S* q, int nsap, int nm, float* fb, long long fbs,
+                        const SapSrcUpdate* upd = nullptr);
   template <typename S> float* sap_face_buffer(const Batch& B, long long& fbs);
   static int fb_flags(int flags, const float* fb, bool produce);
   bool use_fb_ = true;
-  template <typename S> void prec_ddd_t(const Batch& B, S* out, const S* in, int nsap, int nm);
+  bool fold_pupd_ = true;
+  // upd: apply BiCGStab's direction update to `in` (== upd->p) first
+  template <typename S>
+  void prec_ddd_t(const Batch& B, S* out, const S* in, int nsap, int nm, const SapSrcUpdate* upd = nullptr);
   template <typename S> void prec_s_t(const Batch& B, S* out, const S* in, int nsap, int nm);
   void msap_h_t(const Batch& B, float* out, const float* in, int nsap, int nm);
   template <typename S>

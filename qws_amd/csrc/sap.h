@@ -16,6 +16,16 @@ enum SapFlags {
   SAP_SRC_IN = 8,        // take s_c from `in` (first sweep: s = in)
   SAP_FACE_OUT = 16,     // also write the face buffer (FP32 slots) for the next launch's prologue
   SAP_FACE_FB = 32,      // the prologue reads the face buffer instead of gathering qface
+  SAP_SRC_PUPD = 64,     // with SAP_SRC_IN (FP32): first in_c = r_c + beta (in_c - omega q_c), stored
+};
+
+// The BiCGStab direction update folded into the first sweep (SAP_SRC_PUPD): `in` is
+// the direction p itself, updated in place from r, q and the RHS's omega, beta.
+struct SapSrcUpdate {
+  float* p = nullptr;
+  const float* r = nullptr;
+  const float* q = nullptr;
+  const Scalars* sc = nullptr;
 };
 
 bool sap_fusable(const Geo& g);
@@ -29,6 +39,6 @@ void launch_sap_face(const Geo& g, BlockSel sel, const Batch& B, int nl, float g
 template <typename S>
 void launch_sap_block(const Geo& g, BlockSel sel, const Batch& B, int nl, float kappa, int nm, int flags, S* s,
                       const S* in, const S* qface, S* qout, S* out, const S* U, const S* C, float* fb,
-                      long long fbs, hipStream_t st);
+                      long long fbs, hipStream_t st, const SapSrcUpdate& upd = SapSrcUpdate());
 
 }  // namespace qws

@@ -41,13 +41,18 @@ long long sap_face_buffer_floats(const Geo& g) {
 template <typename S>
 void launch_sap_block(const Geo& g, BlockSel sel, const Batch& B, int nl, float kappa, int nm, int flags, S* s,
                       const S* in, const S* qface, S* qout, S* out, const S* U, const S* C, float* fb,
-                      long long fbs, hipStream_t st) {
+                      long long fbs, hipStream_t st, const SapSrcUpdate& upd) {
   if (sel.n == 0) return;
   if (nm < 1) nm = 1;  // the first application is unconditional (static_solver.cc:77-101)
   if ((flags & (SAP_FACE_OUT | SAP_FACE_FB)) && (!fb || sap_face_buffer_floats(g) == 0))
     throw std::invalid_argument("fused SAP: face-buffer flags need a buffer and a block shape that allows one");
-  if (g.Vb == 512) sap512::launch<S>(g, sel, B, nl, kappa, nm, flags, s, in, qface, qout, out, U, C, fb, fbs, st);
-  else sap256::launch<S>(g, sel, B, nl, kappa, nm, flags, s, in, qface, qout, out, U, C, fb, fbs, st);
+  if (flags & SAP_SRC_PUPD) {
+    if (!std::is_same<S, float>::value || !(flags & SAP_SRC_IN) || !upd.p || !upd.r || !upd.q || !upd.sc ||
+        (const void*)upd.p != (const void*)in)
+      throw std::invalid_argument("fused SAP: the source update needs FP32, SAP_SRC_IN and in == p, r, q, scalars");
+  }
+  if (g.Vb == 512) sap512::launch<S>(g, sel, B, nl, kappa, nm, flags, s, in, qface, qout, out, U, C, fb, fbs, st, upd);
+  else sap256::launch<S>(g, sel, B, nl, kappa, nm, flags, s, in, qface, qout, out, U, C, fb, fbs, st, upd);
 }
 
 void launch_sap_face(const Geo& g, BlockSel sel, const Batch& B, int nl, float gamma, float* out, const float* fb,
@@ -60,9 +65,9 @@ void launch_sap_face(const Geo& g, BlockSel sel, const Batch& B, int nl, float g
 
 template void launch_sap_block<float>(const Geo&, BlockSel, const Batch&, int, float, int, int, float*, const float*,
                                       const float*, float*, float*, const float*, const float*, float*, long long,
-                                      hipStream_t);
+                                      hipStream_t, const SapSrcUpdate&);
 template void launch_sap_block<__half>(const Geo&, BlockSel, const Batch&, int, float, int, int, __half*,
                                        const __half*, const __half*, __half*, __half*, const __half*, const __half*,
-                                       float*, long long, hipStream_t);
+                                       float*, long long, hipStream_t, const SapSrcUpdate&);
 
 }  // namespace qws

@@ -202,6 +202,41 @@ def test_fused_sap_face_buffer_is_bitwise_the_gather(L, B, su3, nsap, nm):
     assert rel(res[True][0], p.orc.prec_ddd_s(srcs[0], nsap, nm)) < 5e-6
 
 
+def test_bicgstab_direction_update_in_the_sap_sweep_is_bitwise_the_separate_pass():
+    """The inner BiCGStab's direction update p = r + beta (p - omega q) done by the first
+    fused SAP sweep of the next preconditioner application gives the separate update
+    pass's results bit for bit: the inner solve alone (x, iterations), a mixed solve,
+    and a batch of Gaussian and point sources."""
+    lat = Q.Lattice([8, 8, 8, 16], [4, 4, 4, 4], -1, 12, kappa=0.13)
+    lat.generate_gauge(0.3, 11)
+    lat.compute_clover(1.0)
+    srcs = [lat.gaussian(Q.F64, 100), lat.gaussian(Q.F64, 101), lat.point(Q.F64, (0, 0, 0, 0), 0, 0),
+            lat.point(Q.F64, (3, 1, 2, 5), 2, 1)]
+    b32 = lat.spinor(Q.F32, srcs[0].download().astype(np.float32))
+    prm = dict(nsap=2, nm=3, tol=1e-10, maxiter=30, tol_s=1e-4, maxiter_s=50)
+    res = {}
+    for on in (True, False):
+        lat.set_sap_source_update(on)
+        x = lat.spinor(Q.F32)
+        it = lat.bicgstab_precdd(x, b32, 1e-6, 80, 2, 3)
+        xd = lat.spinor(Q.F64)
+        st = lat.solve(xd, srcs[1], **prm)
+        xs = [lat.spinor(Q.F64) for _ in srcs]
+        sts = lat.solve_batch(xs, srcs, **prm)
+        res[on] = (x.download(), it, xd.download(), (st["outer_iters"], st["inner_iters"]), st["true_residual"],
+                   [v.download() for v in xs], [(s_["outer_iters"], s_["inner_iters"]) for s_ in sts])
+    lat.set_sap_source_update(True)
+    a, b = res[True], res[False]
+    print("inner solve", a[1], "solve", a[3], "batch", a[6])
+    assert a[1] == b[1] and 1 < a[1] < 80
+    assert np.array_equal(a[0], b[0])
+    assert np.array_equal(a[2], b[2]) and a[3] == b[3]
+    assert 0 < a[4] < 1e-9
+    assert a[6] == b[6]
+    for u, v in zip(a[5], b[5]):
+        assert np.array_equal(u, v)
+
+
 def test_time_sap_runs_the_fused_kernel():
     """The SAP timing entry point runs the fused launches of one preconditioner
     application (FP32 and FP16 storage), leaves the operator's results unchanged and
