@@ -186,6 +186,19 @@ void mtilde_s_(scs_t* out, scs_t* in);
 /* reordering.cc:141,108 -- FP64 <-> FP32 with normalisation */
 void fermi_reorder_d2s_dd_renorm_(scs_t* out, const scd_t* in, double* dnorm);
 void fermi_reorder_s2d_dd_renorm_(scd_t* out, const scs_t* in, const double* dnorm);
+/* reordering.cc:83,59 -- FP64 <-> FP32 without normalisation */
+void fermi_reorder_d2s_dd_(scs_t* out, const scd_t* in);
+void fermi_reorder_s2d_dd_(scd_t* out, const scs_t* in);
+/* qws.cc:634 -- D on one domain (face terms with factor mkappa) into out + vols*domain */
+void ddd_eo_s_(scs_t* out, scs_t* in, int* domain);
+/* bicgstab_dd_s.cc:67, bicgstab_dd_d.cc:64 -- plain BiCGStab on ddd_s_ / ddd_d_ (x: initial
+ * guess in, solution out; stops at sqrt(|r|^2/|b|^2) < 1e-6 (FP32) / 1e-14 (FP64) or
+ * after maxiter-1 iterations; *conviter = the last iteration).  Reductions in FP64. */
+void bicgstab_dd_s_(scs_t* x, scs_t* b, int* conviter, int* maxiter);
+void bicgstab_dd_d_(scd_t* x, scd_t* b, int* conviter, int* maxiter);
+/* bicgstab_precdd_s.cc:561 -- prints "@C iter err", err = |b - prec_ddd_s_(x)| / sqrt(bnorm) */
+void check_residual_s_(const scs_t* x, const scs_t* b, const int* iter, const float* bnorm, const int* nsap,
+                       const int* nm);
 
 /* No counterpart in the reference: mark the host field arrays as modified, so
  * that the next call re-uploads them; and choose SAP domains finer than the two

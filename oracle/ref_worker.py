@@ -223,6 +223,14 @@ def run(inp):
                                  _i(nsap), _i(nm))
             res[op] = r.back_d(x)
             res[op + "_iter"] = np.array(conv.value)
+        elif op in ("bicgstab_dd_s", "bicgstab_dd_d"):  # plain BiCGStab on D from x0
+            fp64 = op.endswith("_d")
+            x = r.d(inp["x0"]) if fp64 else r.s(inp["x0"])
+            bb = r.d(b) if fp64 else r.s(b)
+            conv = ctypes.c_int(0)
+            getattr(lib, op + "_")(_ptr(x), _ptr(bb), ctypes.byref(conv), _i(int(inp["maxiter"])))
+            res[op] = r.back_d(x) if fp64 else r.back_s(x)
+            res[op + "_iter"] = np.array(conv.value)
         elif op in ("cg_vm", "mcg_vm"):
             # even/odd solvers on mtdagmt_vm_: the parity field is the first x-domain of the
             # driver's AoSoA spinor (as main.cc:200-213 passes it); also runs the oracle's

@@ -1076,6 +1076,78 @@ void fermi_reorder_s2d_dd_renorm_(scd_t* out, const scs_t* in, const double* dno
         }
 }
 
+// reordering.cc:59-105 -- the same site maps without the normalisation
+void fermi_reorder_d2s_dd_(scs_t* out, const scd_t* in) {
+  for (int x = 0; x < nx; ++x)
+    for (int t = 0; t < nt; ++t)
+      for (int z = 0; z < nz; ++z)
+        for (int y = 0; y < ny; ++y) {
+          const int id = (x % nxh) / QWS_VLEND + nxd * (y + ny * (z + nz * t)) + vold * (x / nxh);
+          const int jd = (x % nxh) % QWS_VLEND;
+          const int is = (x % nxh) / QWS_VLENS + nxs * (y + ny * (z + nz * t)) + vols * (x / nxh);
+          const int js = (x % nxh) % QWS_VLENS;
+          for (int c = 0; c < 3; ++c)
+            for (int s = 0; s < 4; ++s)
+              for (int ri = 0; ri < 2; ++ri) out[is].c[c][s][ri][js] = (float)in[id].c[c][s][ri][jd];
+        }
+}
+void fermi_reorder_s2d_dd_(scd_t* out, const scs_t* in) {
+  const double one = 1.0;
+  fermi_reorder_s2d_dd_renorm_(out, in, &one);
+}
+
+// qws.cc:634-639 -- D on one domain from the domain-local and face parts
+void ddd_eo_s_(scs_t* out, scs_t* in, int* domain) {
+  ddd_out_pre_s_(in, domain);
+  ddd_in_s_(&out[(size_t)vols * *domain], &in[(size_t)vols * *domain], domain);
+  ddd_out_pos_s_(&out[(size_t)vols * *domain], in, domain, (float)mkappa);
+}
+
+// bicgstab_dd_s.cc:67, bicgstab_dd_d.cc:64 -- plain BiCGStab on D
+void bicgstab_dd_s_(scs_t* x, scs_t* b, int* conviter, int* maxiter) {
+  drop_in(__func__, [&] {
+    Lattice& L = lat_sync("bicgstab_dd_s_");
+    void* xx = dbuf(2, qws::P32);
+    void* bb = dbuf(3, qws::P32);
+    L.upload_spinor(qws::P32, xx, x, 1, 0);
+    L.upload_spinor(qws::P32, bb, b, 1, 0);
+    const int it = L.bicgstab_dd(qws::P32, xx, bb, *maxiter, 1e-6);
+    L.download_spinor(qws::P32, xx, x, 1, 0);
+    if (conviter) *conviter = it;
+  });
+}
+void bicgstab_dd_d_(scd_t* x, scd_t* b, int* conviter, int* maxiter) {
+  drop_in(__func__, [&] {
+    Lattice& L = lat_sync("bicgstab_dd_d_");
+    void* xx = dbuf(0, qws::P64);
+    void* bb = dbuf(1, qws::P64);
+    L.upload_spinor(qws::P64, xx, x, 1, 1);
+    L.upload_spinor(qws::P64, bb, b, 1, 1);
+    const int it = L.bicgstab_dd(qws::P64, xx, bb, *maxiter, 1e-14);
+    L.download_spinor(qws::P64, xx, x, 1, 1);
+    if (conviter) *conviter = it;
+  });
+}
+
+// bicgstab_precdd_s.cc:561-597 -- prints "@C iter |b - D M_SAP x| / sqrt(bnorm)"
+void check_residual_s_(const scs_t* x, const scs_t* b, const int* iter, const float* bnorm, const int* nsap,
+                       const int* nm) {
+  const size_t n = (size_t)vols * 2;
+  std::vector<scs_t> w(n);
+  prec_ddd_s_(w.data(), const_cast<scs_t*>(x), const_cast<int*>(nsap), const_cast<int*>(nm));
+  float rnorm = 0.0f;
+  for (size_t i = 0; i < n; ++i)
+    for (int c = 0; c < 3; ++c)
+      for (int s = 0; s < 4; ++s)
+        for (int ri = 0; ri < 2; ++ri)
+          for (int v = 0; v < QWS_VLENS; ++v) {
+            const float d = b[i].c[c][s][ri][v] - w[i].c[c][s][ri][v];
+            rnorm += d * d;
+          }
+  const float err = (float)std::sqrt((double)(rnorm / *bnorm));
+  if (rank == 0) std::printf("@C %5d %12.7e\n", *iter, err);
+}
+
 }  // extern "C"
 
 // ---- BQCD loaders (qws_bqcd.cc:66-647).  BQCD numbers the sites of each parity

@@ -743,6 +743,57 @@ int Lattice::bicgstab_eo(double* x, const double* b, int maxiter, double tol, in
   return iter;
 }
 
+// bicgstab_dd_s.cc:67-330 (FP32) and bicgstab_dd_d.cc:64-265 (FP64): the same
+// iteration as bicgstab_eo on D = 1 - kappa T^-1 H; the reductions are FP64 here
+template <typename S>
+int Lattice::bicgstab_dd_t(S* x, const S* b, int maxiter, double tol) {
+  using R = typename Store<S>::R;
+  const int pr = sizeof(S) == 8 ? P64 : P32;
+  const long long n = g_.V * 12;
+  const size_t bytes = spinor_bytes(pr);
+  S* q = (S*)work("bdd_q", bytes);
+  S* r = (S*)work("bdd_r", bytes);
+  S* p = (S*)work("bdd_p", bytes);
+  S* t = (S*)work("bdd_t", bytes);
+  S* r0 = (S*)work("bdd_r0", bytes);
+  if (!sc_eo_) sc_eo_ = dalloc<Scalars>(1);
+  double* part = reduction_partials(1);
+  const Batch B = single();
+  const R mk = (R)(-kappa_);
+  auto D = [&](S* out, const S* in) { hop<S>(sel(-1), B, HOPS_ALL, mk, out, in, nullptr, nullptr, in); };
+  D(q, x);
+  launch_bicg_init_outer<S>(B, n, r, p, r0, b, q, part, st_);
+  launch_finalize(B, part, 2, sc_eo_, FIN_INIT_OUTER, st_);
+  auto converged = [&] {
+    read_scalars(sc_eo_, 1);
+    return std::sqrt(sc_host_[0].rnorm / sc_host_[0].bnorm) < tol;
+  };
+  int iter;
+  for (iter = 1; iter < maxiter; ++iter) {
+    D(q, p);
+    launch_dot<S>(B, n, r0, q, part, st_);
+    launch_finalize(B, part, 2, sc_eo_, FIN_ALPHA, st_);
+    launch_update_xr<S>(B, n, x, p, r, q, sc_eo_, 0, nullptr, part, st_);
+    launch_finalize(B, part, 1, sc_eo_, FIN_RNORM, st_);
+    if (converged()) break;
+    D(t, r);
+    launch_norm_dot<S>(B, n, t, r, part, st_);
+    launch_finalize(B, part, 3, sc_eo_, FIN_OMEGA, st_);
+    launch_update_xr<S>(B, n, x, r, r, t, sc_eo_, 1, r0, part, st_);
+    launch_finalize(B, part, 3, sc_eo_, FIN_RHO, st_);
+    if (converged()) break;
+    launch_update_p<S>(B, n, p, r, q, sc_eo_, st_);
+  }
+  sync();
+  return iter;
+}
+
+int Lattice::bicgstab_dd(int prec, void* x, const void* b, int maxiter, double tol) {
+  if (prec == P64) return bicgstab_dd_t<double>((double*)x, (const double*)b, maxiter, tol);
+  if (prec == P32) return bicgstab_dd_t<float>((float*)x, (const float*)b, maxiter, tol);
+  throw std::invalid_argument("bicgstab_dd: FP32 or FP64 vectors");
+}
+
 double Lattice::reduce_real(long long n, const double* a, const double* b) {
   if (n % 2) throw std::invalid_argument("reduce_real: an even number of reals");
   if (!sc_eo_) sc_eo_ = dalloc<Scalars>(1);

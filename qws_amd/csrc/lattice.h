@@ -189,6 +189,10 @@ class Lattice {
   // -kappa^2) from the initial guess in x, iterations 1..maxiter-1, stopping when
   // sqrt(|r|^2/|b|^2) < tol (the reference's 1e-14); returns the last iteration
   int bicgstab_eo(double* x, const double* b, int maxiter, double tol, int pce, double fac);
+  // plain BiCGStab on the full operator D (FP32 or FP64 vectors, FP64 reductions): x
+  // is the initial guess; stops at sqrt(|r|^2/|b|^2) < tol or after maxiter-1 iterations
+  // and returns the last iteration
+  int bicgstab_dd(int prec, void* x, const void* b, int maxiter, double tol);
   // cg.cc:65 -- CG on mtilde^+ mtilde from the initial guess in x; returns the iterations
   int cg_eo(double* x, const double* b, int maxiter, double tol, int pce, double fac);
   // mcg.cc:65 -- multi-shift CG: the parity field xs[k] solves
@@ -227,6 +231,7 @@ class Lattice {
   template <typename S>
   void prec_ddd_t(const Batch& B, S* out, const S* in, int nsap, int nm, const SapSrcUpdate* upd = nullptr);
   template <typename S> void prec_s_t(const Batch& B, S* out, const S* in, int nsap, int nm);
+  template <typename S> int bicgstab_dd_t(S* x, const S* b, int maxiter, double tol);
   void msap_h_t(const Batch& B, float* out, const float* in, int nsap, int nm);
   template <typename S>
   void bicg_t(const Batch& B, S* x, const S* b, double tol, int maxiter, int nsap, int nm, std::vector<int>& iters,
