@@ -271,6 +271,122 @@ def test_dropin_plain_dd_solvers_and_helpers_case0():
     assert rel(RW.spinor_from_aosoa(xs, nloc, 16), ref["bicgstab_dd_s"]) < 1e-5
 
 
+def _case0_dropin(L):
+    """qws_init_ on data/CASE0's geometry with its fields in the host arrays (kappa 0.05)."""
+    nloc = [32, 6, 4, 3]
+    _, U, C, b, x0 = O.case_inputs(nloc)
+    ci = lambda v: ctypes.byref(ctypes.c_int(v))
+    keep = [(ctypes.c_int * 4)(1, 1, 1, 1), (ctypes.c_int * 4)(1, 1, 1, -1), ctypes.c_int(0), ctypes.c_int(1),
+            (ctypes.c_int * 4)(1, 2, 2, 2)]
+    L.qws_init_(ci(32), ci(6), ci(4), ci(3), keep[0], keep[1], ctypes.byref(keep[2]), ctypes.byref(keep[3]), keep[4])
+    vold = ctypes.c_int.in_dll(L, "vold").value
+    vols = ctypes.c_int.in_dll(L, "vols").value
+
+    def glob(name, ct, n):
+        return np.ctypeslib.as_array((ct * n).from_address(ctypes.c_void_p.in_dll(L, name).value))
+
+    glob("glud", ctypes.c_double, vold * 8 * 144)[:] = RW.links_to_aosoa(U, nloc, 8).ravel()
+    glob("glus", ctypes.c_float, vols * 8 * 288)[:] = RW.links_to_aosoa(U.astype(np.float32), nloc, 16).ravel()
+    glob("clvd", ctypes.c_double, vold * 2 * 576)[:] = RW.clover_to_aosoa(C, nloc, 8).ravel()
+    glob("clvs", ctypes.c_float, vols * 2 * 1152)[:] = RW.clover_to_aosoa(C.astype(np.float32), nloc, 16).ravel()
+    ctypes.c_double.in_dll(L, "kappa").value = 0.05
+    ctypes.c_double.in_dll(L, "mkappa").value = -0.05
+    ctypes.c_double.in_dll(L, "kappa2").value = 0.0025
+    L.qws_mi355x_fields_modified_()
+    return nloc, U, C, b, x0, keep, vold, vols
+
+
+@pytest.mark.skipif(not RW.available(), reason="oracle/_ref/libqws_ref.so not built")
+def test_dropin_remaining_entry_points_case0(capfd):
+    """The QWS entry points not covered elsewhere, on data/CASE0's host arrays: the FP64
+    domain-local and face terms and the clover multiplies reproduce CASE0's golden norms
+    (main.cc:636-690); jinv_ddd_in_s_, prec_ddd_s_, prec_s_ and bicgstab_precdd_s_ agree
+    with the reference library (oracle/_ref) on the same input; the renormalising
+    reorders follow their closed forms (out = in / |in|, out = in * dnorm);
+    check_residual_s_ prints |b - prec_ddd_s_(x)| / sqrt(bnorm); and a solve with SAP
+    blocks set by qws_mi355x_set_sap_blocks_ still meets its tolerance."""
+    L = Q.lib()
+    nloc, U, C, b, x0, keep, vold, vols = _case0_dropin(L)
+    ci = lambda v: ctypes.byref(ctypes.c_int(v))
+    cd = lambda v: ctypes.byref(ctypes.c_double(v))
+    bd = RW.spinor_to_aosoa(b, nloc, 8)
+    bs = RW.spinor_to_aosoa(b.astype(np.float32), nloc, 16)
+    got = {}
+    out = np.zeros_like(bd)
+    L.ddd_out_pre_d_(_ptr(bd), ci(0))
+    L.ddd_out_pos_d_(_ptr(out), _ptr(bd), ci(0))
+    got["ddd_out_pre|pos_d_"] = O.norm2(out)
+    out[:] = 0
+    L.ddd_in_d_(_ptr(out), _ptr(bd), ci(0))
+    got["ddd_in_d_"] = O.norm2(out)
+    inp = np.ascontiguousarray(bd.reshape(2 * vold, -1)[:vold])
+    o1 = inp.copy()
+    L.clvd_vm_(ci(0), _ptr(o1))
+    got["clvd_vm_"] = O.norm2(o1)
+    o2 = np.zeros_like(inp)
+    L.clvd_vm2_(ci(0), _ptr(o2), _ptr(inp))
+    got["clvd_vm2_"] = O.norm2(o2)
+    # the SAP pieces and the inner solver (nsap 4, nm 2; 6 fixed inner iterations)
+    mine = {}
+    for d in (0, 1):
+        o = np.zeros_like(bs)
+        L.jinv_ddd_in_s_(_ptr(o[d]), _ptr(bs[d]), ci(d), ci(2))
+        mine[f"jinv{d}"] = o
+    o = np.zeros_like(bs)
+    L.prec_ddd_s_(_ptr(o), _ptr(bs), ci(4), ci(2))
+    mine["prec_ddd_s"] = o
+    o = np.zeros_like(bs)
+    L.prec_s_(_ptr(o), _ptr(bs), ci(4), ci(2))
+    mine["prec_s"] = o
+    o = np.zeros_like(bs)
+    it = ctypes.c_int(0)
+    L.bicgstab_precdd_s_(_ptr(o), _ptr(bs), cd(-1.0), ctypes.byref(it), ci(6), ci(4), ci(2))
+    mine["bicgstab_precdd_s"] = o
+    # renormalising reorders (reordering.cc:108-204)
+    L.fermi_reorder_d2s_dd_renorm_.argtypes = [ctypes.c_void_p] * 3
+    L.fermi_reorder_s2d_dd_renorm_.argtypes = [ctypes.c_void_p] * 3
+    dn = ctypes.c_double(0)
+    s32 = np.zeros_like(bs)
+    L.fermi_reorder_d2s_dd_renorm_(_ptr(s32), _ptr(bd), ctypes.byref(dn))
+    back = np.zeros_like(bd)
+    L.fermi_reorder_s2d_dd_renorm_(_ptr(back), _ptr(s32), ctypes.byref(dn))
+    # check_residual_s_ of the inner solution
+    L.check_residual_s_.argtypes = [ctypes.c_void_p] * 6
+    bn = ctypes.c_float(np.float32(O.norm2(bs)))
+    capfd.readouterr()
+    L.check_residual_s_(_ptr(mine["bicgstab_precdd_s"]), _ptr(bs), ci(6), ctypes.byref(bn), ci(4), ci(2))
+    printed = capfd.readouterr().out
+    w = np.zeros_like(bs)
+    L.prec_ddd_s_(_ptr(w), _ptr(mine["bicgstab_precdd_s"]), ci(4), ci(2))
+    # SAP blocks of 8x6x4x3 instead of the two x-halves
+    L.qws_mi355x_set_sap_blocks_((ctypes.c_int * 4)(8, 6, 4, 3))
+    x = RW.spinor_to_aosoa(x0, nloc, 8)
+    L.bicgstab_dd_mix_(_ptr(x), _ptr(bd), cd(1e-12), ctypes.byref(it), ci(50), cd(1e-4), ci(50), ci(4), ci(2))
+    r = np.zeros_like(bd)
+    L.ddd_d_(_ptr(r), _ptr(x))
+    L.qws_finalize_()
+    for name, v in got.items():
+        assert K.check_value(name, v, K.expected("CASE0", name)), (name, v)
+    ref = RW.call(nloc, 0.05, U, C, b, ["jinv0", "jinv1", "prec_ddd_s", "prec_s", "bicgstab_precdd_s"], nsap=4,
+                  nm=2, tol_s=-1.0, maxiter_s=6)
+    # FP32 operators to rounding; the solver after 6 iterations also carries the
+    # difference of FP32 (reference) and FP64 (this library) reductions
+    for name, v in mine.items():
+        tol = 2e-5 if name == "bicgstab_precdd_s" else 2e-6
+        assert rel(RW.spinor_from_aosoa(v, nloc, 16), ref[name]) < tol, name
+    nb = np.sqrt(O.norm2(bd))
+    assert abs(dn.value - nb) < 1e-14 * nb
+    s32p = RW.spinor_from_aosoa(s32, nloc, 16)
+    assert rel(s32p, (b / nb).astype(np.float32)) < 1e-7
+    assert np.array_equal(RW.spinor_from_aosoa(back, nloc, 8), s32p.astype(np.float64) * dn.value)
+    line = [t for t in printed.splitlines() if t.startswith("@C")]
+    assert len(line) == 1 and int(line[0].split()[1]) == 6, printed
+    err = np.sqrt(O.norm2((bs - w).astype(np.float64)) / float(bn.value))
+    assert abs(float(line[0].split()[2]) - err) < 1e-4 * err + 1e-12
+    assert 1 < it.value < 50
+    assert O.norm2(bd - r) < 1e-22 * O.norm2(bd)
+
+
 def test_bicgstab_direction_update_in_the_sap_sweep_is_bitwise_the_separate_pass():
     """The inner BiCGStab's direction update p = r + beta (p - omega q) done by the first
     fused SAP sweep of the next preconditioner application gives the separate update
