@@ -20,6 +20,8 @@
 #ifndef QWS_MI355X_H
 #define QWS_MI355X_H
 
+#include <stdio.h>
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -50,6 +52,8 @@ extern int nx, ny, nz, nt, nxh, nxd, nxs, vold, vols;
 extern int npe[4];
 extern double fbc[4][2];
 extern int rank, domain_e, domain_o;
+/* qws.cc:124 -- QWS's per-rank output file (opened there only for per-rank timing output) */
+extern FILE* para_outputfile;
 
 /* qws.cc:130 -- geometry, field allocation, t boundary (fbc_f[3] = -1: antiperiodic).
  * block_size is ignored, as in the reference; the SAP domains are the two x-halves
@@ -189,6 +193,19 @@ void fermi_reorder_s2d_dd_renorm_(scd_t* out, const scs_t* in, const double* dno
 /* reordering.cc:83,59 -- FP64 <-> FP32 without normalisation */
 void fermi_reorder_d2s_dd_(scs_t* out, const scd_t* in);
 void fermi_reorder_s2d_dd_(scd_t* out, const scs_t* in);
+/* ddd_out_s.cc:96,100 -- the face terms without per-call timers (the same functions here) */
+void ddd_out_pre_s_no_timer_(scs_t* in, int* domain);
+void ddd_out_pos_s_no_timer_(scs_t* out, scs_t* in, int* domain, float factor);
+/* qws.cc:445-500 -- driver helpers: clear size vectors; print "%30s : %24.14e" of |field|^2 */
+void zero_scd_field(scd_t* inout, int size);
+void zero_scs_field(scs_t* inout, int size);
+void print_scdnorm2(const char* a, scd_t* in, int size);
+void print_scsnorm2(const char* a, scs_t* in, int size);
+/* timing.c:171,185 -- wall clock in seconds; the timing report (this library keeps no
+ * per-function timers: check_timing_(NULL) and print_timing_ say so) */
+double get_clock(void);
+void check_timing_(const char* name);
+void print_timing_(void);
 /* qws.cc:634 -- D on one domain (face terms with factor mkappa) into out + vols*domain */
 void ddd_eo_s_(scs_t* out, scs_t* in, int* domain);
 /* bicgstab_dd_s.cc:67, bicgstab_dd_d.cc:64 -- plain BiCGStab on ddd_s_ / ddd_d_ (x: initial

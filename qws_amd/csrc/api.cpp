@@ -8,6 +8,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <ctime>
 #include <string>
 #include <vector>
 
@@ -360,6 +361,9 @@ int nx = 0, ny = 0, nz = 0, nt = 0, nxh = 0, nxd = 0, nxs = 0, vold = 0, vols = 
 int npe[4] = {1, 1, 1, 1};
 double fbc[4][2] = {{2, 2}, {2, 2}, {2, 2}, {2, 2}};
 int rank = 0, domain_e = 0, domain_o = 1;
+// qws.cc:124 -- per-rank output file of QWS's drivers (opened there only for per-rank
+// timing output; never opened here)
+FILE* para_outputfile = nullptr;
 
 // BQCD provides these (qws_bqcd.cc:56-57); weak so that the library loads without BQCD.
 int std_xyzt2i_(int*) __attribute__((weak));
@@ -1095,6 +1099,48 @@ void fermi_reorder_s2d_dd_(scd_t* out, const scs_t* in) {
   const double one = 1.0;
   fermi_reorder_s2d_dd_renorm_(out, in, &one);
 }
+
+// ddd_out_s.cc:96,100 -- the face terms without QWS's per-call timers (this library
+// keeps none): the same functions
+void ddd_out_pre_s_no_timer_(scs_t* in, int* domain) { ddd_out_pre_s_(in, domain); }
+void ddd_out_pos_s_no_timer_(scs_t* out, scs_t* in, int* domain, float factor) {
+  ddd_out_pos_s_(out, in, domain, factor);
+}
+
+// qws.cc:445-500 -- host helpers of QWS's drivers: clear a field, print |field|^2
+// ("%30s : %24.14e"; the FP32 sum is formed in FP64 and rounded to FP32)
+void zero_scd_field(scd_t* inout, int size) { std::memset(inout, 0, sizeof(scd_t) * (size_t)size); }
+void zero_scs_field(scs_t* inout, int size) { std::memset(inout, 0, sizeof(scs_t) * (size_t)size); }
+void print_scdnorm2(const char* a, scd_t* in, int size) {
+  const double* v = reinterpret_cast<const double*>(in);
+  double acc = 0;
+  for (size_t i = 0; i < (size_t)size * 24 * QWS_VLEND; ++i) acc += v[i] * v[i];
+  if (rank == 0) {
+    std::printf("%30s : %24.14e\n", a, acc);
+    std::fflush(stdout);
+  }
+}
+void print_scsnorm2(const char* a, scs_t* in, int size) {
+  const float* v = reinterpret_cast<const float*>(in);
+  double acc = 0;
+  for (size_t i = 0; i < (size_t)size * 24 * QWS_VLENS; ++i) acc += (double)v[i] * v[i];
+  if (rank == 0) {
+    std::printf("%30s : %24.14e\n", a, (double)(float)acc);
+    std::fflush(stdout);
+  }
+}
+
+// timing.c:171,185 -- QWS's driver clock and its per-function timing report; this
+// library keeps no per-function timers, so the report says so
+double get_clock(void) {
+  timespec tp;
+  clock_gettime(CLOCK_REALTIME, &tp);
+  return (double)tp.tv_sec + (double)tp.tv_nsec * 1e-9;
+}
+void check_timing_(const char* name) {
+  if (name == nullptr && rank == 0) std::printf("(libqws_mi355x keeps no per-function timers)\n");
+}
+void print_timing_(void) { check_timing_(nullptr); }
 
 // qws.cc:634-639 -- D on one domain from the domain-local and face parts
 void ddd_eo_s_(scs_t* out, scs_t* in, int* domain) {
