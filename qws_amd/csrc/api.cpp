@@ -437,18 +437,19 @@ void require_domain_blocks(const char* where) {
 }
 
 void release_drop_in() {
+  // (frees on the way out: a failure here has nothing left to report to)
   for (auto& p : g_qws.dev)
     if (p) {
-      hipFree(p);
+      (void)hipFree(p);
       p = nullptr;
     }
   for (auto& row : g_qws.eo)
     for (auto& p : row)
       if (p) {
-        hipFree(p);
+        (void)hipFree(p);
         p = nullptr;
       }
-  if (g_qws.eo_multi) hipFree(g_qws.eo_multi);
+  if (g_qws.eo_multi) (void)hipFree(g_qws.eo_multi);
   g_qws.eo_multi = nullptr;
   g_qws.eo_multi_bytes = 0;
   delete g_qws.lat;

@@ -271,7 +271,6 @@ template <typename R> __device__ __forceinline__ void link_row2(Cx<R> u[9]) {
 // u[i*3+j] = QWS c[i][j] of the link at site s, direction mu.
 template <typename S, int NL>
 __device__ __forceinline__ void load_link(const S* U, int mu, long long s, long long V, Cx<typename Store<S>::R> u[9]) {
-  using R = typename Store<S>::R;
   constexpr int NC = NL / 2;
   const long long base = (long long)mu * NC * V + s;
 #pragma unroll
