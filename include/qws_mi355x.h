@@ -95,6 +95,12 @@ void bicgstab_dd_mix_(scd_t* x, scd_t* b, double* tol, int* conviter, int* maxit
  * errors are reported on stderr and never end the process. */
 int bicgstab_dd_mix_status_(scd_t* x, scd_t* b, double* tol, int* conviter, int* maxiter, double* tol_s,
                             int* maxiter_s, int* nsap, int* nm, double* true_residual);
+/* Not in the reference: bicgstab_dd_mix2_hf_ (the FP16-storage SAP method) with the same
+ * return codes and true residual as bicgstab_dd_mix_status_; *conviter as
+ * bicgstab_dd_mix2_hf_ reports it (set only on convergence). */
+int bicgstab_dd_mix2_hf_status_(scd_t* x, const scd_t* b, const double* tol, int* conviter, const int* maxiter,
+                                const double* tol_s, const int* maxiter_s, const int* nsap, const int* nm,
+                                double* true_residual);
 /* bicgstab_precdd_s.cc:85 -- FP32 right-preconditioned BiCGStab on D M_SAP, x0 = 0 */
 void bicgstab_precdd_s_(scs_t* x, scs_t* b, double* tol, int* conviter, int* maxiter, int* nsap, int* nm);
 /* qws.cc:776 -- out = D M_SAP in */

@@ -762,7 +762,7 @@ void bicgstab_precdd_s_(scs_t* x, scs_t* b, double* tol, int* conviter, int* max
 
 // The outer solve on the host arrays; returns |b - D x| / |b| (FP64, on the GPU) when
 // true_res is set.  Throws on any failure; the callers decide whether that ends the process.
-static void dd_mix_body(const char* where, scd_t* x, const scd_t* b, const double* tol, int* conviter,
+static bool dd_mix_body(const char* where, scd_t* x, const scd_t* b, const double* tol, int* conviter,
                         const int* maxiter, const double* tol_s, const int* maxiter_s, const int* nsap, const int* nm,
                         int sap_prec, double* true_res) {
     if (!g_qws.lat) throw std::runtime_error("qws_init_ has not been called");
@@ -791,6 +791,9 @@ static void dd_mix_body(const char* where, scd_t* x, const scd_t* b, const doubl
     } else if (it < p.maxiter && conviter) {
       *conviter = it - 1;
     }
+    // the recursive residual met tol within the iterations run (a solve that ran out
+    // reports the iteration limit)
+    return it < p.maxiter;
 }
 
 static void dd_mix_generic(const char* where, scd_t* x, const scd_t* b, const double* tol, int* conviter,
@@ -804,35 +807,45 @@ void bicgstab_dd_mix_(scd_t* x, scd_t* b, double* tol, int* conviter, int* maxit
   dd_mix_generic("bicgstab_dd_mix_", x, b, tol, conviter, maxiter, tol_s, maxiter_s, nsap, nm, qws::P32);
 }
 
-int bicgstab_dd_mix_status_(scd_t* x, scd_t* b, double* tol, int* conviter, int* maxiter, double* tol_s,
-                            int* maxiter_s, int* nsap, int* nm, double* true_residual) {
+// The status variants: the same solve, the outcome as a return code and the FP64 true
+// residual, errors on stderr instead of ending the process.
+static int dd_mix_status(const char* where, scd_t* x, const scd_t* b, const double* tol, int* conviter,
+                         const int* maxiter, const double* tol_s, const int* maxiter_s, const int* nsap, const int* nm,
+                         int sap_prec, double* true_residual) {
   if (true_residual) *true_residual = -1;
   if (!g_qws.lat) {
-    std::fprintf(stderr, "ERROR at %s : qws_init_ has not been called\n", __func__);
+    std::fprintf(stderr, "ERROR at %s : qws_init_ has not been called\n", where);
     return -1;
   }
   if (!x || !b || !tol || !maxiter || !tol_s || !maxiter_s || !nsap || !nm) {
-    std::fprintf(stderr, "ERROR at %s : null argument\n", __func__);
+    std::fprintf(stderr, "ERROR at %s : null argument\n", where);
     return -2;
   }
   if (*nsap < 1 || *nsap > 10 || *nm < 1 || *nm > 10) {
-    std::fprintf(stderr, "ERROR at %s : nsap and nm must be in 1..10\n", __func__);
+    std::fprintf(stderr, "ERROR at %s : nsap and nm must be in 1..10\n", where);
     return -2;
   }
   double res = -1;
-  int it = 0;
+  bool converged = false;
   try {
-    dd_mix_body(__func__, x, b, tol, &it, maxiter, tol_s, maxiter_s, nsap, nm, qws::P32, &res);
+    converged = dd_mix_body(where, x, b, tol, conviter, maxiter, tol_s, maxiter_s, nsap, nm, sap_prec, &res);
   } catch (const std::exception& e) {
-    std::fprintf(stderr, "ERROR at %s : %s\n", __func__, e.what());
+    std::fprintf(stderr, "ERROR at %s : %s\n", where, e.what());
     return -3;
   }
-  if (conviter) *conviter = it;
   if (true_residual) *true_residual = res;
   if (!std::isfinite(res)) return 2;
-  // the recursive residual met tol within the iterations run (bicgstab_dd_mix.cc:249,336);
-  // a solve that ran out reports conviter = maxiter
-  return (*tol < 0 || it < *maxiter) ? 0 : 1;
+  return (*tol < 0 || converged) ? 0 : 1;
+}
+
+int bicgstab_dd_mix_status_(scd_t* x, scd_t* b, double* tol, int* conviter, int* maxiter, double* tol_s,
+                            int* maxiter_s, int* nsap, int* nm, double* true_residual) {
+  return dd_mix_status(__func__, x, b, tol, conviter, maxiter, tol_s, maxiter_s, nsap, nm, qws::P32, true_residual);
+}
+int bicgstab_dd_mix2_hf_status_(scd_t* x, const scd_t* b, const double* tol, int* conviter, const int* maxiter,
+                                const double* tol_s, const int* maxiter_s, const int* nsap, const int* nm,
+                                double* true_residual) {
+  return dd_mix_status(__func__, x, b, tol, conviter, maxiter, tol_s, maxiter_s, nsap, nm, qws::P16, true_residual);
 }
 
 // ---- half-precision variant (qws_h.cc, prec_ddd_s_h.h, bicgstab_precdd_s_h.cc, bicgstab_dd_mix2_hf.cc)

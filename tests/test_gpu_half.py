@@ -199,4 +199,18 @@ def test_dropin_hp_case0():
     L.ddd_d_(_ptr(r), _ptr(x))
     assert O.norm2(bd - r) < 1e-15
     assert K.check_value("xnorm^2", O.norm2(x), K.expected("CASE0", "xnorm^2", "DD"))
+    # the status variant: the same solution bit for bit; a fixed iteration count (tol < 0)
+    # is success; a tolerance two outer iterations cannot reach returns 1
+    L.bicgstab_dd_mix2_hf_status_.restype = ctypes.c_int
+    xs = RW.spinor_to_aosoa(x0, nloc, 8)
+    tr = ctypes.c_double(0)
+    rc = L.bicgstab_dd_mix2_hf_status_(_ptr(xs), _ptr(bd), cd(-1.0), ctypes.byref(it), ci(6), cd(-1.0), ci(50),
+                                       ci(4), ci(2), ctypes.byref(tr))
+    assert rc == 0 and np.array_equal(xs, x)
+    assert 0 <= tr.value < 1e-9 and abs(tr.value - np.sqrt(O.norm2(bd - r) / O.norm2(bd))) < 1e-12
+    xs = RW.spinor_to_aosoa(x0, nloc, 8)
+    # (inner solves to 0.1 only: two outer iterations cannot reach 1e-14)
+    rc = L.bicgstab_dd_mix2_hf_status_(_ptr(xs), _ptr(bd), cd(1e-14), ctypes.byref(it), ci(2), cd(0.1), ci(50),
+                                       ci(4), ci(2), ctypes.byref(tr))
+    assert rc == 1 and tr.value > 1e-10, (rc, it.value, tr.value)
     L.qws_finalize_()

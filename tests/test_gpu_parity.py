@@ -573,9 +573,10 @@ def test_dropin_abi_case0():
     assert 0 <= tr.value < 1e-9 and abs(tr.value - np.sqrt(O.norm2(bd - r) / O.norm2(bd))) < 1e-12
     # a tolerance 2 outer iterations do not meet: 1, conviter = maxiter
     xs = RW.spinor_to_aosoa(x0, nloc, 8)
-    rc = L.bicgstab_dd_mix_status_(_ptr(xs), _ptr(bd), cd(1e-14), ctypes.byref(its), ci(2), cd(1e-6), ci(50), ci(4),
+    # (inner solves to 0.1 only: two outer iterations cannot reach 1e-14)
+    rc = L.bicgstab_dd_mix_status_(_ptr(xs), _ptr(bd), cd(1e-14), ctypes.byref(its), ci(2), cd(0.1), ci(50), ci(4),
                                    ci(2), ctypes.byref(tr))
-    assert rc == 1 and its.value == 2 and tr.value > 1e-14
+    assert rc == 1 and its.value == 2 and tr.value > 1e-10, (rc, its.value, tr.value)
     rc = L.bicgstab_dd_mix_status_(_ptr(xs), _ptr(bd), cd(1e-14), ctypes.byref(its), ci(2), cd(1e-6), ci(50), ci(11),
                                    ci(2), ctypes.byref(tr))
     assert rc == -2 and tr.value == -1
