@@ -1,6 +1,6 @@
 """BQCD's host layouts, as QWS's loaders read them (qws_bqcd.cc:66-642).
 
-TEST INFRASTRUCTURE ONLY: tests/ use it as the checker; the product package
+TEST INFRASTRUCTURE ONLY: tests/ compare the product's results with it; the product package
 ``qws_amd`` never imports it.
 
 BQCD is not part of QWS: it passes its own arrays and numbers the sites of each

@@ -1,6 +1,6 @@
 """Restatement of QWS's even/odd Krylov solvers on a Hermitian positive operator.
 
-TEST INFRASTRUCTURE ONLY: tests/ use it as the checker; the product package
+TEST INFRASTRUCTURE ONLY: tests/ compare the product's results with it; the product package
 ``qws_amd`` never imports it.
 
 Both solvers work on real vectors (a parity field as its 24*V/2 reals) with an

@@ -1,7 +1,8 @@
 """ctypes wrapper of the CPU oracle (oracle/liboracle.so).
 
 TEST INFRASTRUCTURE ONLY: tests/, ``__graft_entry__.smoke()`` and the
-cpu_baseline leg of ``bench.py`` use this module as the checker.  The product
+cpu_baseline leg of ``bench.py`` use this module as the reference results to compare
+with.  The product
 package ``qws_amd`` never imports it.
 
 Arrays are numpy, in the oracle's plain site-major layout:

@@ -1,7 +1,7 @@
 // qws_oracle.cpp -- CPU restatement of QWS's domain-decomposed Wilson-clover
 // solve, written from the mathematics of the reference, not from its code.
 //
-// TEST INFRASTRUCTURE ONLY.  This library is the checker for the HIP product:
+// TEST INFRASTRUCTURE ONLY.  This library gives the results the HIP product is compared with:
 // only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may load
 // it.  Nothing in qws_amd/ links, loads or calls it.
 //
