@@ -162,6 +162,39 @@ def precision_variant(Q, kappa=None, budget_s=240.0):
             "reference_seconds": ref["seconds"]}
 
 
+def eo_operator(Q, lat, reps=20):
+    """The even/odd Schur operator mtilde = 1 - kappa^2 T_ee^-1 D_eo T_oo^-1 D_oe
+    (qws.cc:551-558, mtilde_vm_ / mtilde_s_) on the even sites of the workload lattice,
+    fields taken from the lattice's own: device time per application (host clock over
+    `reps` applications between synchronisations) and its HBM rate.  Algorithmic bytes
+    per even site: two hop applications, each reading the 8 links of an output site
+    (4 of its own, 4 of its neighbours), the clover inverse, the input and writing the
+    output; the second also reads in0 -- FP64 (18-real links): 2 x (8 x 144 + 576 + 192 +
+    192) + 192 = 4416 B; FP32 with the lattice's 12-real links: 2 x (8 x 48 + 288 + 96 +
+    96) + 96 = 1824 B (18-real: 2208 B)."""
+    lat.eo_from_lattice()
+    vh = lat.V // 2
+    nl32 = lat.links32
+    out = {"workload": "mtilde on the %d even sites of 32^3x64 (FP64 18-real, FP32 %d-real links; fields from "
+                       "the lattice)" % (vh, nl32)}
+    rng = np.random.default_rng(3)
+    x = rng.standard_normal((vh, 24))
+    bps32 = 2 * (8 * nl32 * 4 + 288 + 96 + 96) + 96
+    for prec, name, bps in ((Q.F64, "f64", 4416), (Q.F32, "f32", bps32)):
+        a = lat.eo_spinor(prec, x)
+        o = lat.eo_spinor(prec)
+        lat.mtilde(o, a)
+        lat.sync()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            lat.mtilde(o, a)
+        lat.sync()
+        ms = (time.perf_counter() - t0) * 1e3 / reps
+        gbs = bps * vh / (ms * 1e-3) / 1e9
+        out[name] = {"ms": ms, "GBs": gbs, "frac": gbs / HBM_PEAK_GBS, "bytes_per_site": bps}
+    return out
+
+
 SERVE_VOLUMES = ([8, 8, 8, 8], [16, 16, 16, 32], [24, 24, 24, 48], [32, 32, 32, 64])
 
 
@@ -237,6 +270,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--dslash-only", type=int, default=0, help="only time N dslash launches (for counter runs)")
+    ap.add_argument("--eo-only", type=int, default=0,
+                    help="only time N applications of the even/odd operator mtilde (FP64, FP32)")
     ap.add_argument("--sap-only", type=int, default=0,
                     help="only time N preconditioner applications of fused SAP launches (for counter runs)")
     ap.add_argument("--sweep", action="store_true", help="print solve statistics for a grid of solver settings")
@@ -290,6 +325,9 @@ def main():
     if a.dslash_only:
         ms = lat.time_dslash(Q.F32, a.dslash_only)
         print(json.dumps({"dslash_ms": ms, "GBs": V * BYTES_PER_SITE_F32_12 / (ms * 1e-3) / 1e9}))
+        return
+    if a.eo_only:
+        print(json.dumps(eo_operator(Q, lat, a.eo_only)))
         return
     if a.sap_only:
         ms = lat.time_sap(Q.F32, a.sap_only, SOLVE["nsap"], SOLVE["nm"])
@@ -358,6 +396,8 @@ def main():
         "cpu_baseline": None,
     }
     progress("plain solves and dslash done")
+    line["eo"] = eo_operator(Q, lat)
+    progress("even/odd operator done")
     if a.only_plain:
         print(json.dumps(line), flush=True)
         return

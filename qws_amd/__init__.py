@@ -254,6 +254,7 @@ class Lattice:
 
     def __init__(self, L, B, tbc=-1, links32=18, kappa=None):
         self.L, self.B = list(L), list(B)
+        self.links32 = int(links32)
         h = ctypes.c_void_p()
         Li = (ctypes.c_int * 4)(*L)
         Bi = (ctypes.c_int * 4)(*B)
