@@ -195,6 +195,83 @@ def eo_operator(Q, lat, reps=20):
     return out
 
 
+def qws_host_layout(a, kind, L, vlen):
+    """Plain fields (site x + Lx (y + Ly (z + Lz t))) -> QWS's host AoSoA (qws.h:111-192,
+    main.cc:426-429,485-488): x split into two domains of Lx/2, vector i = xv + nxv (y + Ly
+    (z + Lz t)), lane = x % vlen; spinors [i][col][spin][ri][lane], links
+    [i + vol mu + 4 vol d][row][col][ri][lane], clover [i + vol d][chirality][k][lane]."""
+    nx, ny, nz, nt = L
+    nxv = nx // 2 // vlen
+    if kind == "spinor":
+        p = np.asarray(a).reshape(nt, nz, ny, 2, nxv, vlen, 3, 4, 2)
+        return np.ascontiguousarray(p.transpose(3, 0, 1, 2, 4, 6, 7, 8, 5))
+    if kind == "links":
+        p = np.asarray(a).reshape(4, nt, nz, ny, 2, nxv, vlen, 3, 3, 2)
+        return np.ascontiguousarray(p.transpose(4, 0, 1, 2, 3, 5, 7, 8, 9, 6))
+    p = np.asarray(a).reshape(nt, nz, ny, 2, nxv, vlen, 2, 36)
+    return np.ascontiguousarray(p.transpose(3, 0, 1, 2, 4, 6, 7, 5))
+
+
+def dropin_solve(Q, reps=2):
+    """The workload through QWS's own entry point bicgstab_dd_mix_ (C ABI, host AoSoA
+    arrays in and out): fields written into glud/glus/clvd/clvs as a QWS driver does, 4^4
+    SAP blocks (qws_mi355x_set_sap_blocks_), the solve's settings without the restarts on
+    the true residual.  The first call also uploads the fields; the timed calls include
+    the host<->device copies and layout conversions of x and b (the PCIe-inclusive rate)."""
+    import ctypes
+    w = WORKLOAD
+    lat, b = setup(Q)
+    U, C, bp = lat.gauge(), lat.clover_field(), b.download()
+    lat.close()
+    del b
+    L = Q.lib()
+    nx, ny, nz, nt = w["L"]
+    ci = lambda v: ctypes.byref(ctypes.c_int(v))
+    cd = lambda v: ctypes.byref(ctypes.c_double(v))
+    keep = [(ctypes.c_int * 4)(1, 1, 1, 1), (ctypes.c_int * 4)(1, 1, 1, w["tbc"]), ctypes.c_int(0), ctypes.c_int(1),
+            (ctypes.c_int * 4)(1, 2, 2, 2)]
+    L.qws_init_(ci(nx), ci(ny), ci(nz), ci(nt), keep[0], keep[1], ctypes.byref(keep[2]), ctypes.byref(keep[3]),
+                keep[4])
+
+    def fill(name, arr):
+        dst = np.ctypeslib.as_array((ctypes.c_byte * arr.nbytes).from_address(ctypes.c_void_p.in_dll(L, name).value))
+        dst[:] = arr.view(np.int8).ravel()
+
+    fill("glud", qws_host_layout(U, "links", w["L"], 8))
+    fill("glus", qws_host_layout(U.astype(np.float32), "links", w["L"], 16))
+    del U
+    fill("clvd", qws_host_layout(C, "clover", w["L"], 8))
+    fill("clvs", qws_host_layout(C.astype(np.float32), "clover", w["L"], 16))
+    del C
+    for name, v in (("kappa", w["kappa"]), ("mkappa", -w["kappa"]), ("kappa2", w["kappa"] ** 2)):
+        ctypes.c_double.in_dll(L, name).value = v
+    L.qws_mi355x_fields_modified_()
+    L.qws_mi355x_set_sap_blocks_((ctypes.c_int * 4)(*w["B"]))
+    bh = qws_host_layout(bp, "spinor", w["L"], 8)
+    x = np.zeros_like(bh)
+    it = ctypes.c_int(0)
+    args = lambda: (x.ctypes.data_as(ctypes.c_void_p), bh.ctypes.data_as(ctypes.c_void_p), cd(SOLVE["tol"]),
+                    ctypes.byref(it), ci(SOLVE["maxiter"]), cd(SOLVE["tol_s"]), ci(SOLVE["maxiter_s"]),
+                    ci(SOLVE["nsap"]), ci(SOLVE["nm"]))
+    t0 = time.perf_counter()
+    L.bicgstab_dd_mix_(*args())
+    first = time.perf_counter() - t0
+    times = []
+    for _ in range(reps):
+        x[:] = 0
+        t0 = time.perf_counter()
+        L.bicgstab_dd_mix_(*args())
+        times.append(time.perf_counter() - t0)
+    r = np.zeros_like(bh)
+    L.ddd_d_(r.ctypes.data_as(ctypes.c_void_p), x.ctypes.data_as(ctypes.c_void_p))
+    res = float(np.sqrt(np.sum((bh - r) ** 2) / np.sum(bh ** 2)))
+    L.qws_finalize_()
+    t = float(np.mean(times))
+    return {"workload": "bicgstab_dd_mix_ through the QWS C ABI on the same 32^3x64 fields and source, host "
+                        "AoSoA x and b, 4^4 SAP blocks", "solves_per_s": 1.0 / t, "s_per_solve": t,
+            "first_call_s": first, "outer_iters": it.value, "true_residual": res}
+
+
 SERVE_VOLUMES = ([8, 8, 8, 8], [16, 16, 16, 32], [24, 24, 24, 48], [32, 32, 32, 64])
 
 
@@ -270,6 +347,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--dslash-only", type=int, default=0, help="only time N dslash launches (for counter runs)")
+    ap.add_argument("--dropin", action="store_true",
+                    help="the workload through bicgstab_dd_mix_ with host arrays (PCIe-inclusive rate)")
     ap.add_argument("--eo-only", type=int, default=0,
                     help="only time N applications of the even/odd operator mtilde (FP64, FP32)")
     ap.add_argument("--sap-only", type=int, default=0,
@@ -319,6 +398,9 @@ def main():
             res = srv.run(tr)
             print(json.dumps({"rate_req_per_s": rate, **serve_stats(res, tr)}), flush=True)
         srv.close()
+        return
+    if a.dropin:
+        print(json.dumps(dropin_solve(Q)))
         return
     lat, b = setup(Q)
     V = lat.V

@@ -9,6 +9,7 @@
 #include <cstddef>
 #include <cstdlib>
 #include <cstring>
+#include <thread>
 #include <type_traits>
 
 #include "fieldgen.h"
@@ -112,7 +113,71 @@ Lattice::~Lattice() {
     if (p) (void)hipFree(p);
   for (auto& w : work_) (void)hipFree(w.second.first);
   if (sc_host_) (void)hipHostFree(sc_host_);
+  for (int k = 0; k < 2; ++k) {
+    if (pin_[k]) (void)hipHostFree(pin_[k]);
+    if (pin_ev_[k]) (void)hipEventDestroy(pin_ev_[k]);
+  }
   if (st_) (void)hipStreamDestroy(st_);
+}
+
+// Host copies of the pinned-chunk transfers: split over a few threads (a single
+// thread's memcpy is well below the host's memory bandwidth and the link's rate).
+namespace {
+constexpr size_t PIN_CHUNK = (size_t)32 << 20;
+void par_memcpy(void* dst, const void* src, size_t n) {
+  constexpr int T = 8;
+  if (n < ((size_t)1 << 20)) {
+    std::memcpy(dst, src, n);
+    return;
+  }
+  std::thread th[T - 1];
+  const size_t part = (n / T + 63) & ~(size_t)63;
+  for (int i = 1; i < T; ++i) {
+    const size_t o = std::min(n, part * i), e = std::min(n, part * (i + 1));
+    th[i - 1] = std::thread([=] { std::memcpy((char*)dst + o, (const char*)src + o, e - o); });
+  }
+  std::memcpy(dst, src, std::min(n, part));
+  for (auto& t : th) t.join();
+}
+}  // namespace
+
+// Caller memory is pageable: it goes through two pinned chunks, the host copy of one
+// chunk overlapping the DMA of the other.  Both return with the copy complete.
+void Lattice::copy_to_device(void* dev, const void* host, size_t bytes) {
+  for (int k = 0; k < 2; ++k) {
+    if (!pin_[k]) QWS_HIP(hipHostMalloc(&pin_[k], PIN_CHUNK, hipHostMallocDefault));
+    if (!pin_ev_[k]) QWS_HIP(hipEventCreateWithFlags(&pin_ev_[k], hipEventDisableTiming));
+  }
+  int k = 0;
+  for (size_t off = 0; off < bytes; off += PIN_CHUNK, k ^= 1) {
+    const size_t n = std::min(PIN_CHUNK, bytes - off);
+    QWS_HIP(hipEventSynchronize(pin_ev_[k]));  // the DMA that last used this chunk is done
+    par_memcpy(pin_[k], (const char*)host + off, n);
+    QWS_HIP(hipMemcpyAsync((char*)dev + off, pin_[k], n, hipMemcpyHostToDevice, st_));
+    QWS_HIP(hipEventRecord(pin_ev_[k], st_));
+  }
+  QWS_HIP(hipStreamSynchronize(st_));
+}
+
+void Lattice::copy_to_host(void* host, const void* dev, size_t bytes) {
+  for (int k = 0; k < 2; ++k) {
+    if (!pin_[k]) QWS_HIP(hipHostMalloc(&pin_[k], PIN_CHUNK, hipHostMallocDefault));
+    if (!pin_ev_[k]) QWS_HIP(hipEventCreateWithFlags(&pin_ev_[k], hipEventDisableTiming));
+  }
+  const size_t nchunk = (bytes + PIN_CHUNK - 1) / PIN_CHUNK;
+  // DMA of chunk i+1 runs while chunk i is copied out of its pinned buffer
+  auto dma = [&](size_t i) {
+    const size_t off = i * PIN_CHUNK, n = std::min(PIN_CHUNK, bytes - off);
+    QWS_HIP(hipMemcpyAsync(pin_[i & 1], (const char*)dev + off, n, hipMemcpyDeviceToHost, st_));
+    QWS_HIP(hipEventRecord(pin_ev_[i & 1], st_));
+  };
+  if (nchunk) dma(0);
+  for (size_t i = 0; i < nchunk; ++i) {
+    QWS_HIP(hipEventSynchronize(pin_ev_[i & 1]));
+    if (i + 1 < nchunk) dma(i + 1);
+    const size_t off = i * PIN_CHUNK, n = std::min(PIN_CHUNK, bytes - off);
+    par_memcpy((char*)host + off, pin_[i & 1], n);
+  }
 }
 
 void* Lattice::work(const std::string& name, size_t bytes) {
@@ -293,7 +358,7 @@ void Lattice::ensure_half_fields() {
 void Lattice::upload_spinor(int prec, void* dev, const void* host, int layout, int host_is_double) {
   const size_t bytes = (size_t)g_.V * 24 * (host_is_double ? 8 : 4);
   void* stage = work("stage_spinor", bytes);
-  QWS_HIP(hipMemcpyAsync(stage, host, bytes, hipMemcpyHostToDevice, st_));
+  copy_to_device(stage, host, bytes);
   const int vlen = layout == 1 ? (host_is_double ? 8 : 16) : 1;
   if (prec == P64) launch_import<double>(g_, 0, layout, vlen, 0, stage, host_is_double, (double*)dev, st_);
   else if (prec == P32) launch_import<float>(g_, 0, layout, vlen, 0, stage, host_is_double, (float*)dev, st_);
@@ -308,8 +373,7 @@ void Lattice::download_spinor(int prec, const void* dev, void* host, int layout,
   if (prec == P64) launch_export<double>(g_, 0, layout, vlen, (const double*)dev, stage, host_is_double, st_);
   else if (prec == P32) launch_export<float>(g_, 0, layout, vlen, (const float*)dev, stage, host_is_double, st_);
   else launch_export<__half>(g_, 0, layout, vlen, (const __half*)dev, stage, host_is_double, st_);
-  QWS_HIP(hipMemcpyAsync(host, stage, bytes, hipMemcpyDeviceToHost, st_));
-  QWS_HIP(hipStreamSynchronize(st_));
+  copy_to_host(host, stage, bytes);
 }
 
 // ------------------------------------------------------------------ operators

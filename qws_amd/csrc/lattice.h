@@ -262,6 +262,11 @@ class Lattice {
   int sc_in_n_ = 0;
   Scalars* sc_one_ = nullptr;
   Scalars* sc_host_ = nullptr;  // pinned, room for MAX_BATCH
+  // host <-> device copies of caller memory (pageable) through two pinned chunks
+  void* pin_[2] = {nullptr, nullptr};
+  hipEvent_t pin_ev_[2] = {nullptr, nullptr};
+  void copy_to_device(void* dev, const void* host, size_t bytes);
+  void copy_to_host(void* host, const void* dev, size_t bytes);
   Scalars* sc_h_ = nullptr;     // FP16 SAP input norms, room for MAX_BATCH
   Scalars* sc_eo_ = nullptr;    // even/odd solver
   int* d_inner_mask_ = nullptr;

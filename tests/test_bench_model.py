@@ -56,3 +56,19 @@ def test_sap_flop_model_4444_value():
     h_face = 1.5 * 168 + 0.5 * 144
     assert enumerate_hops([4, 4, 4, 4]) == pytest.approx((h_in, h_face))
     assert bench.sap_flops_per_site([4, 4, 4, 4], 2, 6) == pytest.approx(11814.0)
+
+
+def test_bench_host_layout_is_the_reference_layout():
+    """bench.py's conversion to QWS's host AoSoA (for the drop-in measurement) equals the
+    layout the reference-library parity tests use (oracle/ref_worker.py)."""
+    import numpy as np
+
+    from oracle import ref_worker as RW
+    L = [32, 2, 2, 4]
+    V = int(np.prod(L))
+    rng = np.random.default_rng(9)
+    U, C, s = rng.standard_normal((4, V, 3, 3, 2)), rng.standard_normal((V, 2, 36)), rng.standard_normal((V, 3, 4, 2))
+    for vlen in (8, 16):
+        assert np.array_equal(bench.qws_host_layout(U, "links", L, vlen), RW.links_to_aosoa(U, L, vlen))
+        assert np.array_equal(bench.qws_host_layout(C, "clover", L, vlen), RW.clover_to_aosoa(C, L, vlen))
+        assert np.array_equal(bench.qws_host_layout(s, "spinor", L, vlen), RW.spinor_to_aosoa(s, L, vlen))
