@@ -247,6 +247,8 @@ def dropin_solve(Q, reps=2):
         ctypes.c_double.in_dll(L, name).value = v
     L.qws_mi355x_fields_modified_()
     L.qws_mi355x_set_sap_blocks_((ctypes.c_int * 4)(*w["B"]))
+    if w["links32"] == 12:  # SU(3) links: 12-real FP32 storage on the GPU, as the workload
+        L.qws_mi355x_set_links12_(ci(1))
     bh = qws_host_layout(bp, "spinor", w["L"], 8)
     x = np.zeros_like(bh)
     it = ctypes.c_int(0)
@@ -268,7 +270,8 @@ def dropin_solve(Q, reps=2):
     L.qws_finalize_()
     t = float(np.mean(times))
     return {"workload": "bicgstab_dd_mix_ through the QWS C ABI on the same 32^3x64 fields and source, host "
-                        "AoSoA x and b, 4^4 SAP blocks", "solves_per_s": 1.0 / t, "s_per_solve": t,
+                        "AoSoA x and b, 4^4 SAP blocks, %d-real FP32 links" % w["links32"],
+            "solves_per_s": 1.0 / t, "s_per_solve": t,
             "first_call_s": first, "outer_iters": it.value, "true_residual": res}
 
 

@@ -229,6 +229,12 @@ void check_residual_s_(const scs_t* x, const scs_t* b, const int* iter, const fl
  * glud/glus/clvd/clvs themselves or want a GPU-sized SAP block. */
 void qws_mi355x_fields_modified_(void);
 void qws_mi355x_set_sap_blocks_(int* block_extents);
+/* No counterpart as a call in the reference (QWS chooses it at compile time,
+ * SU3_RECONSTRUCT_S = 12, qws.h:60-61): *on != 0 stores the FP32 links on the GPU as
+ * their first two rows and rebuilds row 2 as conj(row0 x row1) -- valid only for SU(3)
+ * links; the host arrays keep all 18 reals.  Halves the LDS of the fused SAP kernel (two
+ * workgroups per CU).  Default off (QWS's own driver's random links are not SU(3)). */
+void qws_mi355x_set_links12_(int* on);
 
 /* ---------------------------------------------------------------- device-resident interface */
 typedef struct qwsg_lattice qwsg_lattice;

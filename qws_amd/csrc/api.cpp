@@ -379,6 +379,8 @@ struct DropIn {
   bool eo_dirty = true;  // the even/odd reading of the same host arrays
   bool default_blocks = true;
   int tbc = -1;
+  int blocks[4] = {0, 0, 0, 0};  // the SAP blocks of the device lattice
+  int links32 = 18;              // FP32 link storage on the device (12: row 2 rebuilt)
   int* pce = nullptr;
   int* pco = nullptr;
   void* dev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};  // d0 d1 s0 s1 pre_d pre_s
@@ -459,7 +461,8 @@ void release_drop_in() {
 void create_drop_in_lattice(const int B[4]) {
   int L[4] = {nx, ny, nz, nt};
   release_drop_in();
-  g_qws.lat = new Lattice(L, B, g_qws.tbc, 18);
+  for (int i = 0; i < 4; ++i) g_qws.blocks[i] = B[i];
+  g_qws.lat = new Lattice(L, B, g_qws.tbc, g_qws.links32);
   g_qws.dirty = true;
   g_qws.eo_dirty = true;
 }
@@ -596,6 +599,7 @@ void qws_init_(int* lx, int* ly, int* lz, int* lt, int* npe_f, int* fbc_f, int* 
   clvs = (clvs_t*)alloc(sizeof(clvs_t) * (size_t)vols * 2);
   int B[4] = {nxh, ny, nz, nt};
   g_qws.default_blocks = true;
+  g_qws.links32 = 18;
   drop_in(__func__, [&] { create_drop_in_lattice(B); });
 }
 
@@ -620,6 +624,13 @@ void qws_mi355x_set_sap_blocks_(int* b) {
   if (!g_qws.lat) fatal(__func__, "qws_init_ has not been called");
   int B[4] = {b[0], b[1], b[2], b[3]};
   g_qws.default_blocks = (B[0] == nxh && B[1] == ny && B[2] == nz && B[3] == nt);
+  drop_in(__func__, [&] { create_drop_in_lattice(B); });
+}
+
+void qws_mi355x_set_links12_(int* on) {
+  if (!g_qws.lat) fatal(__func__, "qws_init_ has not been called");
+  g_qws.links32 = (on && *on) ? 12 : 18;
+  const int B[4] = {g_qws.blocks[0], g_qws.blocks[1], g_qws.blocks[2], g_qws.blocks[3]};
   drop_in(__func__, [&] { create_drop_in_lattice(B); });
 }
 

@@ -1217,7 +1217,14 @@ int Lattice::bicgstab_dd_mix(double* x, const double* b, const SolveParams& prm,
     if (inner_total) *inner_total = 0;
     return prm.maxiter;
   }
-  OuterBatch ob(*this, 1, prm);
+  // the solver's vectors persist between calls (eight spinors: allocating and freeing
+  // them per call costs more than moving x and b)
+  if (!batch_ || batch_->capacity() != 1) {
+    batch_.reset();
+    batch_.reset(new OuterBatch(*this, 1, prm));
+  }
+  OuterBatch& ob = *batch_;
+  ob.set_params(prm);
   ob.admit(0, b, x);
   while (ob.step() > 0) {
   }

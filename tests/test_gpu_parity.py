@@ -387,6 +387,59 @@ def test_dropin_remaining_entry_points_case0(capfd):
     assert O.norm2(bd - r) < 1e-22 * O.norm2(bd)
 
 
+def test_dropin_links12_matches_links18_on_su3_fields():
+    """qws_mi355x_set_links12_: with SU(3) links in the host arrays, the drop-in's FP32
+    operator, SAP preconditioner and mixed solve with 12-real device links (row 2
+    rebuilt) agree with the 18-real ones to FP32 rounding; the solve meets 1e-10."""
+    import bench
+    L = Q.lib()
+    nloc = [32, 8, 8, 8]
+    g = Q.Lattice(nloc, [4, 4, 4, 4], -1, 12, kappa=0.13)
+    g.generate_gauge(0.3, 5)
+    g.compute_clover(1.0)
+    U, C, b = g.gauge(), g.clover_field(), g.gaussian(Q.F64, 6).download()
+    g.close()
+    ci = lambda v: ctypes.byref(ctypes.c_int(v))
+    cd = lambda v: ctypes.byref(ctypes.c_double(v))
+    keep = [(ctypes.c_int * 4)(1, 1, 1, 1), (ctypes.c_int * 4)(1, 1, 1, -1), ctypes.c_int(0), ctypes.c_int(1),
+            (ctypes.c_int * 4)(1, 2, 2, 2)]
+    L.qws_init_(ci(32), ci(8), ci(8), ci(8), keep[0], keep[1], ctypes.byref(keep[2]), ctypes.byref(keep[3]), keep[4])
+
+    def fill(name, arr):
+        dst = np.ctypeslib.as_array((ctypes.c_byte * arr.nbytes).from_address(ctypes.c_void_p.in_dll(L, name).value))
+        dst[:] = arr.view(np.int8).ravel()
+
+    fill("glud", bench.qws_host_layout(U, "links", nloc, 8))
+    fill("glus", bench.qws_host_layout(U.astype(np.float32), "links", nloc, 16))
+    fill("clvd", bench.qws_host_layout(C, "clover", nloc, 8))
+    fill("clvs", bench.qws_host_layout(C.astype(np.float32), "clover", nloc, 16))
+    for name, v in (("kappa", 0.13), ("mkappa", -0.13), ("kappa2", 0.13 ** 2)):
+        ctypes.c_double.in_dll(L, name).value = v
+    L.qws_mi355x_fields_modified_()
+    L.qws_mi355x_set_sap_blocks_((ctypes.c_int * 4)(4, 4, 4, 4))
+    bd = bench.qws_host_layout(b, "spinor", nloc, 8)
+    bs = bench.qws_host_layout(b.astype(np.float32), "spinor", nloc, 16)
+    L.bicgstab_dd_mix_status_.restype = ctypes.c_int
+    res = {}
+    for on in (0, 1):
+        L.qws_mi355x_set_links12_(ci(on))
+        d = np.zeros_like(bs)
+        L.ddd_s_(_ptr(d), _ptr(bs))
+        m = np.zeros_like(bs)
+        L.prec_ddd_s_(_ptr(m), _ptr(bs), ci(2), ci(6))
+        x = np.zeros_like(bd)
+        it, tr = ctypes.c_int(0), ctypes.c_double(0)
+        rc = L.bicgstab_dd_mix_status_(_ptr(x), _ptr(bd), cd(1e-11), ctypes.byref(it), ci(40), cd(1e-3), ci(80),
+                                       ci(2), ci(6), ctypes.byref(tr))
+        res[on] = (d, m, x, rc, it.value, tr.value)
+    L.qws_finalize_()
+    a, b18 = res[1], res[0]
+    assert rel(a[0], b18[0]) < 1e-6
+    assert rel(a[1], b18[1]) < 1e-5
+    assert a[3] == 0 and b18[3] == 0 and a[5] < 1e-10 and b18[5] < 1e-10, (a[3:], b18[3:])
+    assert rel(a[2], b18[2]) < 1e-7
+
+
 def test_bicgstab_direction_update_in_the_sap_sweep_is_bitwise_the_separate_pass():
     """The inner BiCGStab's direction update p = r + beta (p - omega q) done by the first
     fused SAP sweep of the next preconditioner application gives the separate update
