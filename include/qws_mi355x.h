@@ -27,15 +27,16 @@ extern "C" {
 #endif
 
 /* ---------------------------------------------------------------- QWS host types
- * Same memory layout as qws.h:111-192 with VLEND = 8, VLENS = 16. */
+ * Same memory layout as qws.h:111-192 with VLEND = 8, VLENS = 16 and 18-real links
+ * (SU3_RECONSTRUCT_D/S = 18, qws.h:60-61). */
 #define QWS_VLEND 8
 #define QWS_VLENS 16
-typedef struct { double c[3][4][2][QWS_VLEND]; } scd_t;   /* qws.h:155-160 */
-typedef struct { float c[3][4][2][QWS_VLENS]; } scs_t;    /* qws.h:183-189 */
-typedef struct { double c[3][3][2][QWS_VLEND]; } glud_t;  /* qws.h:119-125 */
-typedef struct { float c[3][3][2][QWS_VLENS]; } glus_t;   /* qws.h:144-152 */
-typedef struct { double c[2][36][QWS_VLEND]; } clvd_t;    /* qws.h:127-130 */
-typedef struct { float c[2][36][QWS_VLENS]; } clvs_t;     /* qws.h:154-158 */
+typedef struct { double c[3][4][2][QWS_VLEND]; } scd_t;   /* qws.h:134-139 */
+typedef struct { float c[3][4][2][QWS_VLENS]; } scs_t;    /* qws.h:176-182 */
+typedef struct { double c[3][3][2][QWS_VLEND]; } glud_t;  /* qws.h:121-127 */
+typedef struct { float c[3][3][2][QWS_VLENS]; } glus_t;   /* qws.h:160-168 */
+typedef struct { double c[2][36][QWS_VLEND]; } clvd_t;    /* qws.h:129-132 */
+typedef struct { float c[2][36][QWS_VLENS]; } clvs_t;     /* qws.h:170-174 */
 
 /* ---------------------------------------------------------------- QWS drop-in: globals
  * Host mirrors of QWS's exported data symbols (qws.cc:74-124).  qws_init_
@@ -229,8 +230,9 @@ void check_residual_s_(const scs_t* x, const scs_t* b, const int* iter, const fl
  * glud/glus/clvd/clvs themselves or want a GPU-sized SAP block. */
 void qws_mi355x_fields_modified_(void);
 void qws_mi355x_set_sap_blocks_(int* block_extents);
-/* No counterpart as a call in the reference (QWS chooses it at compile time,
- * SU3_RECONSTRUCT_S = 12, qws.h:60-61): *on != 0 stores the FP32 links on the GPU as
+/* No counterpart as a call in the reference (QWS chooses it at compile time: its
+ * default SU3_RECONSTRUCT_S = 18 at qws.h:61, the 12-real layout at qws.h:164-166 is
+ * compiled out): *on != 0 stores the FP32 links on the GPU as
  * their first two rows and rebuilds row 2 as conj(row0 x row1) -- valid only for SU(3)
  * links; the host arrays keep all 18 reals.  Halves the LDS of the fused SAP kernel (two
  * workgroups per CU).  Default off (QWS's own driver's random links are not SU(3)). */
@@ -337,6 +339,12 @@ int qwsg_msap_h(qwsg_lattice* lat, void* out, const void* in, int nsap, int nm);
 /* bicgstab_precdd_s_h.cc:83 -- inner solve D x = b with the FP16-stored SAP, x0 = 0 (FP32 vectors) */
 int qwsg_bicgstab_precdd_h(qwsg_lattice* lat, void* x, const void* b, double tol, int maxiter, int nsap, int nm,
                            int* iters);
+/* bicgstab_dd_d.cc:64 / bicgstab_dd_s.cc:67 -- plain (unpreconditioned) BiCGStab on D
+ * with FP64 or FP32 vectors (FP64 reductions), x holds the initial guess; stops when
+ * sqrt(|r|^2/|b|^2) < tol or after maxiter-1 iterations and returns the last one in
+ * *iters.  With FP64 vectors this is the all-FP64 solve that mixed-precision results
+ * are compared with. */
+int qwsg_bicgstab_dd(qwsg_lattice* lat, int prec, void* x, const void* b, int maxiter, double tol, int* iters);
 /* FP64 outer solve; x (FP64) holds the initial guess */
 int qwsg_solve(qwsg_lattice* lat, void* x, const void* b, const qwsg_solve_params* p, qwsg_solve_stats* st);
 /* No counterpart in the reference (it solves one source per call): n independent

@@ -155,7 +155,13 @@ def run(inp):
     kappa = float(inp["kappa"])
     r = Ref(nloc, kappa)
     r.set_fields(inp["U"], inp["C"])
+    return run_ops(r, inp)
+
+
+def run_ops(r, inp):
+    """The operations inp["ops"] on the reference instance r (fields already set)."""
     lib = r.lib
+    kappa = ctypes.c_double.in_dll(lib, "kappa").value
     vols, vold = r.vols, r.vold
     b = inp["b"]
     nsap, nm = int(inp["nsap"]), int(inp["nm"])

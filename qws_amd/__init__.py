@@ -86,6 +86,7 @@ def lib():
                                   ctypes.c_double, ctypes.c_int, ip]
         L.qwsg_bicgstab_precdd_h.argtypes = [vp, vp, vp, ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_int, ip]
         L.qwsg_solve.argtypes = [vp, vp, vp, ctypes.POINTER(SolveParams), ctypes.POINTER(SolveStats)]
+        L.qwsg_bicgstab_dd.argtypes = [vp, ctypes.c_int, vp, vp, ctypes.c_int, ctypes.c_double, ip]
         L.qwsg_norm2.argtypes = [vp, ctypes.c_int, vp, ctypes.POINTER(ctypes.c_double)]
         L.qwsg_solve_batch.argtypes = [vp, ctypes.c_int, vp, vp, ctypes.POINTER(SolveParams), ctypes.POINTER(SolveStats)]
         L.qwsg_synchronize.argtypes = [vp]
@@ -419,6 +420,14 @@ class Lattice:
         assert x.prec == F32 and b.prec == F32
         it = ctypes.c_int()
         check(lib().qwsg_bicgstab_precdd_h(self.h, x.ptr, b.ptr, float(tol), maxiter, nsap, nm, ctypes.byref(it)))
+        return it.value
+
+    def bicgstab_dd(self, x, b, maxiter, tol):
+        """Plain BiCGStab on D (bicgstab_dd_d.cc:64) from the initial guess in x; with
+        F64 vectors the all-FP64 solve.  Returns the last iteration."""
+        assert x.prec == b.prec
+        it = ctypes.c_int()
+        check(lib().qwsg_bicgstab_dd(self.h, x.prec, x.ptr, b.ptr, int(maxiter), float(tol), ctypes.byref(it)))
         return it.value
 
     def solve(self, x, b, nsap=4, nm=2, tol=-1.0, maxiter=6, tol_s=-1.0, maxiter_s=50, true_tol=-1.0,

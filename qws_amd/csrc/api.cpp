@@ -164,6 +164,13 @@ int qwsg_bicgstab_precdd_h(qwsg_lattice* h, void* x, const void* b, double tol, 
     if (iters) *iters = it;
   });
 }
+int qwsg_bicgstab_dd(qwsg_lattice* h, int prec, void* x, const void* b, int maxiter, double tol, int* iters) {
+  return guarded([&] {
+    if (prec != qws::P64 && prec != qws::P32) throw std::invalid_argument("bicgstab_dd: FP64 or FP32 vectors");
+    const int it = h->lat->bicgstab_dd(prec, x, b, maxiter, tol);
+    if (iters) *iters = it;
+  });
+}
 int qwsg_solve(qwsg_lattice* h, void* x, const void* b, const qwsg_solve_params* p, qwsg_solve_stats* st) {
   return guarded([&] {
     qws::SolveParams sp;
@@ -176,7 +183,7 @@ int qwsg_solve(qwsg_lattice* h, void* x, const void* b, const qwsg_solve_params*
     sp.true_tol = p->true_tol;
     sp.max_restarts = p->max_restarts;
     sp.sap_prec = p->sap_precision;
-    if (sp.nsap > 10 || sp.nm > 10) throw std::invalid_argument("nsap and nm must be <= 10");
+    qws::check_sap_params(sp.nsap, sp.nm);
     qws::SolveStats s = h->lat->solve((double*)x, (const double*)b, sp);
     if (st) {
       st->outer_iters = s.outer_iters;
@@ -201,6 +208,7 @@ int qwsg_solve_batch(qwsg_lattice* h, int n, void* const* x, const void* const* 
     sp.true_tol = p->true_tol;
     sp.max_restarts = p->max_restarts;
     sp.sap_prec = p->sap_precision;
+    qws::check_sap_params(sp.nsap, sp.nm);
     std::vector<qws::SolveStats> s =
         h->lat->solve_batch((double* const*)x, (const double* const*)b, n, sp);
     for (int k = 0; k < n && st; ++k) {
@@ -276,7 +284,7 @@ int qwsg_server_create(const qwsg_field_spec* fields, int nfields, int links32, 
     cfg.prm.true_tol = p->true_tol;
     cfg.prm.max_restarts = p->max_restarts;
     cfg.prm.sap_prec = p->sap_precision;
-    if (cfg.prm.nsap > 10 || cfg.prm.nm > 10) throw std::invalid_argument("nsap and nm must be <= 10");
+    qws::check_sap_params(cfg.prm.nsap, cfg.prm.nm);
     *out = new qwsg_server{new qws::Server(fs, cfg)};
   });
 }
@@ -810,6 +818,9 @@ static bool dd_mix_body(const char* where, scd_t* x, const scd_t* b, const doubl
 static void dd_mix_generic(const char* where, scd_t* x, const scd_t* b, const double* tol, int* conviter,
                            const int* maxiter, const double* tol_s, const int* maxiter_s, const int* nsap, const int* nm,
                            int sap_prec) {
+  // the reference's inner solve ends the process in prec_ddd_s_ (qws.cc:798-799)
+  if (*nsap > 10) { std::printf("prec_ddd_s_ nsap > 10\n"); std::exit(1); }
+  if (*nm > 10) { std::printf("prec_ddd_s_ nn   > 10\n"); std::exit(1); }
   drop_in(where, [&] { dd_mix_body(where, x, b, tol, conviter, maxiter, tol_s, maxiter_s, nsap, nm, sap_prec, nullptr); });
 }
 

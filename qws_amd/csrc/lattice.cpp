@@ -13,6 +13,7 @@
 #include <type_traits>
 
 #include "fieldgen.h"
+#include "hostcopy.h"
 #include "sap.h"
 
 namespace qws {
@@ -72,6 +73,7 @@ Lattice::Lattice(const int L[4], const int B[4], int tbc, int links32) {
   g_.dslash_kernel = dk ? std::atoi(dk) : 4;
   if (g_.dslash_kernel != 0 && g_.dslash_kernel != 3) g_.dslash_kernel = 4;
   nl32_ = links32;
+  pin_chunk_ = pin_chunk_from_env();
   QWS_HIP(hipStreamCreate(&st_));
   // Launch order of the blocks: index order.  (A Morton order of the block
   // coordinates was measured slower for D on 32^3x64: consecutive workgroups then
@@ -120,37 +122,18 @@ Lattice::~Lattice() {
   if (st_) (void)hipStreamDestroy(st_);
 }
 
-// Host copies of the pinned-chunk transfers: split over a few threads (a single
-// thread's memcpy is well below the host's memory bandwidth and the link's rate).
-namespace {
-constexpr size_t PIN_CHUNK = (size_t)32 << 20;
-void par_memcpy(void* dst, const void* src, size_t n) {
-  constexpr int T = 8;
-  if (n < ((size_t)1 << 20)) {
-    std::memcpy(dst, src, n);
-    return;
-  }
-  std::thread th[T - 1];
-  const size_t part = (n / T + 63) & ~(size_t)63;
-  for (int i = 1; i < T; ++i) {
-    const size_t o = std::min(n, part * i), e = std::min(n, part * (i + 1));
-    th[i - 1] = std::thread([=] { std::memcpy((char*)dst + o, (const char*)src + o, e - o); });
-  }
-  std::memcpy(dst, src, std::min(n, part));
-  for (auto& t : th) t.join();
-}
-}  // namespace
+
 
 // Caller memory is pageable: it goes through two pinned chunks, the host copy of one
 // chunk overlapping the DMA of the other.  Both return with the copy complete.
 void Lattice::copy_to_device(void* dev, const void* host, size_t bytes) {
   for (int k = 0; k < 2; ++k) {
-    if (!pin_[k]) QWS_HIP(hipHostMalloc(&pin_[k], PIN_CHUNK, hipHostMallocDefault));
+    if (!pin_[k]) QWS_HIP(hipHostMalloc(&pin_[k], pin_chunk_, hipHostMallocDefault));
     if (!pin_ev_[k]) QWS_HIP(hipEventCreateWithFlags(&pin_ev_[k], hipEventDisableTiming));
   }
   int k = 0;
-  for (size_t off = 0; off < bytes; off += PIN_CHUNK, k ^= 1) {
-    const size_t n = std::min(PIN_CHUNK, bytes - off);
+  for (size_t off = 0; off < bytes; off += pin_chunk_, k ^= 1) {
+    const size_t n = std::min(pin_chunk_, bytes - off);
     QWS_HIP(hipEventSynchronize(pin_ev_[k]));  // the DMA that last used this chunk is done
     par_memcpy(pin_[k], (const char*)host + off, n);
     QWS_HIP(hipMemcpyAsync((char*)dev + off, pin_[k], n, hipMemcpyHostToDevice, st_));
@@ -161,13 +144,13 @@ void Lattice::copy_to_device(void* dev, const void* host, size_t bytes) {
 
 void Lattice::copy_to_host(void* host, const void* dev, size_t bytes) {
   for (int k = 0; k < 2; ++k) {
-    if (!pin_[k]) QWS_HIP(hipHostMalloc(&pin_[k], PIN_CHUNK, hipHostMallocDefault));
+    if (!pin_[k]) QWS_HIP(hipHostMalloc(&pin_[k], pin_chunk_, hipHostMallocDefault));
     if (!pin_ev_[k]) QWS_HIP(hipEventCreateWithFlags(&pin_ev_[k], hipEventDisableTiming));
   }
-  const size_t nchunk = (bytes + PIN_CHUNK - 1) / PIN_CHUNK;
+  const size_t nchunk = (bytes + pin_chunk_ - 1) / pin_chunk_;
   // DMA of chunk i+1 runs while chunk i is copied out of its pinned buffer
   auto dma = [&](size_t i) {
-    const size_t off = i * PIN_CHUNK, n = std::min(PIN_CHUNK, bytes - off);
+    const size_t off = i * pin_chunk_, n = std::min(pin_chunk_, bytes - off);
     QWS_HIP(hipMemcpyAsync(pin_[i & 1], (const char*)dev + off, n, hipMemcpyDeviceToHost, st_));
     QWS_HIP(hipEventRecord(pin_ev_[i & 1], st_));
   };
@@ -175,7 +158,7 @@ void Lattice::copy_to_host(void* host, const void* dev, size_t bytes) {
   for (size_t i = 0; i < nchunk; ++i) {
     QWS_HIP(hipEventSynchronize(pin_ev_[i & 1]));
     if (i + 1 < nchunk) dma(i + 1);
-    const size_t off = i * PIN_CHUNK, n = std::min(PIN_CHUNK, bytes - off);
+    const size_t off = i * pin_chunk_, n = std::min(pin_chunk_, bytes - off);
     par_memcpy((char*)host + off, pin_[i & 1], n);
   }
 }
@@ -1096,9 +1079,19 @@ void Lattice::approx_inv(const Batch& B, Scalars* sc, const std::vector<int>& li
     if (live[k]) inner[k] += it[k];
 }
 
+void check_sap_params(int nsap, int nm) {
+  if (nsap < 0 || nsap > 10 || nm < 0 || nm > 10)
+    throw std::invalid_argument("nsap and nm must be in 0..10 (qws.cc:798-799)");
+}
+
+void OuterBatch::set_params(const SolveParams& p) {
+  check_sap_params(p.nsap, p.nm);
+  prm_ = p;
+}
+
 OuterBatch::OuterBatch(Lattice& L, int capacity, const SolveParams& p) : L_(L), cap_(capacity), prm_(p) {
   if (cap_ < 1 || cap_ > MAX_BATCH) throw std::invalid_argument("batch capacity must be in 1..64");
-  if (p.nsap > 10 || p.nm > 10) throw std::invalid_argument("nsap and nm must be <= 10 (qws.cc:798-799)");
+  check_sap_params(p.nsap, p.nm);
   stride_ = L.volume() * 24;
   const size_t bytes = sizeof(double) * (size_t)stride_ * cap_;
   double** bufs[] = {&x_, &b_, &q_, &r_, &p_, &t_, &u_, &r0_};

@@ -47,6 +47,10 @@ struct SolveParams {
   int sap_prec = P32;
 };
 
+// nsap and nm of QWS's SAP must not exceed 10 (prec_ddd_s_ exits otherwise,
+// qws.cc:798-799) nor be negative; throws std::invalid_argument
+void check_sap_params(int nsap, int nm);
+
 struct SolveStats {
   int outer_iters = 0, inner_iters = 0, restarts = 0;
   double true_residual = -1, seconds = 0;
@@ -64,7 +68,8 @@ class OuterBatch {
   OuterBatch(Lattice& L, int capacity, const SolveParams& p);
   ~OuterBatch();
   int capacity() const { return cap_; }
-  void set_params(const SolveParams& p) { prm_ = p; }
+  // rejects nsap or nm outside 0..10 (qws.cc:798-799), as the constructor does
+  void set_params(const SolveParams& p);
   // start slot k on source b (device FP64 spinor) with initial guess x0 (nullptr: zero)
   void admit(int k, const double* b, const double* x0);
   // one outer iteration of every running slot; returns the number still running
@@ -264,6 +269,7 @@ class Lattice {
   Scalars* sc_host_ = nullptr;  // pinned, room for MAX_BATCH
   // host <-> device copies of caller memory (pageable) through two pinned chunks
   void* pin_[2] = {nullptr, nullptr};
+  size_t pin_chunk_ = (size_t)32 << 20;  // bytes per chunk (QWS_MI355X_PIN_CHUNK)
   hipEvent_t pin_ev_[2] = {nullptr, nullptr};
   void copy_to_device(void* dev, const void* host, size_t bytes);
   void copy_to_host(void* host, const void* dev, size_t bytes);

@@ -1,0 +1,94 @@
+"""The benchmark configurations at their full sizes, checked against the CPU oracle.
+
+32^3x64 plain solve (links exp(0.3 i H), Philox seed 2, kappa = 0.1365, c_sw = 1.0,
+antiperiodic t, 4^4 SAP blocks, 12-real FP32 links): the FP64 and FP32 operators and
+the fused SAP preconditioner against the oracle on the whole lattice, then the mixed
+solve to an FP64 true residual of 1e-10, recomputed by the oracle's FP64 operator on
+the downloaded solution (gate: data/CASE0:61-63 and check.sh:17-26 check the final
+solution of the reference's solve the same way).
+
+The 12 spin-colour point sources at the origin on the same field, one batched solve:
+every solution's residual recomputed by the oracle, and two of them equal to their
+single solves bit for bit.
+"""
+import numpy as np
+import pytest
+
+import qws_amd as Q
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+C2 = dict(L=[32, 32, 32, 64], B=[4, 4, 4, 4], eps=0.3, seed=2, kappa=0.1365, csw=1.0)
+SOLVE = dict(nsap=2, nm=6, tol=1e-11, maxiter=40, tol_s=1e-3, maxiter_s=80, true_tol=1e-10, max_restarts=4)
+
+
+def rel(a, b):
+    return float(np.linalg.norm((a - b).ravel()) / np.linalg.norm(b.ravel()))
+
+
+def oracle_residual(orc, x, b):
+    return float(np.sqrt(O.norm2(b - orc.dslash(x)) / O.norm2(b)))
+
+
+@pytest.fixture(scope="module")
+def c2():
+    print("[configs] 32^3x64 fields", flush=True)
+    lat = Q.Lattice(C2["L"], C2["B"], -1, 12, kappa=C2["kappa"])
+    lat.generate_gauge(C2["eps"], C2["seed"])
+    lat.compute_clover(C2["csw"])
+    orc = O.Oracle(C2["L"], C2["B"], -1, C2["kappa"], lat.gauge(), lat.clover_field())
+    yield lat, orc
+    lat.close()
+
+
+def test_c2_operators_full_lattice(c2):
+    lat, orc = c2
+    b = lat.gaussian(Q.F64, 2).download()
+    for prec, dt, tol in ((Q.F64, np.float64, 1e-13), (Q.F32, np.float32, 1e-6)):
+        src = b.astype(dt)
+        out = lat.spinor(prec)
+        lat.dslash(out, lat.spinor(prec, src))
+        assert rel(out.download(), orc.dslash(src)) < tol, prec
+    # the fused FP32 SAP preconditioner of the solve (prec_ddd_s_ with nsap = 2, nm = 6)
+    src = b.astype(np.float32)
+    out = lat.spinor(Q.F32)
+    lat.prec_ddd(out, lat.spinor(Q.F32, src), SOLVE["nsap"], SOLVE["nm"])
+    assert rel(out.download(), orc.prec_ddd_s(src, SOLVE["nsap"], SOLVE["nm"])) < 5e-6
+
+
+def test_c2_solve_to_1e10(c2):
+    lat, orc = c2
+    b = lat.gaussian(Q.F64, C2["seed"])
+    x = lat.spinor(Q.F64)
+    st = lat.solve(x, b, **SOLVE)
+    assert st["true_residual"] <= 1e-10 and st["outer_iters"] >= 1
+    res = oracle_residual(orc, x.download(), b.download())
+    assert res <= 1e-10, res
+    # the library's own FP64 residual agrees with the oracle's
+    assert abs(res - st["true_residual"]) <= 1e-3 * res + 1e-15
+
+
+def test_c4_twelve_point_sources_one_batch(c2):
+    lat, orc = c2
+    bs = [lat.point(Q.F64, (0, 0, 0, 0), sp, c) for c in range(3) for sp in range(4)]
+    xs = [lat.spinor(Q.F64) for _ in bs]
+    st = lat.solve_batch(xs, bs, **SOLVE)
+    sols = []
+    for k in range(12):
+        x, b = xs[k].download(), bs[k].download()
+        assert st[k]["true_residual"] <= 1e-10, k
+        res = oracle_residual(orc, x, b)
+        assert res <= 1e-10, (k, res)
+        sols.append(x)
+    # a point source's solution is a column of D^-1: the 12 are linearly independent
+    # (Gram matrix of full rank), and two of them equal their single solves bit for bit
+    S = np.stack([a.ravel() for a in sols])
+    w = np.linalg.eigvalsh(S @ S.T)
+    del S
+    assert w.min() > 1e-8 * w.max()
+    for k in (0, 7):
+        x1 = lat.spinor(Q.F64)
+        s1 = lat.solve(x1, bs[k], **SOLVE)
+        assert np.array_equal(x1.download(), sols[k]), k
+        assert s1["inner_iters"] == st[k]["inner_iters"]

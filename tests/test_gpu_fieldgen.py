@@ -107,3 +107,21 @@ def test_generated_fields_drive_the_operator(gen):
     out = gen.spinor(Q.F32)
     gen.dslash(out, gen.spinor(Q.F32, b))
     assert rel(out.download(), orc.dslash(b.astype(np.float32))) < 1e-6
+
+
+@pytest.mark.parametrize("n,kappa,csw", [(1, 0.13, 2.0), (3, 0.12, 1.5)])
+def test_clover_constant_abelian_field_closed_form(n, kappa, csw):
+    """The GPU's clover inverse on a constant x-y field strength has the spectrum
+    1 -+ kappa csw sin(phi q) by spin (closed form, tests/clover_field.py): pins the
+    normalisation and sign of the clover term, which gauge covariance and the unit
+    gauge leave open."""
+    from tests import clover_field as CF
+
+    Lg, Bg = [4, 4, 4, 8], [2, 2, 2, 4]
+    U, phi = CF.abelian_links(Lg, n)
+    lat = Q.Lattice(Lg, Bg, -1, 18, kappa=kappa)
+    lat.load_gauge(U)
+    lat.compute_clover(csw)
+    C = lat.clover_field()
+    want = CF.expected_packed_clover(int(np.prod(Lg)), phi, kappa, csw)
+    assert np.abs(C - want).max() < 1e-14

@@ -480,3 +480,22 @@ def test_bqcd_layouts():
         h = he if eo[s] == 0 else ho
         assert h[ri + 2 * sp + 8 * c + 24 * (i[s] - 1)] == psi[s, c, sp, ri]
     assert np.array_equal(BQ.spinor_from(he, ho, nloc), psi)
+
+
+def test_clover_constant_abelian_field_closed_form():
+    """The oracle's clover-from-links on a constant x-y field strength: T's spectrum is
+    1 -+ kappa csw sin(phi q) by spin, known in closed form (tests/clover_field.py), so a
+    wrong factor of F (e.g. (Q - Q^+)/4) or a flipped sign of F or sigma fails."""
+    from oracle import fieldgen as FG
+    from tests import clover_field as CF
+
+    L = [4, 4, 2, 2]
+    for n, kappa, csw in ((1, 0.13, 2.0), (3, 0.12, 1.5)):
+        U, phi = CF.abelian_links(L, n)
+        packed, off, T = FG.clover_inverse(L, U, kappa, csw)
+        want = CF.expected_packed_clover(int(np.prod(L)), phi, kappa, csw)
+        assert off < 1e-15
+        assert np.abs(packed - want).max() < 1e-14
+        # the factor and the sign matter: both alternatives are far off
+        assert np.abs(CF.expected_packed_clover(int(np.prod(L)), phi, kappa, 2 * csw) - packed).max() > 1e-2
+        assert np.abs(CF.expected_packed_clover(int(np.prod(L)), -phi, kappa, csw) - packed).max() > 1e-2
