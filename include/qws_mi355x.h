@@ -383,6 +383,10 @@ int qwsg_eo_cg(qwsg_lattice* lat, void* x, const void* b, int maxiter, double to
 int qwsg_eo_mcg(qwsg_lattice* lat, void* const* xs, const void* b, int nshift, const double* sigma, int maxiter,
                 double tol, int pce, int* iters);
 int qwsg_norm2(qwsg_lattice* lat, int prec, const void* a, double* out);
+/* No counterpart in the reference: an order-independent 64-bit digest of a device
+ * spinor's bytes, the digest a served request reports for its solutions (XOR over the
+ * solutions of their digests); equal for bit-identical spinors. */
+int qwsg_spinor_digest(qwsg_lattice* lat, int prec, const void* a, unsigned long long* out);
 
 /* No counterpart in the reference (it solves one source per call, serially): a solve
  * server on one GPU.  The gauge fields stay resident; each request names a field, a

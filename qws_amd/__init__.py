@@ -158,6 +158,12 @@ class Spinor:
         check(lib().qwsg_spinor_zero(self.lat.h, self.prec, self.ptr))
         return self
 
+    def digest(self):
+        """Order-independent digest of the spinor's bytes (equal for bit-identical data)."""
+        d = ctypes.c_ulonglong()
+        check(lib().qwsg_spinor_digest(self.lat.h, self.prec, self.ptr, ctypes.byref(d)))
+        return d.value
+
     def norm2(self):
         v = ctypes.c_double()
         check(lib().qwsg_norm2(self.lat.h, self.prec, self.ptr, ctypes.byref(v)))

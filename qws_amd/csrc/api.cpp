@@ -339,6 +339,9 @@ int qwsg_eo_mcg(qwsg_lattice* h, void* const* xs, const void* b, int nshift, con
 int qwsg_norm2(qwsg_lattice* h, int prec, const void* a, double* out) {
   return guarded([&] { *out = h->lat->norm2(prec, a); });
 }
+int qwsg_spinor_digest(qwsg_lattice* h, int prec, const void* a, unsigned long long* out) {
+  return guarded([&] { *out = h->lat->digest(prec, a); });
+}
 int qwsg_synchronize(qwsg_lattice* h) {
   return guarded([&] { h->lat->sync(); });
 }

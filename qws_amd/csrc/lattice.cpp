@@ -642,6 +642,17 @@ double Lattice::norm2(int prec, const void* a) {
   return read_scalar(sc_one_, offsetof(Scalars, red));
 }
 
+unsigned long long Lattice::digest(int prec, const void* a) {
+  const size_t words = spinor_bytes(prec) / 8;
+  unsigned long long* d = (unsigned long long*)work("digest", sizeof(unsigned long long));
+  QWS_HIP(hipMemsetAsync(d, 0, sizeof(unsigned long long), st_));
+  launch_digest((long long)words, a, d, st_);
+  unsigned long long h = 0;
+  QWS_HIP(hipMemcpyAsync(&h, d, sizeof(h), hipMemcpyDeviceToHost, st_));
+  QWS_HIP(hipStreamSynchronize(st_));
+  return h;
+}
+
 // ------------------------------------------------------------------ even/odd family
 void Lattice::eo_load_gauge(const void* U, int layout, int vlen, int is_double, int target) {
   const size_t bytes = (size_t)4 * g_.V * 18 * (is_double ? 8 : 4);

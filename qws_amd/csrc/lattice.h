@@ -152,6 +152,9 @@ class Lattice {
   // n independent right-hand sides solved together (x[k] holds the initial guess)
   std::vector<SolveStats> solve_batch(double* const* x, const double* const* b, int n, const SolveParams& p);
   double norm2(int prec, const void* a);
+  // order-independent digest of a device spinor's bytes (launch_digest): equal for
+  // bit-identical spinors, whatever produced them
+  unsigned long long digest(int prec, const void* a);
   // FP64 reductions over n reals to the host: |a|^2 (b == nullptr) or sum a_i b_i
   double reduce_real(long long n, const double* a, const double* b);
   double true_residual(const double* x, const double* b);

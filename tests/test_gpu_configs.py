@@ -92,3 +92,81 @@ def test_c4_twelve_point_sources_one_batch(c2):
         s1 = lat.solve(x1, bs[k], **SOLVE)
         assert np.array_equal(x1.download(), sols[k]), k
         assert s1["inner_iters"] == st[k]["inner_iters"]
+
+
+# ---- the uneven batch and the served mix, at their full volumes
+SERVE_VOLUMES = ([8, 8, 8, 8], [16, 16, 16, 32], [24, 24, 24, 48], [32, 32, 32, 64])
+SERVE_FIELDS = [(L, [4, 4, 4, 4], 0.3, 8 + i) for i, L in enumerate(SERVE_VOLUMES)]
+
+
+@pytest.fixture(scope="module")
+def server():
+    print("[configs] server with the four resident fields", flush=True)
+    s = Q.Server(SERVE_FIELDS, links32=12, slots=4, **SOLVE)
+    yield s
+    s.close()
+
+
+def check_request_with_oracle(req, res, nrhs_checked):
+    """Re-solve a served request on a lattice of its own (same links, clover for its
+    kappa, sources seed*1000+k): the XOR of the solutions' digests must equal the
+    request's digest (the served solutions are these, bit for bit), and the oracle's
+    FP64 residual of the first `nrhs_checked` solutions must be <= 1e-10."""
+    L, B, eps, fseed = SERVE_FIELDS[req["field"]]
+    lat = Q.Lattice(L, B, -1, 12, kappa=req["kappa"])
+    lat.generate_gauge(eps, fseed)
+    lat.compute_clover(1.0)
+    bs = [lat.gaussian(Q.F64, req["seed"] * 1000 + k) for k in range(req["nrhs"])]
+    xs = [lat.spinor(Q.F64) for _ in bs]
+    st = lat.solve_batch(xs, bs, **SOLVE)
+    dig = 0
+    for x in xs:
+        dig ^= x.digest()
+    assert dig == res["digest"], req
+    assert sum(s["inner_iters"] for s in st) == res["inner_iters"]
+    orc = O.Oracle(L, B, -1, req["kappa"], lat.gauge(), lat.clover_field())
+    for k in range(min(nrhs_checked, req["nrhs"])):
+        r = oracle_residual(orc, xs[k].download(), bs[k].download())
+        assert r <= 1e-10, (req, k, r)
+    del orc
+    lat.close()
+
+
+def test_c4_uneven_batch(server):
+    """16^3x32, 24^3x48 and 32^3x64 with one Gaussian source each (seeds 4-6),
+    kappa in [0.130, 0.137], submitted together: every solution meets 1e-10 (oracle)."""
+    rng = np.random.default_rng(4)
+    reqs = [dict(field=i, kappa=float(rng.uniform(0.130, 0.137)), nrhs=1, seed=4 + k, arrival=0.0)
+            for k, i in enumerate((1, 2, 3))]
+    res = server.run(reqs)
+    for q, r in zip(reqs, res):
+        print(f"[configs] uneven: field {q['field']} kappa {q['kappa']:.4f}", flush=True)
+        assert r["true_residual_max"] <= 1e-10
+        check_request_with_oracle(q, r, 1)
+
+
+def test_c5_served_mix_at_full_volumes(server):
+    """A Poisson trace over the four volumes, kappa in [0.125, 0.137], 1-12 RHS: each
+    request's result is the same alone and in the mix (digests), and one request per
+    volume is checked against the oracle."""
+    rng = np.random.default_rng(7)
+    fields = [0, 1, 2, 3, 1, 0, 2]
+    nrhs = [12, 3, 2, 1, 6, 5, 1]
+    t = np.cumsum(rng.exponential(1.0 / 20.0, len(fields)))
+    tr = [dict(field=f, kappa=float(rng.uniform(0.125, 0.137)), nrhs=n, seed=300 + i, arrival=float(t[i]))
+          for i, (f, n) in enumerate(zip(fields, nrhs))]
+    mix = server.run(tr)
+    for i, q in enumerate(tr):
+        assert mix[i]["true_residual_max"] <= 1e-10, q
+        alone = server.run([dict(q, arrival=0.0)])[0]
+        assert alone["digest"] == mix[i]["digest"], q
+        assert alone["inner_iters"] == mix[i]["inner_iters"], q
+    iv = sorted((m["admit"], m["finish"]) for m in mix)
+    assert any(iv[k + 1][0] < iv[k][1] for k in range(len(iv) - 1))  # some ran at the same time
+    seen = set()
+    for i, q in enumerate(tr):
+        if q["field"] in seen:
+            continue
+        seen.add(q["field"])
+        print(f"[configs] served: field {q['field']} nrhs {q['nrhs']} kappa {q['kappa']:.4f}", flush=True)
+        check_request_with_oracle(q, mix[i], 2)
