@@ -355,6 +355,16 @@ int qwsg_solve(qwsg_lattice* lat, void* x, const void* b, const qwsg_solve_param
  * to solving it alone.  st receives n entries. */
 int qwsg_solve_batch(qwsg_lattice* lat, int n, void* const* x, const void* const* b, const qwsg_solve_params* p,
                      qwsg_solve_stats* st);
+/* No counterpart in the reference: solves on nlat lattices of any volumes, fields and
+ * kappas at once (the SAP runs as the reference's bicgstab_dd_mix_ on each,
+ * bicgstab_dd_mix.cc:105-388).  Lattice i solves nrhs[i] right-hand sides (1..64);
+ * x, b and st hold them lattice after lattice (x holds the initial guesses).  The
+ * lattices advance in lockstep on the first lattice's stream: every SAP half-sweep
+ * and prec_ddd_s_ face term is one launch over the blocks of all of them (where
+ * their block volume and link storage agree), the vector updates and reductions run
+ * per lattice, so each result equals the lattice's own qwsg_solve_batch bit for bit. */
+int qwsg_solve_multi(int nlat, qwsg_lattice* const* lats, const int* nrhs, void* const* x, const void* const* b,
+                     const qwsg_solve_params* p, qwsg_solve_stats* st);
 /* Even/odd family on x-compressed parity fields of V/2 sites h = xh + (Lx/2)(y + Ly(z + Lz t))
  * (host layout: spinor [V/2][24]; links [2][4][V/2][18], link from site h of parity p;
  * clover [2][V/2][72]).  Fields come from host arrays or from the lattice's own
@@ -389,16 +399,19 @@ int qwsg_norm2(qwsg_lattice* lat, int prec, const void* a, double* out);
 int qwsg_spinor_digest(qwsg_lattice* lat, int prec, const void* a, unsigned long long* out);
 
 /* No counterpart in the reference (it solves one source per call, serially): a solve
- * server on one GPU.  The gauge fields stay resident; each request names a field, a
- * kappa and 1..64 Gaussian sources (source k uses seed*1000 + k).  qwsg_server_run
- * replays a trace in real time: a request is released at its arrival time, admitted in
- * arrival order when one of `slots` (1..16) is free and its memory estimate fits
- * `mem_budget` bytes (0: 90% of the free HBM), and runs on its own HIP stream at the
- * same time as the other admitted requests: clover inverse for its kappa on the shared
- * links, then one batched solve of all its sources.  Results carry arrival, admission
- * and finish times (seconds from the start of the trace), iteration counts, the
- * largest FP64 true residual and an order-independent digest of the solutions, which
- * is the same whether the request runs alone or among others. */
+ * server on one GPU with continuous batching.  The gauge fields stay resident; each
+ * request names a field, a kappa and 1..64 Gaussian sources (source k uses seed*1000 +
+ * k).  qwsg_server_run replays a trace in real time: a request is released at its
+ * arrival time and admitted in arrival order when fewer than `slots` (1..16) requests
+ * run and a pooled slot for it (a lattice sharing the field's links, the clover inverse
+ * for its kappa, its solver vectors) is free or fits `mem_budget` bytes (0: 90% of the
+ * free HBM).  All admitted requests advance together, one outer iteration at a time,
+ * every SAP half-sweep one launch over the blocks of all of them; a request that
+ * arrives meanwhile joins at the next outer iteration.  Slots are kept for later
+ * requests (nothing is allocated once the pool has warmed up).  Results carry arrival,
+ * admission and finish times (seconds from the start of the trace), iteration counts,
+ * the largest FP64 true residual and an order-independent digest of the solutions,
+ * which is the same whether the request runs alone or among others. */
 typedef struct qwsg_server qwsg_server;
 typedef struct {
   int L[4], B[4];
@@ -421,6 +434,10 @@ typedef struct {
 int qwsg_server_create(const qwsg_field_spec* fields, int nfields, int links32, int tbc, double csw,
                        const qwsg_solve_params* p, int slots, double mem_budget, qwsg_server** out);
 int qwsg_server_run(qwsg_server* srv, const qwsg_request* reqs, int nreq, qwsg_request_result* out);
+/* device memory (bytes) the server reserves for a request of nrhs sources on V sites:
+ * the pooled slot that serves it (clover inverse, solver vectors for the slot's RHS
+ * capacity: 1, 2, 4, 8, 12, 16, 32 or 64) */
+int qwsg_server_request_bytes(long long V, int nrhs, double* bytes);
 int qwsg_server_destroy(qwsg_server* srv);
 int qwsg_synchronize(qwsg_lattice* lat);
 /* average device time (ms) of `reps` full FP32/FP16/FP64 dslash launches on the

@@ -89,6 +89,9 @@ def lib():
         L.qwsg_bicgstab_dd.argtypes = [vp, ctypes.c_int, vp, vp, ctypes.c_int, ctypes.c_double, ip]
         L.qwsg_norm2.argtypes = [vp, ctypes.c_int, vp, ctypes.POINTER(ctypes.c_double)]
         L.qwsg_solve_batch.argtypes = [vp, ctypes.c_int, vp, vp, ctypes.POINTER(SolveParams), ctypes.POINTER(SolveStats)]
+        L.qwsg_server_request_bytes.argtypes = [ctypes.c_longlong, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
+        L.qwsg_solve_multi.argtypes = [ctypes.c_int, vp, vp, vp, vp, ctypes.POINTER(SolveParams),
+                                       ctypes.POINTER(SolveStats)]
         L.qwsg_synchronize.argtypes = [vp]
         L.qwsg_time_dslash.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
         L.qwsg_set_dslash_kernel.argtypes = [vp, ctypes.c_int]
@@ -220,6 +223,13 @@ class Server:
         if self.h:
             check(lib().qwsg_server_destroy(self.h))
             self.h = None
+
+    @staticmethod
+    def request_bytes(V, nrhs):
+        """Device memory the server reserves for a request of nrhs sources on V sites."""
+        b = ctypes.c_double()
+        check(lib().qwsg_server_request_bytes(int(V), int(nrhs), ctypes.byref(b)))
+        return b.value
 
     def __del__(self):
         try:
@@ -490,3 +500,27 @@ def device_count():
     n = ctypes.c_int()
     check(lib().qwsg_device_count(ctypes.byref(n)))
     return n.value
+
+
+def solve_multi(parts, nsap=4, nm=2, tol=-1.0, maxiter=6, tol_s=-1.0, maxiter_s=50, true_tol=-1.0,
+                max_restarts=0, sap_precision=F32):
+    """qwsg_solve_multi: parts = [(lattice, [x...], [b...]), ...]; the lattices solve
+    their right-hand sides at once, the SAP launches fused over them.  Returns the
+    stats per lattice and right-hand side."""
+    n = len(parts)
+    lats = (ctypes.c_void_p * n)(*[lat.h for lat, _, _ in parts])
+    nr = (ctypes.c_int * n)(*[len(xs) for _, xs, _ in parts])
+    xs = [x for _, xl, _ in parts for x in xl]
+    bs = [b for _, _, bl in parts for b in bl]
+    assert len(xs) == len(bs)
+    tot = len(xs)
+    p = SolveParams(nsap, nm, tol, maxiter, tol_s, maxiter_s, true_tol, max_restarts, sap_precision)
+    st = (SolveStats * tot)()
+    xp = (ctypes.c_void_p * tot)(*[x.ptr.value for x in xs])
+    bp = (ctypes.c_void_p * tot)(*[b.ptr.value for b in bs])
+    check(lib().qwsg_solve_multi(n, lats, nr, xp, bp, ctypes.byref(p), st))
+    out, at = [], 0
+    for _, xl, _ in parts:
+        out.append([{k: getattr(s, k) for k, _ in SolveStats._fields_} for s in st[at:at + len(xl)]])
+        at += len(xl)
+    return out

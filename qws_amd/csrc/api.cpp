@@ -10,41 +10,29 @@
 #include <cstring>
 #include <ctime>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/qws_mi355x.h"
+#include "handle.h"
 #include "lattice.h"
+#include "multi.h"
 #include "serve.h"
 
 using qws::Lattice;
 
-struct qwsg_lattice {
-  Lattice* lat;
-};
-
-namespace {
-thread_local std::string g_err;
-
-template <typename F> int guarded(F&& f) {
-  try {
-    f();
-    return 0;
-  } catch (const qws::HipError& e) {
-    g_err = e.what();
-    return 2;
-  } catch (const std::invalid_argument& e) {
-    g_err = e.what();
-    return 1;
-  } catch (const std::exception& e) {
-    g_err = e.what();
-    return 3;
-  }
+namespace qws {
+std::string& last_error() {
+  static thread_local std::string err;
+  return err;
 }
-}  // namespace
+}  // namespace qws
+
+using qws::guarded;
 
 extern "C" {
 
-const char* qwsg_last_error(void) { return g_err.c_str(); }
+const char* qwsg_last_error(void) { return qws::last_error().c_str(); }
 
 int qwsg_device_count(int* n) {
   return guarded([&] { QWS_HIP(hipGetDeviceCount(n)); });
@@ -171,53 +159,85 @@ int qwsg_bicgstab_dd(qwsg_lattice* h, int prec, void* x, const void* b, int maxi
     if (iters) *iters = it;
   });
 }
+namespace {
+qws::SolveParams solve_params(const qwsg_solve_params* p) {
+  if (!p) throw std::invalid_argument("no solve parameters");
+  qws::SolveParams sp;
+  sp.nsap = p->nsap;
+  sp.nm = p->nm;
+  sp.tol = p->tol;
+  sp.maxiter = p->maxiter;
+  sp.tol_s = p->tol_s;
+  sp.maxiter_s = p->maxiter_s;
+  sp.true_tol = p->true_tol;
+  sp.max_restarts = p->max_restarts;
+  sp.sap_prec = p->sap_precision;
+  qws::check_sap_params(sp.nsap, sp.nm);
+  return sp;
+}
+void put_stats(const qws::SolveStats& s, qwsg_solve_stats* st) {
+  st->outer_iters = s.outer_iters;
+  st->inner_iters = s.inner_iters;
+  st->restarts = s.restarts;
+  st->true_residual = s.true_residual;
+  st->seconds = s.seconds;
+}
+}  // namespace
+
 int qwsg_solve(qwsg_lattice* h, void* x, const void* b, const qwsg_solve_params* p, qwsg_solve_stats* st) {
   return guarded([&] {
-    qws::SolveParams sp;
-    sp.nsap = p->nsap;
-    sp.nm = p->nm;
-    sp.tol = p->tol;
-    sp.maxiter = p->maxiter;
-    sp.tol_s = p->tol_s;
-    sp.maxiter_s = p->maxiter_s;
-    sp.true_tol = p->true_tol;
-    sp.max_restarts = p->max_restarts;
-    sp.sap_prec = p->sap_precision;
-    qws::check_sap_params(sp.nsap, sp.nm);
+    const qws::SolveParams sp = solve_params(p);
     qws::SolveStats s = h->lat->solve((double*)x, (const double*)b, sp);
-    if (st) {
-      st->outer_iters = s.outer_iters;
-      st->inner_iters = s.inner_iters;
-      st->restarts = s.restarts;
-      st->true_residual = s.true_residual;
-      st->seconds = s.seconds;
-    }
+    if (st) put_stats(s, st);
   });
 }
 int qwsg_solve_batch(qwsg_lattice* h, int n, void* const* x, const void* const* b, const qwsg_solve_params* p,
                      qwsg_solve_stats* st) {
   return guarded([&] {
     if (n < 1 || n > qws::MAX_BATCH) throw std::invalid_argument("batch size must be in 1..64");
-    qws::SolveParams sp;
-    sp.nsap = p->nsap;
-    sp.nm = p->nm;
-    sp.tol = p->tol;
-    sp.maxiter = p->maxiter;
-    sp.tol_s = p->tol_s;
-    sp.maxiter_s = p->maxiter_s;
-    sp.true_tol = p->true_tol;
-    sp.max_restarts = p->max_restarts;
-    sp.sap_prec = p->sap_precision;
-    qws::check_sap_params(sp.nsap, sp.nm);
+    const qws::SolveParams sp = solve_params(p);
     std::vector<qws::SolveStats> s =
         h->lat->solve_batch((double* const*)x, (const double* const*)b, n, sp);
-    for (int k = 0; k < n && st; ++k) {
-      st[k].outer_iters = s[k].outer_iters;
-      st[k].inner_iters = s[k].inner_iters;
-      st[k].restarts = s[k].restarts;
-      st[k].true_residual = s[k].true_residual;
-      st[k].seconds = s[k].seconds;
+    for (int k = 0; k < n && st; ++k) put_stats(s[k], st + k);
+  });
+}
+int qwsg_solve_multi(int nlat, qwsg_lattice* const* lats, const int* nrhs, void* const* x, const void* const* b,
+                     const qwsg_solve_params* p, qwsg_solve_stats* st) {
+  return guarded([&] {
+    if (nlat < 1 || nlat > 64) throw std::invalid_argument("solve_multi: 1..64 lattices");
+    const qws::SolveParams sp = solve_params(p);
+    std::vector<Lattice*> L(nlat);
+    std::vector<std::vector<double*>> xs(nlat);
+    std::vector<std::vector<const double*>> bs(nlat);
+    int at = 0;
+    for (int i = 0; i < nlat; ++i) {
+      if (!lats[i] || !lats[i]->lat) throw std::invalid_argument("solve_multi: no lattice");
+      if (nrhs[i] < 1 || nrhs[i] > qws::MAX_BATCH) throw std::invalid_argument("solve_multi: 1..64 RHS per lattice");
+      L[i] = lats[i]->lat;
+      for (int j = 0; j < i; ++j)
+        if (L[j] == L[i]) throw std::invalid_argument("solve_multi: a lattice appears twice");
+      for (int k = 0; k < nrhs[i]; ++k, ++at) {
+        xs[i].push_back((double*)x[at]);
+        bs[i].push_back((const double*)b[at]);
+      }
     }
+    // every lattice works on the first one's stream for the call
+    hipStream_t s0 = L[0]->stream();
+    for (Lattice* l : L) l->sync();
+    struct Restore {
+      std::vector<Lattice*>& L;
+      ~Restore() {
+        (void)hipStreamSynchronize(L[0]->stream());
+        for (size_t i = 1; i < L.size(); ++i) L[i]->set_stream(nullptr);
+      }
+    } restore{L};
+    for (Lattice* l : L) l->set_stream(s0);
+    qws::SapFusion fz(s0);
+    const auto res = qws::Fused::solve(L, xs, bs, sp, &fz);
+    at = 0;
+    for (int i = 0; i < nlat; ++i)
+      for (int k = 0; k < nrhs[i]; ++k, ++at)
+        if (st) put_stats(res[i][k], st + at);
   });
 }
 int qwsg_eo_load_gauge(qwsg_lattice* h, const double* U) {
@@ -306,6 +326,12 @@ int qwsg_server_run(qwsg_server* h, const qwsg_request* reqs, int nreq, qwsg_req
     }
   });
 }
+int qwsg_server_request_bytes(long long V, int nrhs, double* bytes) {
+  return guarded([&] {
+    if (V < 1 || nrhs < 1 || nrhs > qws::MAX_BATCH) throw std::invalid_argument("server_request_bytes: V >= 1, 1..64 RHS");
+    *bytes = qws::Server::request_bytes(V, qws::Server::slot_capacity(nrhs));
+  });
+}
 int qwsg_server_destroy(qwsg_server* h) {
   return guarded([&] {
     if (h) {
@@ -386,8 +412,11 @@ namespace {
 
 struct DropIn {
   Lattice* lat = nullptr;
-  bool dirty = true;
-  bool eo_dirty = true;  // the even/odd reading of the same host arrays
+  // the host arrays changed since the device fields were built from them: links and
+  // clover, for the DD reading (dirty_*) and the even/odd reading (eo_dirty_*)
+  bool dirty_g = true, dirty_c = true;
+  bool eo_dirty_g = true, eo_dirty_c = true;
+  int* bqcd_map = nullptr;  // device: 2 (i - 1) + eo per lexicographic site (BQCD's numbering)
   bool default_blocks = true;
   int tbc = -1;
   int blocks[4] = {0, 0, 0, 0};  // the SAP blocks of the device lattice
@@ -416,14 +445,22 @@ template <typename F> void drop_in(const char* where, F&& f) {
   }
 }
 
+void mark_fields_modified() {
+  g_qws.dirty_g = g_qws.dirty_c = true;
+  g_qws.eo_dirty_g = g_qws.eo_dirty_c = true;
+}
+
 Lattice& lat_sync(const char* where) {
   if (!g_qws.lat) fatal(where, "qws_init_ has not been called");
-  if (g_qws.dirty) {
+  if (g_qws.dirty_g) {
     g_qws.lat->load_gauge(glud, 1, 1, 1);
     g_qws.lat->load_gauge(glus, 1, 0, 2);
+    g_qws.dirty_g = false;
+  }
+  if (g_qws.dirty_c) {
     g_qws.lat->load_clover(clvd, 1, 1, 1);
     g_qws.lat->load_clover(clvs, 1, 0, 2);
-    g_qws.dirty = false;
+    g_qws.dirty_c = false;
   }
   g_qws.lat->set_kappa(kappa);  // the caller may set kappa directly (main.cc:173-175)
   return *g_qws.lat;
@@ -464,6 +501,8 @@ void release_drop_in() {
       }
   if (g_qws.eo_multi) (void)hipFree(g_qws.eo_multi);
   g_qws.eo_multi = nullptr;
+  if (g_qws.bqcd_map) (void)hipFree(g_qws.bqcd_map);
+  g_qws.bqcd_map = nullptr;
   g_qws.eo_multi_bytes = 0;
   delete g_qws.lat;
   g_qws.lat = nullptr;
@@ -474,20 +513,22 @@ void create_drop_in_lattice(const int B[4]) {
   release_drop_in();
   for (int i = 0; i < 4; ++i) g_qws.blocks[i] = B[i];
   g_qws.lat = new Lattice(L, B, g_qws.tbc, g_qws.links32);
-  g_qws.dirty = true;
-  g_qws.eo_dirty = true;
+  mark_fields_modified();
 }
 
 // The even/odd family reads glud/glus and clvd/clvs as two parity halves of
 // x-compressed sites (qws_bqcd.cc:82-85); upload that reading when they changed.
 Lattice& lat_sync_eo(const char* where) {
   Lattice& L = lat_sync(where);
-  if (g_qws.eo_dirty) {
+  if (g_qws.eo_dirty_g) {
     L.eo_load_gauge(glud, 1, QWS_VLEND, 1, 1);
     L.eo_load_gauge(glus, 1, QWS_VLENS, 0, 2);
+    g_qws.eo_dirty_g = false;
+  }
+  if (g_qws.eo_dirty_c) {
     L.eo_load_clover(clvd, 1, QWS_VLEND, 1, 1);
     L.eo_load_clover(clvs, 1, QWS_VLENS, 0, 2);
-    g_qws.eo_dirty = false;
+    g_qws.eo_dirty_c = false;
   }
   return L;
 }
@@ -627,8 +668,7 @@ void qws_finalize_(void) {
 }
 
 void qws_mi355x_fields_modified_(void) {
-  g_qws.dirty = true;
-  g_qws.eo_dirty = true;
+  mark_fields_modified();
 }
 
 void qws_mi355x_set_sap_blocks_(int* b) {
@@ -1238,35 +1278,39 @@ void check_residual_s_(const scs_t* x, const scs_t* b, const int* iter, const fl
 // ---- BQCD loaders (qws_bqcd.cc:66-647).  BQCD numbers the sites of each parity
 // itself: std_xyzt2i_ (1-based index within the parity) and e_o_ (the parity) come
 // from the caller.  They are found as weak references at load time, or else in the
-// process's global symbol scope at call time (a caller that loads them later).
+// process's global symbol scope at call time (a caller that loads them later), and
+// tabulated once per geometry.  The permutations run on the GPU (launch_bqcd): a
+// BQCD array goes up once, the kernel writes QWS's host AoSoA images in device
+// memory, and these come back into the caller's arrays or QWS's field arrays.  The
+// device fields are then built from the images already on the device.
 namespace {
 
 using bqcd_site_fn = int (*)(int*);
 
-template <class F>
-void for_each_bqcd_site(const char* where, F&& f) {
+const int* bqcd_map(const char* where) {
+  if (!g_qws.lat) fatal(where, "qws_init_ has not been called");
+  if (g_qws.bqcd_map) return g_qws.bqcd_map;
   bqcd_site_fn idx = std_xyzt2i_ ? std_xyzt2i_ : (bqcd_site_fn)dlsym(RTLD_DEFAULT, "std_xyzt2i_");
   bqcd_site_fn par = e_o_ ? e_o_ : (bqcd_site_fn)dlsym(RTLD_DEFAULT, "e_o_");
   if (!idx || !par) fatal(where, "BQCD's std_xyzt2i_/e_o_ are not linked");
+  const long V = (long)nx * ny * nz * nt;
+  std::vector<int> map(V);
   for (int t = 0; t < nt; ++t)
     for (int z = 0; z < nz; ++z)
       for (int y = 0; y < ny; ++y)
         for (int x = 0; x < nx; ++x) {
           int j[4] = {x, y, z, t};
-          const int i = idx(j);
-          const int eo = par(j);
-          f(x, y, z, t, i, eo);
+          const int i = idx(j), eo = par(j);
+          if (i < 1 || i > V / 2 || (eo != 0 && eo != 1)) fatal(where, "std_xyzt2i_/e_o_ out of range");
+          map[x + (long)nx * (y + (long)ny * (z + (long)nz * t))] = 2 * (i - 1) + eo;
         }
-}
-
-// QWS clover index k <- BQCD index (qws_bqcd.cc:301-337)
-int bqcd_clover_index(int k) {
-  if (k < 2) return k;
-  if (k < 4) return 20 + (k - 2);
-  if (k < 6) return 32 + (k - 4);
-  if (k < 24) return 2 + (k - 6);
-  if (k < 34) return 22 + (k - 24);
-  return k;
+  drop_in(where, [&] {
+    int* d = nullptr;
+    QWS_HIP(hipMalloc((void**)&d, sizeof(int) * V));
+    QWS_HIP(hipMemcpy(d, map.data(), sizeof(int) * V, hipMemcpyHostToDevice));
+    g_qws.bqcd_map = d;
+  });
+  return g_qws.bqcd_map;
 }
 
 void set_kappa(double k) {
@@ -1275,75 +1319,81 @@ void set_kappa(double k) {
   kappa2 = k * k;
 }
 
-// Host AoSoA position of a site: vector index without the domain/parity offset, and
-// lane.  The DD layout splits x into two domains (xx = x % nxh, slot = x / nxh); the
-// even/odd layout compresses x (xx = x / 2, slot = parity) (qws_bqcd.cc:87-90,144-147).
-struct HostPos {
-  int iv, lane;
-};
-HostPos host_pos(int xx, int y, int z, int t, int vlen) {
-  const int nxv = nxh / vlen;
-  return {xx / vlen + nxv * (y + ny * (z + nz * t)), xx % vlen};
+qws::BqcdGeo bqcd_geo(long volh, bool eo_layout) {
+  qws::BqcdGeo b;
+  b.L[0] = nx, b.L[1] = ny, b.L[2] = nz, b.L[3] = nt;
+  b.volh = volh;
+  b.eo_layout = eo_layout ? 1 : 0;
+  return b;
 }
 
-// Links: u[ri + 2*col1 + 6*col2 + 18*(i-1) + 18*volh_tot*eo + 36*volh_tot*mu] ->
-// glu[iv + vol*mu + 4*vol*slot].c[col2][col1][ri] (qws_bqcd.cc:86-112,140-151).
-void load_bqcd_links(const char* where, const double* u, long volh_tot, bool eo_layout) {
-  for_each_bqcd_site(where, [&](int x, int y, int z, int t, int i, int eo) {
-    const long ihh = (long)(i - 1) * 18 + (long)eo * volh_tot * 18;
-    const int xx = eo_layout ? x / 2 : x % nxh;
-    const int slot = eo_layout ? eo : x / nxh;
-    const HostPos d = host_pos(xx, y, z, t, QWS_VLEND), s = host_pos(xx, y, z, t, QWS_VLENS);
-    for (int mu = 0; mu < 4; ++mu)
-      for (int c1 = 0; c1 < 3; ++c1)
-        for (int c2 = 0; c2 < 3; ++c2)
-          for (int ri = 0; ri < 2; ++ri) {
-            const double v = u[ri + 2 * c1 + 6 * c2 + ihh + (long)mu * volh_tot * 36];
-            glud[d.iv + vold * mu + 4 * vold * slot].c[c2][c1][ri][d.lane] = v;
-            glus[s.iv + vols * mu + 4 * vols * slot].c[c2][c1][ri][s.lane] = (float)v;
-          }
+// Links (kind 1) or clover (kind 2) from a BQCD array into QWS's FP64 and FP32 field
+// arrays (glud/glus, clvd/clvs) and, from the same images, into the device fields of
+// the reading the layout is for (DD or even/odd).  The other reading is rebuilt from
+// the host arrays when it is next used.
+void load_bqcd_field(const char* where, int kind, const double* src, long volh, bool eo_layout) {
+  const int* map = bqcd_map(where);
+  Lattice& L = *g_qws.lat;
+  const long V = (long)nx * ny * nz * nt;
+  if (2 * volh != V) fatal(where, "the BQCD array's parity volume does not match qws_init_'s lattice");
+  const size_t n = (size_t)V * 72;  // reals: 4 x 18 links or 72 clover per site
+  void* hd = kind == 1 ? (void*)glud : (void*)clvd;
+  void* hs = kind == 1 ? (void*)glus : (void*)clvs;
+  drop_in(where, [&] {
+    double* bq = (double*)L.work("bqcd_in", n * sizeof(double));
+    double* imd = (double*)L.work("bqcd_img_d", n * sizeof(double));
+    float* ims = (float*)L.work("bqcd_img_s", n * sizeof(float));
+    L.upload_bytes(bq, src, n * sizeof(double));
+    const qws::BqcdGeo b = bqcd_geo(volh, eo_layout);
+    qws::launch_bqcd<double>(b, kind, 1, map, bq, imd, QWS_VLEND, L.stream());
+    qws::launch_bqcd<float>(b, kind, 1, map, bq, ims, QWS_VLENS, L.stream());
+    L.download_bytes(hd, imd, n * sizeof(double));
+    L.download_bytes(hs, ims, n * sizeof(float));
+    bool& dd = kind == 1 ? g_qws.dirty_g : g_qws.dirty_c;
+    bool& eo = kind == 1 ? g_qws.eo_dirty_g : g_qws.eo_dirty_c;
+    dd = eo = true;
+    if (!eo_layout) {
+      if (kind == 1) L.load_gauge(imd, 1, 1, 1, true), L.load_gauge(ims, 1, 0, 2, true);
+      else L.load_clover(imd, 1, 1, 1, true), L.load_clover(ims, 1, 0, 2, true);
+      dd = false;
+    } else {
+      if (kind == 1) L.eo_load_gauge(imd, 1, QWS_VLEND, 1, 1, true), L.eo_load_gauge(ims, 1, QWS_VLENS, 0, 2, true);
+      else L.eo_load_clover(imd, 1, QWS_VLEND, 1, 1, true), L.eo_load_clover(ims, 1, QWS_VLENS, 0, 2, true);
+      eo = false;
+    }
   });
-  g_qws.dirty = true;
-  g_qws.eo_dirty = true;
 }
 
-// Clover inverse: 72 reals per site, two chiralities of 36 in BQCD's order, permuted
-// into QWS's (qws_bqcd.cc:203-265,301-337).
-void load_bqcd_clover(const char* where, const double* c, long volh, bool eo_layout) {
-  for_each_bqcd_site(where, [&](int x, int y, int z, int t, int i, int eo) {
-    const long ihh = (long)(i - 1) * 72 + (long)eo * volh * 72;
-    const int xx = eo_layout ? x / 2 : x % nxh;
-    const int slot = eo_layout ? eo : x / nxh;
-    const HostPos d = host_pos(xx, y, z, t, QWS_VLEND), s = host_pos(xx, y, z, t, QWS_VLENS);
-    for (int ud = 0; ud < 2; ++ud)
-      for (int k = 0; k < 36; ++k) {
-        const double v = c[bqcd_clover_index(k) + ud * 36 + ihh];
-        clvd[d.iv + vold * slot].c[ud][k][d.lane] = v;
-        clvs[s.iv + vols * slot].c[ud][k][s.lane] = (float)v;
-      }
-  });
-  g_qws.dirty = true;
-  g_qws.eo_dirty = true;
-}
-
-// Spinors: h[ri + 2*s + 8*c + 24*(i-1)].  The DD form takes both parities (ine, ino)
-// into the full field (qws_bqcd.cc:500-642); the even/odd form moves the even sites
-// only, into one parity field of x-compressed sites (qws_bqcd.cc:384-498).
+// Spinors: BQCD's per-parity arrays he/ho ([i][24]) <-> a QWS host spinor array (scd_t
+// or scs_t, lane width VL).  The DD form moves both parities into the full field
+// (qws_bqcd.cc:500-642); the even/odd form moves the even sites only, into one parity
+// field of x-compressed sites (qws_bqcd.cc:384-498; ho unused).
 template <typename T, int VL>
-void bqcd_spinor(const char* where, T* dd, double* he, double* ho, bool load, bool eo_layout) {
-  for_each_bqcd_site(where, [&](int x, int y, int z, int t, int i, int eo) {
-    if (eo_layout && eo != 0) return;
-    const long ihh = (long)(i - 1) * 24;
-    const int nxv = nxh / VL;
-    const HostPos p = host_pos(eo_layout ? x / 2 : x % nxh, y, z, t, VL);
-    const int iws = p.iv + (eo_layout ? 0 : nxv * ny * nz * nt * (x / nxh));
-    double* h = eo == 0 ? he : ho;
-    for (int c = 0; c < 3; ++c)
-      for (int s = 0; s < 4; ++s)
-        for (int ri = 0; ri < 2; ++ri) {
-          if (load) dd[iws].c[c][s][ri][p.lane] = h[ri + 2 * s + 8 * c + ihh];
-          else h[ri + 2 * s + 8 * c + ihh] = (double)dd[iws].c[c][s][ri][p.lane];
-        }
+void bqcd_spinor(const char* where, T* qws_arr, double* he, double* ho, bool load, bool eo_layout) {
+  // (scd_t and scs_t have the same size: 8 FP64 lanes or 16 FP32 lanes)
+  using E = typename std::conditional<std::is_same<T, scd_t>::value, double, float>::type;
+  static_assert(std::is_same<T, scd_t>::value || std::is_same<T, scs_t>::value, "QWS spinor types only");
+  const int* map = bqcd_map(where);
+  Lattice& L = *g_qws.lat;
+  const long V = (long)nx * ny * nz * nt, volh = V / 2;
+  const size_t nbq = (size_t)(eo_layout ? volh : V) * 24;  // BQCD reals moved
+  const size_t nimg = (size_t)(eo_layout ? volh : V) * 24;  // QWS image reals (one or both parities)
+  drop_in(where, [&] {
+    double* bq = (double*)L.work("bqcd_in", nbq * sizeof(double));
+    E* img = (E*)L.work(sizeof(E) == 8 ? "bqcd_img_d" : "bqcd_img_s", nimg * sizeof(E));
+    const qws::BqcdGeo b = bqcd_geo(volh, eo_layout);
+    const size_t half = (size_t)volh * 24 * sizeof(double);
+    if (load) {
+      L.upload_bytes(bq, he, half);
+      if (!eo_layout) L.upload_bytes(bq + (size_t)volh * 24, ho, half);
+      qws::launch_bqcd<E>(b, 0, 1, map, bq, img, VL, L.stream());
+      L.download_bytes(qws_arr, img, nimg * sizeof(E));
+    } else {
+      L.upload_bytes(img, qws_arr, nimg * sizeof(E));
+      qws::launch_bqcd<E>(b, 0, 0, map, bq, img, VL, L.stream());
+      L.download_bytes(he, bq, half);
+      if (!eo_layout) L.download_bytes(ho, bq + (size_t)volh * 24, half);
+    }
   });
 }
 
@@ -1353,9 +1403,9 @@ extern "C" {
 
 void qws_loadg_dd_(double* u, int* volh_tot, double* in_kappa) {
   set_kappa(*in_kappa);
-  load_bqcd_links(__func__, u, *volh_tot, false);
+  load_bqcd_field(__func__, 1, u, *volh_tot, false);
 }
-void qws_loadc_dd_(double* c, int* volh) { load_bqcd_clover(__func__, c, *volh, false); }
+void qws_loadc_dd_(double* c, int* volh) { load_bqcd_field(__func__, 2, c, *volh, false); }
 void qws_loadfd_dd_(scd_t* out, double* ine, double* ino) {
   bqcd_spinor<scd_t, QWS_VLEND>(__func__, out, ine, ino, true, false);
 }
@@ -1371,9 +1421,9 @@ void qws_storefs_dd_(double* oute, double* outo, scs_t* in) {
 
 void qws_loadg_bqcd_(double* u, int* volh_tot, double* in_kappa) {
   set_kappa(*in_kappa);
-  load_bqcd_links(__func__, u, *volh_tot, true);
+  load_bqcd_field(__func__, 1, u, *volh_tot, true);
 }
-void qws_loadc_bqcd_(double* c, int* volh) { load_bqcd_clover(__func__, c, *volh, true); }
+void qws_loadc_bqcd_(double* c, int* volh) { load_bqcd_field(__func__, 2, c, *volh, true); }
 void qws_loadfd_bqcd_(scd_t* out, double* in) {
   bqcd_spinor<scd_t, QWS_VLEND>(__func__, out, in, nullptr, true, true);
 }

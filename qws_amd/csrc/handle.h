@@ -1,0 +1,36 @@
+// handle.h -- what the C ABI's source files share: the opaque lattice handle and
+// the wrapper that turns exceptions into return codes and qwsg_last_error() text.
+#pragma once
+
+#include <stdexcept>
+#include <string>
+
+#include "lattice.h"
+
+struct qwsg_lattice {
+  qws::Lattice* lat;
+};
+
+namespace qws {
+
+// the calling thread's last error message (qwsg_last_error)
+std::string& last_error();
+
+// 0 on success; 1 invalid argument, 2 HIP runtime error, 3 any other failure
+template <typename F> int guarded(F&& f) {
+  try {
+    f();
+    return 0;
+  } catch (const HipError& e) {
+    last_error() = e.what();
+    return 2;
+  } catch (const std::invalid_argument& e) {
+    last_error() = e.what();
+    return 1;
+  } catch (const std::exception& e) {
+    last_error() = e.what();
+    return 3;
+  }
+}
+
+}  // namespace qws

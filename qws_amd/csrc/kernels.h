@@ -124,6 +124,27 @@ void launch_export(const Geo& g, int kind, int layout, int vlen, const S* src, v
 // FP64 links (kind 1) or clover (kind 2) to the plain host layout (dst in device memory)
 void launch_export_field(const Geo& g, int kind, const double* src, double* dst, hipStream_t st);
 
+// ---- BQCD layouts (qws_bqcd.cc:66-647) <-> QWS's host AoSoA images in device memory
+struct BqcdGeo {
+  int L[4];
+  long long volh;  // sites per parity in BQCD's arrays
+  int eo_layout;   // 1: QWS's even/odd reading (x compressed), 0: the DD reading (x halves)
+};
+// QWS clover real k (0..35 of one chirality) <- BQCD's position (qws_bqcd.cc:301-337)
+__host__ __device__ inline int bqcd_clover_index(int k) {
+  if (k < 2) return k;
+  if (k < 4) return 20 + (k - 2);
+  if (k < 6) return 32 + (k - 4);
+  if (k < 24) return 2 + (k - 6);
+  if (k < 34) return 22 + (k - 24);
+  return k;
+}
+// kind 0 spinor, 1 links, 2 clover; load 1: bq (BQCD) -> img (QWS AoSoA, lane width
+// vlen), load 0: img -> bq; map: 2 (i - 1) + eo per lexicographic site (device)
+template <typename T>
+void launch_bqcd(const BqcdGeo& b, int kind, int load, const int* map, double* bq, T* img, int vlen,
+                 hipStream_t st);
+
 // ---- even/odd family on x-compressed parity fields (eo.hip)
 // out (parity pe) from in (parity 1-pe): set 0 all hops, 1 no wrap, 2 wrap only, 3 none;
 // dag: H^+; clv: T^-1 of parity pe; mode 0 out = acc, 1 out += fac acc, 2 out = in0 + fac acc

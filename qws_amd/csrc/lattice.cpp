@@ -7,6 +7,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstddef>
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <thread>
@@ -14,6 +15,7 @@
 
 #include "fieldgen.h"
 #include "hostcopy.h"
+#include "multi.h"
 #include "sap.h"
 
 namespace qws {
@@ -25,7 +27,21 @@ template <typename T> T* dalloc(size_t n) {
   return (T*)p;
 }
 Batch single() { return Batch{1, 0, nullptr}; }
+
 }  // namespace
+
+// QWS_MI355X_TRACE=1: print the residual history to stderr the way the reference's
+// solvers do -- "@D <rhs> <outer half-step> <sqrt(|r|^2/|b|^2)>" for the FP64 outer
+// solve (bicgstab_dd_mix.cc:162,244,331) and "@C <rhs> <iterations> <residual>" at the
+// end of each inner solve (bicgstab_precdd_s.cc:594).  Reading the residuals costs a
+// host synchronisation per half-step; off by default.
+bool trace_on() {
+  static const bool on = [] {
+    const char* t = std::getenv("QWS_MI355X_TRACE");
+    return t && t[0] == '1';
+  }();
+  return on;
+}
 
 Lattice::Lattice(const int L[4], const int B[4], int tbc, int links32) {
   if (links32 != 18 && links32 != 12) throw std::invalid_argument("links32 must be 18 or 12");
@@ -75,6 +91,7 @@ Lattice::Lattice(const int L[4], const int B[4], int tbc, int links32) {
   nl32_ = links32;
   pin_chunk_ = pin_chunk_from_env();
   QWS_HIP(hipStreamCreate(&st_));
+  own_st_ = st_;
   // Launch order of the blocks: index order.  (A Morton order of the block
   // coordinates was measured slower for D on 32^3x64: consecutive workgroups then
   // read scattered memory.)
@@ -93,6 +110,7 @@ Lattice::Lattice(const int L[4], const int B[4], int tbc, int links32) {
     lists[sum & 1].push_back(b);
   }
   for (int c = 0; c < 2; ++c) {
+    h_blocks_[c] = lists[c];
     nblk_colour_[c] = (int)lists[c].size();
     d_blocks_[c] = dalloc<int>(lists[c].size() + 1);
     QWS_HIP(hipMemcpy(d_blocks_[c], lists[c].data(), lists[c].size() * sizeof(int), hipMemcpyHostToDevice));
@@ -119,7 +137,7 @@ Lattice::~Lattice() {
     if (pin_[k]) (void)hipHostFree(pin_[k]);
     if (pin_ev_[k]) (void)hipEventDestroy(pin_ev_[k]);
   }
-  if (st_) (void)hipStreamDestroy(st_);
+  if (own_st_) (void)hipStreamDestroy(own_st_);
 }
 
 
@@ -214,12 +232,16 @@ void* Lattice::alloc_spinor(int prec) {
 }
 
 // ------------------------------------------------------------------ field loading
-void Lattice::load_gauge(const void* U, int layout, int is_double, int target) {
+void Lattice::load_gauge(const void* U, int layout, int is_double, int target, bool on_device) {
   if (!owns_gauge_) throw std::runtime_error("load_gauge: this lattice borrows its links");
   const size_t es = is_double ? 8 : 4;
   const size_t bytes = (size_t)4 * g_.V * 18 * es;
-  void* stage = work("stage", bytes);
-  QWS_HIP(hipMemcpyAsync(stage, U, bytes, hipMemcpyHostToDevice, st_));
+  const void* stage = U;
+  if (!on_device) {
+    void* st = work("stage", bytes);
+    QWS_HIP(hipMemcpyAsync(st, U, bytes, hipMemcpyHostToDevice, st_));
+    stage = st;
+  }
   const int vlen = layout == 1 ? (is_double ? 8 : 16) : 1;
   if (target & 1) {
     if (!Ud_) Ud_ = dalloc<double>((size_t)4 * 18 * g_.V);
@@ -233,11 +255,15 @@ void Lattice::load_gauge(const void* U, int layout, int is_double, int target) {
   QWS_HIP(hipStreamSynchronize(st_));
 }
 
-void Lattice::load_clover(const void* C, int layout, int is_double, int target) {
+void Lattice::load_clover(const void* C, int layout, int is_double, int target, bool on_device) {
   const size_t es = is_double ? 8 : 4;
   const size_t bytes = (size_t)g_.V * 72 * es;
-  void* stage = work("stage", bytes);
-  QWS_HIP(hipMemcpyAsync(stage, C, bytes, hipMemcpyHostToDevice, st_));
+  const void* stage = C;
+  if (!on_device) {
+    void* st = work("stage", bytes);
+    QWS_HIP(hipMemcpyAsync(st, C, bytes, hipMemcpyHostToDevice, st_));
+    stage = st;
+  }
   const int vlen = layout == 1 ? (is_double ? 8 : 16) : 1;
   if (target & 1) {
     if (!Cd_) Cd_ = dalloc<double>((size_t)72 * g_.V);
@@ -500,6 +526,64 @@ int Lattice::fb_flags(int flags, const float* fb, bool produce) {
   return flags;
 }
 
+// The fused launches of prec_ddd_s_ (qws.cc:776-926) or prec_s_ (qws.cc:929-975) in
+// FP32: nsap sweeps (qws.cc:809-841, each half-sweep's face update done at the start of
+// the next launch), then the final sequence.  The first sweep's two launches are the
+// first readers of in_c for their colour c: with pupd each applies the direction update
+// to its own sites and stores it.
+std::vector<SapStep> Lattice::sap_plan(const Batch& B, bool ddd, float* out, const float* in, int nsap, bool pupd,
+                                       float*& fb, long long& fbs) {
+  float* s = (float*)work("sap_s", spinor_bytes(P32) * B.n);
+  float* q = (float*)work("sap_q", spinor_bytes(P32) * B.n);
+  fb = sap_face_buffer<float>(B, fbs);
+  std::vector<SapStep> v;
+  auto step = [&](int colour, int flags, bool produce, const float* qface, float* qout, float* o, bool pu) {
+    SapStep st;
+    st.colour = colour, st.flags = fb_flags(flags, fb, produce), st.s = s, st.in = in, st.qface = qface;
+    st.qout = qout, st.out = o, st.pupd = pu;
+    v.push_back(st);
+  };
+  for (int isap = 0; isap < nsap; ++isap)
+    for (int c = 0; c < 2; ++c) {
+      int flags = SAP_EPI_SUPDATE;
+      if (isap == 0) flags |= SAP_SRC_IN;
+      if (isap > 0 || c == 1) flags |= SAP_FACE_PRO;
+      step(c, flags, true, q, q, nullptr, isap == 0 && pupd);
+    }
+  const int src = nsap == 0 ? SAP_SRC_IN : 0;
+  if (ddd) {
+    // final e: q_e = A_e s_e, out_e = D_in q_e
+    step(0, SAP_EPI_OUT | src | (nsap > 0 ? SAP_FACE_PRO : 0), true, q, q, out, false);
+    // final o: s_o += kappa T^-1 H_face q, q_o = A_o s_o, out_o = D_oe q_e + D_in q_o
+    // (and, with the face buffer, the face data of q_o for the last step)
+    step(1, SAP_EPI_OUT_FACE | src | SAP_FACE_PRO, true, q, q, out, false);
+    // out_e += D_eo q_o
+    SapStep f;
+    f.face = true, f.colour = 0, f.qface = q, f.out = out;
+    v.push_back(f);
+  } else {
+    step(0, SAP_EPI_NONE | src | (nsap > 0 ? SAP_FACE_PRO : 0), true, q, out, nullptr, false);
+    step(1, SAP_EPI_NONE | src | SAP_FACE_PRO, false, out, out, nullptr, false);
+  }
+  return v;
+}
+
+void Lattice::run_sap_plan(const Batch& B, const std::vector<SapStep>& plan, float* fb, long long fbs, int nm,
+                           const SapSrcUpdate* upd) {
+  const FieldSet<float> f = fields<float>();
+  const float k = (float)kappa_;
+  const SapSrcUpdate none;
+  for (const SapStep& st : plan) {
+    if (st.face) {
+      if (fb) launch_sap_face(g_, sel(st.colour), B, f.nl, -k, st.out, fb, fbs, f.U, f.C, st_);
+      else hop<float>(sel(st.colour), B, HOPS_FACE, -k, st.out, st.out, nullptr, nullptr, st.qface);
+      continue;
+    }
+    launch_sap_block<float>(g_, sel(st.colour), B, f.nl, k, nm, st.flags | (st.pupd ? SAP_SRC_PUPD : 0), st.s, st.in,
+                            st.qface, st.qout, st.out, f.U, f.C, fb, fbs, st_, st.pupd ? *upd : none);
+  }
+}
+
 // prec_ddd_s_ (qws.cc:776-926): out = D M_SAP in, with the final sequence of the
 // improper (npe[mu]==1) and of the proper colouring.
 template <typename S>
@@ -515,23 +599,10 @@ void Lattice::prec_ddd_t(const Batch& B, S* out, const S* in, int nsap, int nm, 
     const bool fold = upd && fused_ok<S>() && nsap > 0;
     if (upd && !fold) launch_update_p<float>(B, g_.V * 12, upd->p, upd->r, upd->q, upd->sc, st_);
     if (fused_ok<S>()) {
-      const FieldSet<float> f = fields<float>();
-      const int src = nsap == 0 ? SAP_SRC_IN : 0;
+      float* fb;
       long long fbs;
-      float* fb = sap_face_buffer<float>(B, fbs);
-      sap_sweeps_fused<float>(B, in, s, q, nsap, nm, fb, fbs, fold ? upd : nullptr);
-      // final e: q_e = A_e s_e, out_e = D_in q_e
-      launch_sap_block<float>(g_, sel(0), B, f.nl, (float)kappa_, nm,
-                              fb_flags(SAP_EPI_OUT | src | (nsap > 0 ? SAP_FACE_PRO : 0), fb, true), s, in, q, q, out,
-                              f.U, f.C, fb, fbs, st_);
-      // final o: s_o += kappa T^-1 H_face q, q_o = A_o s_o, out_o = D_oe q_e + D_in q_o
-      // (and, with the face buffer, the face data of q_o for the last step)
-      launch_sap_block<float>(g_, sel(1), B, f.nl, (float)kappa_, nm,
-                              fb_flags(SAP_EPI_OUT_FACE | src | SAP_FACE_PRO, fb, true), s, in, q, q, out, f.U, f.C, fb,
-                              fbs, st_);
-      // out_e += D_eo q_o
-      if (fb) launch_sap_face(g_, sel(0), B, f.nl, -k, out, fb, fbs, f.U, f.C, st_);
-      else hop<S>(sel(0), B, HOPS_FACE, -k, out, out, nullptr, nullptr, q);
+      const std::vector<SapStep> plan = sap_plan(B, true, out, in, nsap, fold, fb, fbs);
+      run_sap_plan(B, plan, fb, fbs, nm, upd);
       return;
     }
   }
@@ -563,7 +634,15 @@ void Lattice::prec_s_t(const Batch& B, S* out, const S* in, int nsap, int nm) {
   const int pr = sizeof(S) == 8 ? P64 : sizeof(S) == 4 ? P32 : P16;
   S* s = (S*)work("sap_s", spinor_bytes(pr) * B.n);
   S* q = (S*)work("sap_q", spinor_bytes(pr) * B.n);
-  if constexpr (!std::is_same<S, double>::value) {
+  if constexpr (std::is_same<S, float>::value) {
+    if (fused_ok<S>()) {
+      float* fb;
+      long long fbs;
+      const std::vector<SapStep> plan = sap_plan(B, false, out, in, nsap, false, fb, fbs);
+      run_sap_plan(B, plan, fb, fbs, nm, nullptr);
+      return;
+    }
+  } else if constexpr (!std::is_same<S, double>::value) {
     if (fused_ok<S>()) {
       const FieldSet<S> f = fields<S>();
       const int src = nsap == 0 ? SAP_SRC_IN : 0;
@@ -654,10 +733,14 @@ unsigned long long Lattice::digest(int prec, const void* a) {
 }
 
 // ------------------------------------------------------------------ even/odd family
-void Lattice::eo_load_gauge(const void* U, int layout, int vlen, int is_double, int target) {
+void Lattice::eo_load_gauge(const void* U, int layout, int vlen, int is_double, int target, bool on_device) {
   const size_t bytes = (size_t)4 * g_.V * 18 * (is_double ? 8 : 4);
-  void* stage = work("stage", bytes);
-  QWS_HIP(hipMemcpyAsync(stage, U, bytes, hipMemcpyHostToDevice, st_));
+  const void* stage = U;
+  if (!on_device) {
+    void* st = work("stage", bytes);
+    QWS_HIP(hipMemcpyAsync(st, U, bytes, hipMemcpyHostToDevice, st_));
+    stage = st;
+  }
   if (target & 1) {
     if (!Ueo_d_) Ueo_d_ = dalloc<double>((size_t)4 * 18 * g_.V);
     launch_eo_import<double>(g_, 1, layout, vlen, 18, stage, is_double, Ueo_d_, st_);
@@ -669,10 +752,14 @@ void Lattice::eo_load_gauge(const void* U, int layout, int vlen, int is_double, 
   QWS_HIP(hipStreamSynchronize(st_));
 }
 
-void Lattice::eo_load_clover(const void* C, int layout, int vlen, int is_double, int target) {
+void Lattice::eo_load_clover(const void* C, int layout, int vlen, int is_double, int target, bool on_device) {
   const size_t bytes = (size_t)g_.V * 72 * (is_double ? 8 : 4);
-  void* stage = work("stage", bytes);
-  QWS_HIP(hipMemcpyAsync(stage, C, bytes, hipMemcpyHostToDevice, st_));
+  const void* stage = C;
+  if (!on_device) {
+    void* st = work("stage", bytes);
+    QWS_HIP(hipMemcpyAsync(st, C, bytes, hipMemcpyHostToDevice, st_));
+    stage = st;
+  }
   if (target & 1) {
     if (!Ceo_d_) Ceo_d_ = dalloc<double>((size_t)72 * g_.V);
     launch_eo_import<double>(g_, 2, layout, vlen, 0, stage, is_double, Ceo_d_, st_);
@@ -948,13 +1035,19 @@ int Lattice::mcg_eo(double* const* xs, const double* b, int ns, const double* si
 // for every RHS k with live[k] != 0.  B.active must point at d_inner_mask_; an RHS
 // leaves the batch when its recursive residual meets tol (tol < 0: fixed count).
 //
-// hp: the method of bicgstab_precdd_s_h.cc:83-370 instead -- w = M_SAP p with the
-// FP16 SAP, q = D w with the FP32 operator, and x accumulates w directly (x is the
-// solution of D x = b, no final M_SAP).
+// Without hp it runs as Fused::bicg on this lattice alone.  hp: the method of
+// bicgstab_precdd_s_h.cc:83-370 instead -- w = M_SAP p with the FP16 SAP, q = D w
+// with the FP32 operator, and x accumulates w directly (x is the solution of D x = b,
+// no final M_SAP).
 template <typename S>
 void Lattice::bicg_t(const Batch& Bin, S* x, const S* b, double tol, int maxiter, int nsap, int nm,
                      std::vector<int>& iters, bool hp) {
   static_assert(sizeof(S) == 4, "the inner solver runs on FP32 vectors");
+  if (!hp) {
+    std::vector<Fused::InnerJob> one{{this, Bin, (float*)x, (const float*)b, &iters}};
+    Fused::bicg(one, tol, maxiter, nsap, nm, nullptr);
+    return;
+  }
   const int nb = Bin.n;
   const long long n = g_.V * 12;
   const size_t bytes = spinor_bytes(P32) * nb;
@@ -963,7 +1056,7 @@ void Lattice::bicg_t(const Batch& Bin, S* x, const S* b, double tol, int maxiter
   S* p = (S*)work("bicg_p", bytes);
   S* t = (S*)work("bicg_t", bytes);
   S* r0 = (S*)work("bicg_r0", bytes);
-  S* w = hp ? (S*)work("bicg_w", bytes) : nullptr;
+  S* w = (S*)work("bicg_w", bytes);
   const float mk = (float)(-kappa_);
   if (nb > sc_in_n_) {
     if (sc_in_) QWS_HIP(hipFree(sc_in_));
@@ -1009,40 +1102,29 @@ void Lattice::bicg_t(const Batch& Bin, S* x, const S* b, double tol, int maxiter
       if (v) return true;
     return false;
   };
-  SapSrcUpdate pupd;
-  pupd.p = reinterpret_cast<float*>(p);
-  pupd.r = reinterpret_cast<const float*>(r);
-  pupd.q = reinterpret_cast<const float*>(q);
-  pupd.sc = sc;
-  bool pupd_pending = false;
   for (int iter = 0; iter < maxiter && any_live(); ++iter) {
     for (int k = 0; k < nb; ++k) iters[k] += live[k];
-    if (hp) {
-      msap_h_t(B, w, p, nsap, nm);
-      hop<S>(sel(-1), B, HOPS_ALL, mk, q, w, nullptr, nullptr, w);
-    } else {
-      prec_ddd_t<S>(B, q, p, nsap, nm, pupd_pending ? &pupd : nullptr);
-      pupd_pending = false;
-    }
+    msap_h_t(B, w, p, nsap, nm);
+    hop<S>(sel(-1), B, HOPS_ALL, mk, q, w, nullptr, nullptr, w);
     launch_dot<S>(B, n, r0, q, part, st_);
     launch_finalize(B, part, 2, sc, FIN_ALPHA, st_);
-    launch_update_xr<S>(B, n, x, hp ? w : p, r, q, sc, 0, nullptr, part, st_);
+    launch_update_xr<S>(B, n, x, w, r, q, sc, 0, nullptr, part, st_);
     launch_finalize(B, part, 1, sc, FIN_RNORM, st_);
     if (!converge()) break;
-    if (hp) {
-      msap_h_t(B, w, r, nsap, nm);
-      hop<S>(sel(-1), B, HOPS_ALL, mk, t, w, nullptr, nullptr, w);
-    } else {
-      prec_ddd_t<S>(B, t, r, nsap, nm);
-    }
+    msap_h_t(B, w, r, nsap, nm);
+    hop<S>(sel(-1), B, HOPS_ALL, mk, t, w, nullptr, nullptr, w);
     launch_norm_dot<S>(B, n, t, r, part, st_);
     launch_finalize(B, part, 3, sc, FIN_OMEGA, st_);
-    launch_update_xr<S>(B, n, x, hp ? w : r, r, t, sc, 1, r0, part, st_);
+    launch_update_xr<S>(B, n, x, w, r, t, sc, 1, r0, part, st_);
     launch_finalize(B, part, 3, sc, FIN_RHO, st_);
     if (!converge()) break;
-    // p = r + beta (p - omega q): done by the next q = D M p (bicgstab_precdd_s.cc:518-545)
-    if (!hp && fold_pupd_) pupd_pending = true;
-    else launch_update_p<S>(B, n, p, r, q, sc, st_);
+    launch_update_p<S>(B, n, p, r, q, sc, st_);  // p = r + beta (p - omega q)
+  }
+  if (trace_on()) {
+    read_scalars(sc, nb);
+    for (int k = 0; k < nb; ++k)
+      if (iters[k] > 0)
+        std::fprintf(stderr, "@C %d %5d %12.7e\n", k, iters[k], std::sqrt(sc_host_[k].rnorm / bnorm[k]));
   }
 }
 
@@ -1159,10 +1241,14 @@ void OuterBatch::admit(int k, const double* b, const double* x0) {
 
 // Slots whose recursive residual sqrt(|r|^2/|b|^2) met tol finish
 // (bicgstab_dd_mix.cc:249,336).
-void OuterBatch::check(std::vector<int>& live) {
-  if (prm_.tol < 0) return;
-  L_.read_scalars(sc_, cap_);
+void OuterBatch::check(std::vector<int>& live, int half) {
+  if (prm_.tol < 0 && !trace_on()) return;
   const Scalars* h = L_.host_scalars();
+  if (trace_on())  // outer half-step 2 iter - 1 (after alpha) or 2 iter (after omega)
+    for (int k = 0; k < cap_; ++k)
+      if (live[k])
+        std::fprintf(stderr, "@D %d %5d %24.15e\n", k, 2 * iters_[k] - 1 + half, std::sqrt(h[k].rnorm / h[k].bnorm));
+  if (prm_.tol < 0) return;
   bool changed = false;
   for (int k = 0; k < cap_; ++k)
     if (live[k] && (std::sqrt(h[k].rnorm / h[k].bnorm) < prm_.tol || !std::isfinite(h[k].rnorm))) {
@@ -1174,46 +1260,8 @@ void OuterBatch::check(std::vector<int>& live) {
 }
 
 int OuterBatch::step() {
-  std::vector<int> live(cap_);
-  int nlive = 0;
-  for (int k = 0; k < cap_; ++k) nlive += (live[k] = state_[k] == RUNNING);
-  if (!nlive) return 0;
-  hipStream_t st = L_.stream();
-  const long long n = L_.volume() * 12;
-  double* part = L_.reduction_partials(cap_);
-  upload_mask(live);
-  const Batch B{cap_, stride_, d_mask_};
-  for (int k = 0; k < cap_; ++k) iters_[k] += live[k];
-  // u = M p ; q = D u ; alpha ; x += alpha u ; r -= alpha q
-  L_.approx_inv(B, sc_, live, u_, p_, prm_, inner_);
-  L_.dslash_batch(B, q_, u_);
-  launch_dot<double>(B, n, r0_, q_, part, st);
-  launch_finalize(B, part, 2, sc_, FIN_ALPHA, st);
-  launch_update_xr<double>(B, n, x_, u_, r_, q_, sc_, 0, nullptr, part, st);
-  launch_finalize(B, part, 1, sc_, FIN_RNORM, st);
-  check(live);
-  bool any = false;
-  for (int v : live) any |= v != 0;
-  if (any) {
-    // u = M r ; t = D u ; omega ; x += omega u ; r -= omega t ; rho, beta
-    L_.approx_inv(B, sc_, live, u_, r_, prm_, inner_);
-    L_.dslash_batch(B, t_, u_);
-    launch_norm_dot<double>(B, n, t_, r_, part, st);
-    launch_finalize(B, part, 3, sc_, FIN_OMEGA, st);
-    launch_update_xr<double>(B, n, x_, u_, r_, t_, sc_, 1, r0_, part, st);
-    launch_finalize(B, part, 3, sc_, FIN_RHO, st);
-    check(live);
-    any = false;
-    for (int v : live) any |= v != 0;
-    if (any) launch_update_p<double>(B, n, p_, r_, q_, sc_, st);
-  }
-  // the reference loops iter = 1 .. maxiter-1 and reports maxiter when it ran out
-  for (int k = 0; k < cap_; ++k)
-    if (state_[k] == RUNNING && iters_[k] >= prm_.maxiter - 1) {
-      state_[k] = FINISHED;
-      iters_[k] = prm_.maxiter;
-    }
-  return active_count();
+  std::vector<OuterBatch*> one{this};
+  return Fused::outer_step(one, nullptr);
 }
 
 int Lattice::bicgstab_dd_mix(double* x, const double* b, const SolveParams& prm, int* inner_total) {
@@ -1251,55 +1299,10 @@ double Lattice::true_residual(const double* x, const double* b) {
 
 std::vector<SolveStats> Lattice::solve_batch(double* const* x, const double* const* b, int nrhs,
                                              const SolveParams& prm) {
-  std::vector<SolveStats> out(nrhs);
-  if (nrhs <= 0) return out;
-  auto t0 = std::chrono::steady_clock::now();
-  if (!batch_ || batch_->capacity() != nrhs) {
-    batch_.reset();
-    batch_.reset(new OuterBatch(*this, nrhs, prm));
-  }
-  OuterBatch& ob = *batch_;
-  ob.set_params(prm);
-  // a zero source has the exact solution x = 0 (the relative residual is undefined
-  // there, and the normalised inner solve would divide by zero): it takes no slot
-  std::vector<int> open(nrhs, 1);
-  for (int k = 0; k < nrhs; ++k) {
-    if (norm2(P64, b[k]) == 0.0) {
-      open[k] = 0;
-      out[k].true_residual = 0;
-      QWS_HIP(hipMemsetAsync(x[k], 0, spinor_bytes(P64), st_));
-      ob.release(k);
-    } else {
-      ob.admit(k, b[k], x[k]);
-    }
-  }
-  for (;;) {
-    while (ob.step() > 0) {
-    }
-    // FP64 true residual of every slot that just finished; restart those above true_tol
-    bool again = false;
-    for (int k = 0; k < nrhs; ++k) {
-      if (!open[k]) continue;
-      out[k].outer_iters += ob.iterations(k);
-      out[k].inner_iters += ob.inner_iterations(k);
-      out[k].true_residual = true_residual(ob.x(k), ob.b(k));
-      if (prm.true_tol > 0 && out[k].true_residual > prm.true_tol && out[k].restarts < prm.max_restarts) {
-        out[k].restarts++;
-        ob.admit(k, ob.b(k), ob.x(k));
-        again = true;
-      } else {
-        open[k] = 0;
-      }
-    }
-    if (!again) break;
-  }
-  for (int k = 0; k < nrhs; ++k)
-    if (ob.running(k) || ob.finished(k))
-      QWS_HIP(hipMemcpyAsync(x[k], ob.x(k), spinor_bytes(P64), hipMemcpyDeviceToDevice, st_));
-  sync();
-  const double sec = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-  for (auto& s : out) s.seconds = sec;
-  return out;
+  if (nrhs <= 0) return std::vector<SolveStats>(nrhs > 0 ? nrhs : 0);
+  std::vector<std::vector<double*>> xs{std::vector<double*>(x, x + nrhs)};
+  std::vector<std::vector<const double*>> bs{std::vector<const double*>(b, b + nrhs)};
+  return Fused::solve({this}, xs, bs, prm, nullptr)[0];
 }
 
 SolveStats Lattice::solve(double* x, const double* b, const SolveParams& prm) {
@@ -1439,4 +1442,11 @@ double Lattice::time_sap(int prec, int reps, int nsap, int nm) {
   throw std::invalid_argument("time_sap: FP32 or FP16 storage");
 }
 
+}  // namespace qws
+
+namespace qws {
+// used by the fused multi-lattice solves (multi.cpp)
+template bool Lattice::fused_ok<float>() const;
+template void Lattice::prec_ddd_t<float>(const Batch&, float*, const float*, int, int, const SapSrcUpdate*);
+template void Lattice::prec_s_t<float>(const Batch&, float*, const float*, int, int);
 }  // namespace qws

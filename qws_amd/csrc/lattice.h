@@ -13,6 +13,23 @@
 
 namespace qws {
 struct SapSrcUpdate;
+struct Fused;
+class SapFusion;
+
+// One launch of a fused FP32 SAP sequence on one lattice: the blocks of `colour`, the
+// launch's flags and buffers (s, in, qface, qout, out of launch_sap_block), whether it
+// applies the BiCGStab direction update (pupd); face: instead the final face term
+// out_e += -kappa T^-1 H_eo q_o of prec_ddd_s_ (from the face buffer, or gathered from
+// qface without one).
+struct SapStep {
+  int colour = 0, flags = 0;
+  float* s = nullptr;
+  const float* in = nullptr;
+  const float* qface = nullptr;
+  float* qout = nullptr;
+  float* out = nullptr;
+  bool pupd = false, face = false;
+};
 }
 
 namespace qws {
@@ -86,7 +103,10 @@ class OuterBatch {
  private:
   enum { EMPTY = 0, RUNNING = 1, FINISHED = 2 };
   void upload_mask(const std::vector<int>& m);
-  void check(std::vector<int>& live);  // converged slots leave `live`
+  friend struct Fused;
+  // converged slots leave `live`, from the scalars last read into the lattice's host
+  // copy; half: 0 after the alpha update, 1 after the omega update
+  void check(std::vector<int>& live, int half);
   Lattice& L_;
   int cap_;
   SolveParams prm_;
@@ -110,13 +130,20 @@ class Lattice {
   double kappa() const { return kappa_; }
   void set_kappa(double k) { kappa_ = k; }
   hipStream_t stream() const { return st_; }
+  // run this lattice's work on another stream (nullptr: its own again); the lattices of
+  // a solve fused over several lattices share one stream
+  void set_stream(hipStream_t s) { st_ = s ? s : own_st_; }
   int links32() const { return nl32_; }
   // fused SAP half-sweep kernels (default on where applicable)
   void set_fused_sap(bool on) { use_fused_ = on; }
 
   // fields from host arrays (layout 0 plain, 1 QWS AoSoA)
-  void load_gauge(const void* U, int layout, int is_double, int target);  // target: bit0 FP64, bit1 FP32
-  void load_clover(const void* C, int layout, int is_double, int target);
+  // target: bit0 FP64, bit1 FP32; on_device: U/C are already in device memory
+  void load_gauge(const void* U, int layout, int is_double, int target, bool on_device = false);
+  void load_clover(const void* C, int layout, int is_double, int target, bool on_device = false);
+  // caller (pageable) host memory <-> device memory through the pinned chunks; synchronous
+  void upload_bytes(void* dev, const void* host, size_t bytes) { copy_to_device(dev, host, bytes); }
+  void download_bytes(void* host, const void* dev, size_t bytes) { copy_to_host(host, dev, bytes); }
   // generated inputs: SU(3) links exp(i eps H), clover inverse from the links
   void generate_gauge(double eps, unsigned long long seed);
   // use another lattice's links (same extents and link storage); they stay owned by src,
@@ -181,8 +208,8 @@ class Lattice {
   size_t eo_spinor_bytes(int prec) const { return spinor_bytes(prec) / 2; }
   // host fields in layout 0 (plain: links [2][4][V/2][18], clover [2][V/2][72]) or 1 (QWS
   // AoSoA with lane width vlen, read as parity halves); target bit0 FP64, bit1 FP32
-  void eo_load_gauge(const void* U, int layout, int vlen, int is_double, int target);
-  void eo_load_clover(const void* C, int layout, int vlen, int is_double, int target);
+  void eo_load_gauge(const void* U, int layout, int vlen, int is_double, int target, bool on_device = false);
+  void eo_load_clover(const void* C, int layout, int vlen, int is_double, int target, bool on_device = false);
   // even/odd fields from the lattice's own fields (all extents even)
   void eo_fields_from_lattice();
   void eo_upload_spinor(int prec, void* dev, const void* host, int layout, int vlen, int host_is_double);
@@ -284,7 +311,17 @@ class Lattice {
   std::unique_ptr<OuterBatch> batch_;  // reused by solve/solve_batch (same capacity)
 
   friend class OuterBatch;
+  friend struct Fused;
   void ensure_half_fields();
+  hipStream_t own_st_ = nullptr;
+  std::vector<int> h_blocks_[2];  // host copy of the block lists per colour
+  // The launches of one fused FP32 prec_ddd_s_ (ddd) or prec_s_ application on the
+  // blocks of this lattice (fused_ok<float>() must hold); pupd: the first sweep applies
+  // BiCGStab's direction update.  fb/fbs: the face buffer it uses (nullptr: none).
+  std::vector<SapStep> sap_plan(const Batch& B, bool ddd, float* out, const float* in, int nsap, bool pupd, float*& fb,
+                                long long& fbs);
+  void run_sap_plan(const Batch& B, const std::vector<SapStep>& plan, float* fb, long long fbs, int nm,
+                    const SapSrcUpdate* upd);
 };
 
 constexpr int MAX_BATCH = 64;

@@ -154,3 +154,65 @@ void launch_export_field(const Geo& g, int kind, const double* src, double* dst,
 }
 
 }  // namespace qws
+
+namespace qws {
+
+// BQCD's arrays <-> QWS's single-rank host AoSoA images held in device memory
+// (qws_bqcd.cc:66-647), one thread per lexicographic site lex = x + Lx (y + Ly (z +
+// Lz t)).  map[lex] = 2 (i - 1) + eo, with i BQCD's 1-based index of the site within
+// its parity eo (std_xyzt2i_, e_o_), tabulated once on the host.  BQCD arrays:
+// links [mu][eo][i][18], clover [eo][i][72] (QWS real k of chirality ud at BQCD
+// position clover_perm[k] + 36 ud, qws_bqcd.cc:301-337), spinors [eo][i][24]; every
+// record but the clover's is in QWS's own element order.  The QWS image of lane width
+// vlen: DD reading (x split into halves, slot = x / (Lx/2)) or, with eo_layout, the
+// even/odd reading (x compressed by 2, slot = the parity; spinors: even sites only).
+
+template <typename T>
+__global__ void k_bqcd(BqcdGeo b, int kind, int load, const int* __restrict__ map, double* bq, T* img, int vlen) {
+  const long long lex = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  const long long V = (long long)b.L[0] * b.L[1] * b.L[2] * b.L[3];
+  if (lex >= V) return;
+  long long r = lex;
+  int x[4];
+#pragma unroll
+  for (int m = 0; m < 4; ++m) {
+    x[m] = (int)(r % b.L[m]);
+    r /= b.L[m];
+  }
+  const int code = map[lex];
+  const long long i0 = code >> 1;
+  const int eo = code & 1;
+  if (kind == 0 && b.eo_layout && eo) return;  // the even/odd spinor loaders move the even sites only
+  const int nxh = b.L[0] / 2, nxv = nxh / vlen;
+  const int xx = b.eo_layout ? x[0] / 2 : x[0] % nxh;
+  const int slot = b.eo_layout ? eo : x[0] / nxh;
+  const long long vv = (long long)nxv * b.L[1] * b.L[2] * b.L[3];  // vectors per domain slot
+  const long long iv = xx / vlen + (long long)nxv * (x[1] + (long long)b.L[1] * (x[2] + (long long)b.L[2] * x[3]));
+  const int lane = xx % vlen;
+  const int n = kind == 0 ? 24 : (kind == 1 ? 18 : 72);
+  const int nmu = kind == 1 ? 4 : 1;
+  for (int mu = 0; mu < nmu; ++mu) {
+    const long long bqb = kind == 1 ? 18 * (i0 + b.volh * eo) + 36 * b.volh * mu : n * (i0 + b.volh * eo);
+    const long long qb = kind == 1 ? (iv + vv * mu + 4 * vv * slot) * 18 : (iv + vv * slot) * n;
+    for (int k = 0; k < n; ++k) {
+      const int kb = kind == 2 ? bqcd_clover_index(k % 36) + 36 * (k / 36) : k;
+      const long long qi = (qb + k) * vlen + lane;
+      if (load) img[qi] = (T)bq[bqb + kb];
+      else bq[bqb + kb] = (double)img[qi];
+    }
+  }
+}
+
+template <typename T>
+void launch_bqcd(const BqcdGeo& b, int kind, int load, const int* map, double* bq, T* img, int vlen,
+                 hipStream_t st) {
+  const long long V = (long long)b.L[0] * b.L[1] * b.L[2] * b.L[3];
+  const int NT = 128;
+  hipLaunchKernelGGL((k_bqcd<T>), dim3((unsigned)((V + NT - 1) / NT)), dim3(NT), 0, st, b, kind, load, map, bq, img,
+                     vlen);
+}
+
+template void launch_bqcd<double>(const BqcdGeo&, int, int, const int*, double*, double*, int, hipStream_t);
+template void launch_bqcd<float>(const BqcdGeo&, int, int, const int*, double*, float*, int, hipStream_t);
+
+}  // namespace qws

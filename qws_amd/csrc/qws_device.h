@@ -51,12 +51,13 @@ struct Geo {
 // with its own L2.  Give XCD x a contiguous range of logical workgroup ids (blocks
 // that share faces then share an L2): a bijection of [0, nwg).
 constexpr int NUM_XCD = 8;
-__device__ __forceinline__ int xcd_remap(const Geo& g, int wg, int nwg) {
-  if (!g.xcd_remap) return wg;
+__device__ __forceinline__ int xcd_remap_n(int on, int wg, int nwg) {
+  if (!on) return wg;
   const int q = nwg / NUM_XCD, r = nwg % NUM_XCD;
   const int x = wg % NUM_XCD, i = wg / NUM_XCD;
   return x * q + (x < r ? x : r) + i;
 }
+__device__ __forceinline__ int xcd_remap(const Geo& g, int wg, int nwg) { return xcd_remap_n(g.xcd_remap, wg, nwg); }
 
 // p, hidden from the optimiser.  Loads through the result taken inside a loop over
 // right-hand sides are not loop-invariant to the compiler, so it does not hoist a

@@ -28,6 +28,35 @@ struct SapSrcUpdate {
   const Scalars* sc = nullptr;
 };
 
+// One lattice's share of a launch fused over several lattices: what a single launch
+// takes as arguments, per lattice.  For the face-term kernel, kappa holds its factor.
+template <typename S> struct SapPart {
+  Geo g;
+  const int* active;
+  long long stride;
+  int nrhs;
+  float kappa;
+  S* s;
+  const S* in;
+  const S* qface;
+  S* qout;
+  S* out;
+  const S* U;
+  const S* C;
+  float* fb;
+  long long fbs;
+  SapSrcUpdate upd;
+};
+// A launch fused over several lattices: workgroup w runs block table[w].y of part
+// table[w].x (device arrays).  The parts share the block volume, the link storage,
+// the flags and nm; xcd: map the workgroups to the table XCD by XCD.
+template <typename S> struct SapMulti {
+  const SapPart<S>* parts = nullptr;
+  const int2* table = nullptr;
+  int n = 0;
+  int xcd = 1;
+};
+
 bool sap_fusable(const Geo& g);
 // floats per RHS of the face buffer, or 0 when the block shape does not allow it
 // (its slots must fit the iterate's LDS: 2 sum_mu Vb / B_mu <= 2 Vb)
@@ -40,5 +69,10 @@ template <typename S>
 void launch_sap_block(const Geo& g, BlockSel sel, const Batch& B, int nl, float kappa, int nm, int flags, S* s,
                       const S* in, const S* qface, S* qout, S* out, const S* U, const S* C, float* fb,
                       long long fbs, hipStream_t st, const SapSrcUpdate& upd = SapSrcUpdate());
+// The same over several lattices in one launch (FP32 storage): vb = their sites per
+// block, nl their FP32 link storage; every part has a face buffer when `flags` use one.
+void launch_sap_multi(const SapMulti<float>& m, int vb, int nl, int nm, int flags, hipStream_t st);
+// launch_sap_face over several lattices (part.out, .fb, .fbs, .U, .C; factor in .kappa)
+void launch_sap_face_multi(const SapMulti<float>& m, int vb, int nl, hipStream_t st);
 
 }  // namespace qws

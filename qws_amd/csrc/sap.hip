@@ -63,6 +63,24 @@ void launch_sap_face(const Geo& g, BlockSel sel, const Batch& B, int nl, float g
   else sap256::launch_face(g, sel, B, nl, gamma, out, fb, fbs, U, C, st);
 }
 
+void launch_sap_multi(const SapMulti<float>& m, int vb, int nl, int nm, int flags, hipStream_t st) {
+  if (m.n == 0) return;
+  if (nm < 1) nm = 1;  // the first application is unconditional (static_solver.cc:77-101)
+  if (flags & SAP_SRC_PUPD) {
+    if (!(flags & SAP_SRC_IN)) throw std::invalid_argument("fused SAP: the source update needs SAP_SRC_IN");
+  }
+  if (vb == 512) sap512::launch_multi(m, nl, nm, flags, st);
+  else if (vb == 256) sap256::launch_multi(m, nl, nm, flags, st);
+  else throw std::invalid_argument("fused SAP: 256 or 512 sites per block");
+}
+
+void launch_sap_face_multi(const SapMulti<float>& m, int vb, int nl, hipStream_t st) {
+  if (m.n == 0) return;
+  if (vb == 512) sap512::launch_face_multi(m, nl, st);
+  else if (vb == 256) sap256::launch_face_multi(m, nl, st);
+  else throw std::invalid_argument("face term: 256 or 512 sites per block");
+}
+
 template void launch_sap_block<float>(const Geo&, BlockSel, const Batch&, int, float, int, int, float*, const float*,
                                       const float*, float*, float*, const float*, const float*, float*, long long,
                                       hipStream_t, const SapSrcUpdate&);

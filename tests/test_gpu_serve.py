@@ -1,7 +1,8 @@
 """Solve server: resident gauge fields, requests of mixed volume / kappa / RHS count
-admitted into slots as slots and memory free up, running at the same time on their
-own streams.  Each result is the same alone and in the mix (bit for bit: equal
-digests of the FP64 solutions) and meets the FP64 true residual."""
+admitted as slots and memory free up into the batch that runs (continuous batching:
+every SAP half-sweep one launch over all admitted requests' lattices).  Each result
+is the same alone and in the mix (bit for bit: equal digests of the FP64 solutions,
+equal iteration counts) and meets the FP64 true residual."""
 import numpy as np
 import pytest
 
@@ -49,7 +50,7 @@ def test_same_alone_and_in_the_mix(server):
 
 def test_memory_admission():
     """A budget that holds one request at a time serialises the requests."""
-    one = max(1.1 * int(np.prod(f[0])) * (864 + 2976 * 4) + 64 * 2 ** 20 for f in FIELDS)
+    one = max(Q.Server.request_bytes(int(np.prod(f[0])), 4) for f in FIELDS)
     s = Q.Server(FIELDS, links32=12, slots=4, mem_budget=one + 1.0, **PRM)
     try:
         tr = _trace(6, seed=3, rate=1000.0)

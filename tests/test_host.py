@@ -1,12 +1,12 @@
-"""CPU tests of the library's host-side code: the BQCD loaders and storers
-(qws_bqcd.cc:66-642), the FP64<->FP32 reorders (reordering.cc:59-204) and the driver
-helpers (qws.cc:445-500).  They touch only host memory, so they run without a GPU:
-the geometry globals are set directly and the field pointers point at numpy buffers
-for the duration of a test (restored afterwards, so qws_init_ never frees them).
+"""CPU tests of the library's host-side code: the FP64<->FP32 reorders
+(reordering.cc:59-204) and the driver helpers (qws.cc:445-500).  They touch only host
+memory, so they run without a GPU: the geometry globals are set directly and the field
+pointers point at numpy buffers for the duration of a test (restored afterwards, so
+qws_init_ never frees them).  (The BQCD loaders and storers permute on the GPU; their
+tests are in test_gpu_bqcd.py.)
 
 Expected values: the reference's AoSoA layouts as oracle/ref_worker.py states them (the
-layouts that the reference-library parity tests use), BQCD's layouts from oracle/bqcd.py
-with the tests' stand-in site numbering (oracle/libbqcd_sites.so)."""
+layouts that the reference-library parity tests use)."""
 import ctypes
 import os
 
@@ -14,11 +14,9 @@ import numpy as np
 import pytest
 
 import qws_amd as Q
-from oracle import bqcd as BQ
 from oracle import ref_worker as RW
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-SITES = os.path.join(ROOT, "oracle", "libbqcd_sites.so")
 NLOC = [32, 2, 2, 4]
 
 
@@ -57,78 +55,11 @@ def host():
             g[n].value = v
 
 
-@pytest.fixture(scope="module")
-def sites():
-    lib = ctypes.CDLL(SITES, mode=ctypes.RTLD_GLOBAL)
-    lib.bqcd_sites_set_((ctypes.c_int * 4)(*NLOC))
-    return lib
-
-
 def _fields(seed):
     V = int(np.prod(NLOC))
     rng = np.random.default_rng(seed)
     return (rng.standard_normal((4, V, 3, 3, 2)), rng.standard_normal((V, 2, 36)),
             rng.standard_normal((V, 3, 4, 2)))
-
-
-def _eo_sites():
-    nx, ny, nz, nt = NLOC
-    t, z, y, xh = np.indices((nt, nz, ny, nx // 2)).reshape(4, -1)
-    return np.stack([2 * xh + (p + y + z + t) % 2 + nx * (y + ny * (z + nz * t)) for p in (0, 1)])
-
-
-def test_bqcd_dd_loaders_and_storers(host, sites):
-    L, A, vals = host
-    U, C, b = _fields(1)
-    V = int(np.prod(NLOC))
-    L.qws_loadg_dd_(_ptr(BQ.links(U, NLOC)), _ci(V // 2), ctypes.byref(ctypes.c_double(0.125)))
-    L.qws_loadc_dd_(_ptr(BQ.clover(C, NLOC)), _ci(V // 2))
-    assert ctypes.c_double.in_dll(L, "kappa").value == 0.125
-    assert ctypes.c_double.in_dll(L, "kappa2").value == 0.125 ** 2
-    assert np.array_equal(A["glud"], RW.links_to_aosoa(U, NLOC, 8).ravel())
-    assert np.array_equal(A["glus"], RW.links_to_aosoa(U.astype(np.float32), NLOC, 16).ravel())
-    assert np.array_equal(A["clvd"], RW.clover_to_aosoa(C, NLOC, 8).ravel())
-    assert np.array_equal(A["clvs"], RW.clover_to_aosoa(C.astype(np.float32), NLOC, 16).ravel())
-    he, ho = BQ.spinor(b, NLOC)
-    sd = np.zeros(2 * vals["vold"] * 192)
-    L.qws_loadfd_dd_(_ptr(sd), _ptr(he), _ptr(ho))
-    assert np.array_equal(sd, RW.spinor_to_aosoa(b, NLOC, 8).ravel())
-    ss = np.zeros(2 * vals["vols"] * 384, np.float32)
-    L.qws_loadfs_dd_(_ptr(ss), _ptr(he), _ptr(ho))
-    assert np.array_equal(ss, RW.spinor_to_aosoa(b.astype(np.float32), NLOC, 16).ravel())
-    oe, oo = np.zeros_like(he), np.zeros_like(ho)
-    L.qws_storefd_dd_(_ptr(oe), _ptr(oo), _ptr(sd))
-    assert np.array_equal(oe, he) and np.array_equal(oo, ho)
-    L.qws_storefs_dd_(_ptr(oe), _ptr(oo), _ptr(ss))
-    assert np.array_equal(BQ.spinor_from(oe, oo, NLOC), b.astype(np.float32))
-
-
-def test_bqcd_even_odd_loaders_and_storers(host, sites):
-    L, A, vals = host
-    U, C, b = _fields(2)
-    V = int(np.prod(NLOC))
-    site = _eo_sites()
-    L.qws_loadg_bqcd_(_ptr(BQ.links(U, NLOC)), _ci(V // 2), ctypes.byref(ctypes.c_double(0.1)))
-    L.qws_loadc_bqcd_(_ptr(BQ.clover(C, NLOC)), _ci(V // 2))
-    links = RW.eo_links_from_raw(A["glud"], NLOC, 8)
-    clover = RW.eo_clover_from_raw(A["clvd"], NLOC, 8)
-    for p in (0, 1):
-        assert np.array_equal(links[p], U.reshape(4, V, 18)[:, site[p]]), p
-        assert np.array_equal(clover[p], C.reshape(V, 72)[site[p]]), p
-    assert np.array_equal(RW.eo_links_from_raw(A["glus"], NLOC, 16), links.astype(np.float32))
-    assert np.array_equal(RW.eo_clover_from_raw(A["clvs"], NLOC, 16), clover.astype(np.float32))
-    he, _ = BQ.spinor(b, NLOC)
-    xd = np.zeros(vals["vold"] * 192)
-    L.qws_loadfd_bqcd_(_ptr(xd), _ptr(he))
-    assert np.array_equal(RW.eo_spinor_from_raw(xd, NLOC, 8), b.reshape(V, 24)[site[0]])
-    xs = np.zeros(vals["vols"] * 384, np.float32)
-    L.qws_loadfs_bqcd_(_ptr(xs), _ptr(he))
-    assert np.array_equal(RW.eo_spinor_from_raw(xs, NLOC, 16), b.reshape(V, 24)[site[0]].astype(np.float32))
-    back = np.zeros_like(he)
-    L.qws_storefd_bqcd_(_ptr(back), _ptr(xd))
-    assert np.array_equal(back, he)
-    L.qws_storefs_bqcd_(_ptr(back), _ptr(xs))
-    assert np.array_equal(back, he.astype(np.float32))
 
 
 def test_reorders(host):
