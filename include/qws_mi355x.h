@@ -332,6 +332,15 @@ int qwsg_clover(qwsg_lattice* lat, int prec, void* out, const void* in);
 int qwsg_jinv(qwsg_lattice* lat, int prec, void* x, const void* b, int colour, int nm);
 int qwsg_prec_ddd(qwsg_lattice* lat, int prec, void* out, const void* in, int nsap, int nm);
 int qwsg_prec_s(qwsg_lattice* lat, int prec, void* out, const void* in, int nsap, int nm);
+/* No counterpart in the reference: qwsg_prec_ddd in FP32 (out = D M_SAP in) for callers
+ * that cannot allocate device memory (code under HIP graph capture, serving loops that
+ * must not call hipMalloc).  It allocates nothing: its work buffers (the SAP iterates
+ * and the face buffer, 288 B per site with 4^4 blocks) live in a fixed device array of
+ * QWSG_NOALLOC_SCRATCH_BYTES that each GPU holds for the life of the process; a lattice
+ * that needs more is rejected (return 1).  Results are bit-identical to qwsg_prec_ddd's.
+ * Calls that overlap on one GPU share that array and race. */
+#define QWSG_NOALLOC_SCRATCH_BYTES (192u << 20)
+int qwsg_prec_ddd_noalloc(qwsg_lattice* lat, void* out, const void* in, int nsap, int nm);
 int qwsg_bicgstab_precdd(qwsg_lattice* lat, int prec, void* x, const void* b, double tol, int maxiter, int nsap,
                          int nm, int* iters);
 /* prec_ddd_s_h.h:63 -- out = M_SAP in with the FP16-stored SAP (FP32 vectors in and out) */

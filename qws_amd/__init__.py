@@ -65,6 +65,7 @@ def lib():
         L.qwsg_jinv.argtypes = [vp, ctypes.c_int, vp, vp, ctypes.c_int, ctypes.c_int]
         L.qwsg_prec_ddd.argtypes = [vp, ctypes.c_int, vp, vp, ctypes.c_int, ctypes.c_int]
         L.qwsg_prec_s.argtypes = [vp, ctypes.c_int, vp, vp, ctypes.c_int, ctypes.c_int]
+        L.qwsg_prec_ddd_noalloc.argtypes = [vp, vp, vp, ctypes.c_int, ctypes.c_int]
         L.qwsg_bicgstab_precdd.argtypes = [vp, ctypes.c_int, vp, vp, ctypes.c_double, ctypes.c_int, ctypes.c_int,
                                            ctypes.c_int, ip]
         L.qwsg_msap_h.argtypes = [vp, vp, vp, ctypes.c_int, ctypes.c_int]
@@ -368,6 +369,10 @@ class Lattice:
 
     def prec_ddd(self, out, inp, nsap, nm):
         check(lib().qwsg_prec_ddd(self.h, inp.prec, out.ptr, inp.ptr, nsap, nm))
+
+    def prec_ddd_noalloc(self, out, inp, nsap, nm):
+        """prec_ddd in FP32 through the entry point that allocates nothing."""
+        check(lib().qwsg_prec_ddd_noalloc(self.h, out.ptr, inp.ptr, nsap, nm))
 
     def prec_s(self, out, inp, nsap, nm):
         check(lib().qwsg_prec_s(self.h, inp.prec, out.ptr, inp.ptr, nsap, nm))

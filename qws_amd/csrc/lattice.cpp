@@ -182,6 +182,13 @@ void Lattice::copy_to_host(void* host, const void* dev, size_t bytes) {
 }
 
 void* Lattice::work(const std::string& name, size_t bytes) {
+  if (arena_) {  // carved from caller-provided memory: nothing is allocated
+    const size_t off = (arena_off_ + 255) & ~(size_t)255;
+    if (off + bytes > arena_cap_)
+      throw std::invalid_argument("work buffer '" + name + "' does not fit the provided scratch memory");
+    arena_off_ = off + bytes;
+    return arena_ + off;
+  }
   for (auto& w : work_)
     if (w.first == name) {
       if (w.second.second >= bytes) return w.second.first;
@@ -691,6 +698,29 @@ void Lattice::msap_h(float* out, const float* in, int nsap, int nm) { msap_h_t(s
 void Lattice::prec_ddd(int prec, void* out, const void* in, int nsap, int nm) {
   if (nsap > 10 || nm > 10) throw std::invalid_argument("prec_ddd_s: nsap and nm must be <= 10 (qws.cc:798-799)");
   DISPATCH_PREC(prec, (prec_ddd_t<S>(single(), (S*)out, (const S*)in, nsap, nm)));
+}
+
+size_t Lattice::prec_ddd_scratch_bytes() const {
+  auto a256 = [](size_t b) { return (b + 255) & ~(size_t)255; };
+  const size_t sp = spinor_bytes(P32);
+  size_t third = sp;  // the step-by-step path's third spinor
+  if (fused_ok<float>()) third = use_fb_ ? sizeof(float) * (size_t)sap_face_buffer_floats(g_) : 0;  // face buffer
+  return a256(sp) + a256(sp) + third;  // s, q, then the third buffer, each at a 256-byte boundary
+}
+
+void Lattice::prec_ddd_scratch(void* out, const void* in, int nsap, int nm, void* scratch, size_t bytes) {
+  if (nsap > 10 || nm > 10) throw std::invalid_argument("prec_ddd_s: nsap and nm must be <= 10 (qws.cc:798-799)");
+  const size_t need = prec_ddd_scratch_bytes();
+  if (need > bytes)
+    throw std::invalid_argument("prec_ddd on this lattice needs " + std::to_string(need) +
+                                " bytes of scratch memory, more than the " + std::to_string(bytes) + " provided");
+  if (((uintptr_t)scratch & 255) != 0) throw std::invalid_argument("scratch memory must be 256-byte aligned");
+  struct Restore {
+    Lattice* L;
+    ~Restore() { L->arena_ = nullptr, L->arena_cap_ = 0, L->arena_off_ = 0; }
+  } restore{this};
+  arena_ = (char*)scratch, arena_cap_ = bytes, arena_off_ = 0;
+  prec_ddd_t<float>(single(), (float*)out, (const float*)in, nsap, nm);
 }
 
 void Lattice::prec_s(int prec, void* out, const void* in, int nsap, int nm) {

@@ -168,6 +168,11 @@ class Lattice {
   void clover(int prec, void* out, const void* in);
   void jinv(int prec, void* x, const void* b, int colour, int nm);
   void prec_ddd(int prec, void* out, const void* in, int nsap, int nm);
+  // prec_ddd in FP32 with its work buffers carved from `scratch` (device memory of
+  // `bytes`) instead of allocated: nothing is allocated; prec_ddd_scratch_bytes() is
+  // what it needs.  Every buffer it uses is written before it is read.
+  size_t prec_ddd_scratch_bytes() const;
+  void prec_ddd_scratch(void* out, const void* in, int nsap, int nm, void* scratch, size_t bytes);
   void prec_s(int prec, void* out, const void* in, int nsap, int nm);
   int bicgstab_precdd(int prec, void* x, const void* b, double tol, int maxiter, int nsap, int nm);
   // FP16-storage SAP (assign_mult_mSAP_s_h_) and the inner solver built on it
@@ -307,6 +312,9 @@ class Lattice {
   Scalars* sc_eo_ = nullptr;    // even/odd solver
   int* d_inner_mask_ = nullptr;
   std::vector<std::pair<std::string, std::pair<void*, size_t>>> work_;
+  // while set, work() carves its buffers from this memory instead (prec_ddd_scratch)
+  char* arena_ = nullptr;
+  size_t arena_cap_ = 0, arena_off_ = 0;
 
   std::unique_ptr<OuterBatch> batch_;  // reused by solve/solve_batch (same capacity)
 
