@@ -132,55 +132,43 @@ PRECISION_WORKLOAD = dict(L=[48, 48, 48, 96], B=[4, 4, 4, 4], eps=0.25, seed=3, 
                           links32=12)
 
 
-def solve_in_chunks(Q, lat, x, b, prm, budget_s, label):
-    """The mixed solve restarted every 4 outer iterations from the current x (a
-    progress line per chunk), until the FP64 true residual meets prm["true_tol"] or the
-    time budget is spent."""
-    t0 = time.perf_counter()
-    outer = inner = 0
-    while True:
-        st = lat.solve(x, b, **dict(prm, maxiter=5, max_restarts=0))
-        outer += st["outer_iters"]
-        inner += st["inner_iters"]
-        res = st["true_residual"]
-        progress("%s: %d outer / %d inner iterations, true residual %.2e" % (label, outer, inner, res))
-        if res <= prm["true_tol"] or time.perf_counter() - t0 > budget_s:
-            break
-    return {"seconds": time.perf_counter() - t0, "outer": outer, "inner": inner, "true_residual": res,
-            "converged": bool(res <= prm["true_tol"])}
-
-
 def precision_variant(Q, kappa=None, budget_s=240.0):
-    """48^3x96: FP16-stored SAP (FP32 arithmetic), FP32 inner, FP64 outer to 1e-10; its
-    error against a reference solution from the FP64 outer solver (FP32 SAP) driven to
-    a 1e-13 true residual (the preconditioner's precision bounds only the convergence
-    rate, not the accuracy of the FP64-refined solution).  Near the critical kappa the
-    inner solves hit their cap, so both solves run restarted in chunks of outer
-    iterations within a time budget each."""
+    """48^3x96: FP16-stored SAP (FP32 arithmetic), FP32 inner, FP64 outer to a 1e-10 FP64
+    true residual in one solve (near the critical kappa the FP16-stored SAP can make an
+    inner solve diverge; the solve then goes on with the FP32 SAP: sap_escalated).  Its
+    error against an FP64-refined continuation: the FP32-SAP mixed solve started from
+    the result and driven to a 1e-13 true residual (the preconditioner's precision
+    bounds only the convergence rate, not the accuracy of the FP64-refined solution).
+    budget_s is unused (the solve runs to completion)."""
     w = dict(PRECISION_WORKLOAD, kappa=kappa if kappa is not None else PRECISION_WORKLOAD["kappa"])
     lat, b = setup(Q, w)
     progress("48^3x96 fields ready")
     x = lat.spinor(Q.F64)
-    prm = dict(SOLVE, sap_precision=Q.F16, nsap=3, nm=6, maxiter_s=200)
+    prm = dict(SOLVE, sap_precision=Q.F16, nsap=3, nm=6, maxiter=200, maxiter_s=200)
     lat.solve(x, b, **dict(prm, maxiter=2, true_tol=-1.0, max_restarts=0))  # warm-up
     x.zero()
     lat.sync()
-    hp = solve_in_chunks(Q, lat, x, b, prm, budget_s, "FP16-SAP solve")
-    xr = lat.spinor(Q.F64)
-    ref = solve_in_chunks(Q, lat, xr, b, dict(SOLVE, tol=1e-13, maxiter_s=200, true_tol=1e-12), budget_s,
-                          "FP64-refined reference solve")
+    t0 = time.perf_counter()
+    st = lat.solve(x, b, **prm)
+    sec = time.perf_counter() - t0
+    progress("FP16-SAP solve: %d outer / %d inner iterations, true residual %.2e, escalated %d, %.0f s"
+             % (st["outer_iters"], st["inner_iters"], st["true_residual"], st["sap_escalated"], sec))
+    xr = lat.spinor(Q.F64, x.download())
+    t1 = time.perf_counter()
+    ref = lat.solve(xr, b, **dict(SOLVE, tol=1e-14, maxiter=200, maxiter_s=200, true_tol=1e-13))
+    ref_sec = time.perf_counter() - t1
+    progress("FP64-refined continuation: true residual %.2e, %.0f s" % (ref["true_residual"], ref_sec))
     a, r = x.download(), xr.download()
     err = float(np.linalg.norm(a - r) / np.linalg.norm(r))
     del a, r
     lat.close()
     return {"workload": "48^3x96, eps=0.25 seed 3, kappa=%g, csw=1.1, FP16-stored SAP / FP32 inner / FP64 outer"
                         % w["kappa"],
-            "solves_per_s": 1.0 / hp["seconds"], "seconds": hp["seconds"], "converged": hp["converged"],
-            "true_residual": hp["true_residual"], "error_vs_fp64_reference": err,
-            "reference_true_residual": ref["true_residual"], "reference_converged": ref["converged"],
-            "iterations": {"outer": hp["outer"], "inner": hp["inner"]},
-            "reference_iterations": {"outer": ref["outer"], "inner": ref["inner"]},
-            "reference_seconds": ref["seconds"]}
+            "solves_per_s": 1.0 / sec, "seconds": sec, "converged": bool(st["true_residual"] <= 1e-10),
+            "true_residual": st["true_residual"], "sap_escalated": st["sap_escalated"],
+            "error_vs_fp64_refined": err, "reference_true_residual": ref["true_residual"],
+            "iterations": {"outer": st["outer_iters"], "inner": st["inner_iters"], "restarts": st["restarts"]},
+            "reference_continuation_seconds": ref_sec}
 
 
 def eo_operator(Q, lat, reps=20):
@@ -441,6 +429,8 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--mfma-trial", type=int, default=0,
+                    help="time the multi-RHS clover product on the VALU and on MFMA (N launches each)")
     ap.add_argument("--dslash-only", type=int, default=0, help="only time N dslash launches (for counter runs)")
     ap.add_argument("--dropin", action="store_true",
                     help="the workload through bicgstab_dd_mix_ with host arrays (PCIe-inclusive rate)")
@@ -493,6 +483,15 @@ def main():
             res = srv.run(tr)
             print(json.dumps({"rate_req_per_s": rate, **serve_stats(res, tr)}), flush=True)
         srv.close()
+        return
+    if a.mfma_trial:
+        # the multi-RHS clover product, VALU vs MFMA, on 2^20 sites x 16 RHS
+        for nrep in (1, 4, 16, 64):
+            ms_v, ms_m, diff = Q.time_clover_rhs(1 << 20, nrep, a.mfma_trial)
+            useful = (1 << 20) * 16 * nrep * 2 * 144 * 2  # real-form 12x12 products, flops
+            print(json.dumps({"nrep": nrep, "valu_ms": ms_v, "mfma_ms": ms_m, "mfma_speedup": ms_v / ms_m,
+                              "valu_TFLOPs": useful / (ms_v * 1e-3) / 1e12,
+                              "mfma_TFLOPs": useful / (ms_m * 1e-3) / 1e12, "max_rel_diff": diff}), flush=True)
         return
     if a.dropin:
         print(json.dumps(dropin_solve(Q)))

@@ -262,6 +262,10 @@ typedef struct {
   int restarts;
   double true_residual;    /* |b - D x| / |b| recomputed in FP64 on the GPU */
   double seconds;          /* wall time of the solve */
+  int sap_escalated;       /* sap_precision = QWSG_F16 and tol_s > 0: an inner solve on the FP16-stored
+                              SAP ended with a relative residual >= 1 (this happens near the critical
+                              kappa), and the solve went on with the FP32 SAP from there, with the
+                              same number of sweeps (nsap - 1) */
 } qwsg_solve_stats;
 
 const char* qwsg_last_error(void);
@@ -272,6 +276,14 @@ int qwsg_device_count(int* n);
  * (nin + 1) * bytes of device memory for the call.  Measurement only; no
  * counterpart in the reference. */
 int qwsg_stream_bandwidth(long long bytes, int nin, int reps, double* gbs);
+/* Measurement only, no counterpart in the reference: the multi-RHS clover product (the
+ * two Hermitian 6x6 blocks of the clover inverse, clover_def.h:88-284, applied to 16
+ * right-hand sides per site, FP16 storage, FP32 accumulation, nrep products on data in
+ * registers) on V synthetic sites, once on the vector ALUs and once on the matrix cores
+ * (v_mfma_f32_16x16x16f16, the sites' 16 RHS as the N dimension): ms[0] VALU and ms[1]
+ * MFMA device time per launch, averaged over reps launches; *maxdiff = their largest
+ * difference relative to the largest VALU value. */
+int qwsg_time_clover_rhs(long long V, int nrep, int reps, double* ms, double* maxdiff);
 
 /* L: lattice extents, B: SAP block extents (must divide L; each of the
  * y/z/t block counts even or 1, x block count even), tbc: +1 periodic / -1
@@ -459,6 +471,14 @@ int qwsg_time_dslash(qwsg_lattice* lat, int prec, int reps, double* ms_per_call)
  * it in *chosen.  Tuning only; no counterpart in the reference. */
 int qwsg_set_dslash_kernel(qwsg_lattice* lat, int kernel);
 int qwsg_autotune_dslash(qwsg_lattice* lat, int reps, int* chosen);
+/* The solver's SAP settings tuned on this lattice (no counterpart in the reference):
+ * one solve of b from x = 0 (x is overwritten) for every (nsap[i], nm[j]) with p's
+ * other settings, timed; *best = p with the fastest pair whose FP64 true residual meets
+ * p->true_tol (all pairs when true_tol <= 0); seconds[i * nnm + j] per pair (negative:
+ * missed the target).  It also keeps the fastest full-operator kernel
+ * (qwsg_autotune_dslash). */
+int qwsg_autotune_solve(qwsg_lattice* lat, void* x, const void* b, const qwsg_solve_params* p, const int* nsap,
+                        int nnsap, const int* nm, int nnm, qwsg_solve_params* best, double* seconds);
 /* average device time (ms) of one fused SAP half-sweep launch (FP32 or FP16
  * storage), from `reps` times the 2*nsap+2 launches of one preconditioner
  * application on a Gaussian source, measured with HIP events on the lattice's

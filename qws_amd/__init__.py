@@ -33,7 +33,7 @@ class SolveParams(ctypes.Structure):
 
 class SolveStats(ctypes.Structure):
     _fields_ = [("outer_iters", ctypes.c_int), ("inner_iters", ctypes.c_int), ("restarts", ctypes.c_int),
-                ("true_residual", ctypes.c_double), ("seconds", ctypes.c_double)]
+                ("true_residual", ctypes.c_double), ("seconds", ctypes.c_double), ("sap_escalated", ctypes.c_int)]
 
 
 def lib():
@@ -97,9 +97,13 @@ def lib():
         L.qwsg_time_dslash.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
         L.qwsg_set_dslash_kernel.argtypes = [vp, ctypes.c_int]
         L.qwsg_autotune_dslash.argtypes = [vp, ctypes.c_int, ip]
+        L.qwsg_autotune_solve.argtypes = [vp, vp, vp, ctypes.POINTER(SolveParams), vp, ctypes.c_int, vp, ctypes.c_int,
+                                          ctypes.POINTER(SolveParams), vp]
         L.qwsg_time_sap.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                     ctypes.POINTER(ctypes.c_double)]
         L.qwsg_device_count.argtypes = [ip]
+        L.qwsg_time_clover_rhs.argtypes = [ctypes.c_longlong, ctypes.c_int, ctypes.c_int,
+                                           ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]
         L.qwsg_stream_bandwidth.argtypes = [ctypes.c_longlong, ctypes.c_int, ctypes.c_int,
                                             ctypes.POINTER(ctypes.c_double)]
         L.qwsg_generate_gauge.argtypes = [vp, ctypes.c_double, ctypes.c_ulonglong]
@@ -481,6 +485,22 @@ class Lattice:
         """FP32 full-operator kernel on 256-site blocks: 0 global gathers, 3/4 LDS-staged."""
         check(lib().qwsg_set_dslash_kernel(self.h, int(k)))
 
+    def autotune_solve(self, x, b, nsaps, nms, **params):
+        """Time one solve of b per (nsap, nm) candidate pair; returns (best params dict,
+        seconds[len(nsaps)][len(nms)], negative where the true residual was missed)."""
+        base = dict(dict(nsap=4, nm=2, tol=-1.0, maxiter=6, tol_s=-1.0, maxiter_s=50, true_tol=-1.0, max_restarts=0,
+                         sap_precision=F32), **params)
+        p = SolveParams(base["nsap"], base["nm"], base["tol"], base["maxiter"], base["tol_s"], base["maxiter_s"],
+                        base["true_tol"], base["max_restarts"], base["sap_precision"])
+        best = SolveParams()
+        ns = (ctypes.c_int * len(nsaps))(*nsaps)
+        nm = (ctypes.c_int * len(nms))(*nms)
+        sec = (ctypes.c_double * (len(nsaps) * len(nms)))()
+        check(lib().qwsg_autotune_solve(self.h, x.ptr, b.ptr, ctypes.byref(p), ns, len(nsaps), nm, len(nms),
+                                        ctypes.byref(best), sec))
+        out = dict(base, nsap=best.nsap, nm=best.nm)
+        return out, [[sec[i * len(nms) + j] for j in range(len(nms))] for i in range(len(nsaps))]
+
     def autotune_dslash(self, reps=20):
         """Time the full-operator kernels on this lattice, keep and return the fastest."""
         k = ctypes.c_int()
@@ -529,3 +549,13 @@ def solve_multi(parts, nsap=4, nm=2, tol=-1.0, maxiter=6, tol_s=-1.0, maxiter_s=
         out.append([{k: getattr(s, k) for k, _ in SolveStats._fields_} for s in st[at:at + len(xl)]])
         at += len(xl)
     return out
+
+
+def time_clover_rhs(V, nrep, reps):
+    """The multi-RHS clover product (16 RHS per site, FP16 storage, FP32 accumulation)
+    on the vector ALUs and on the matrix cores: (ms VALU, ms MFMA, max relative
+    difference).  Measurement only."""
+    ms = (ctypes.c_double * 2)()
+    d = ctypes.c_double()
+    check(lib().qwsg_time_clover_rhs(int(V), int(nrep), int(reps), ms, ctypes.byref(d)))
+    return ms[0], ms[1], d.value

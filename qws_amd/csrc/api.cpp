@@ -4,6 +4,7 @@
 
 #include <dlfcn.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -36,6 +37,9 @@ const char* qwsg_last_error(void) { return qws::last_error().c_str(); }
 
 int qwsg_device_count(int* n) {
   return guarded([&] { QWS_HIP(hipGetDeviceCount(n)); });
+}
+int qwsg_time_clover_rhs(long long V, int nrep, int reps, double* ms, double* maxdiff) {
+  return guarded([&] { qws::time_clover_rhs(V, nrep, reps, ms, maxdiff); });
 }
 int qwsg_stream_bandwidth(long long bytes, int nin, int reps, double* gbs) {
   return guarded([&] { *gbs = qws::measure_stream(bytes, nin, reps); });
@@ -181,6 +185,7 @@ void put_stats(const qws::SolveStats& s, qwsg_solve_stats* st) {
   st->restarts = s.restarts;
   st->true_residual = s.true_residual;
   st->seconds = s.seconds;
+  st->sap_escalated = s.sap_escalated;
 }
 }  // namespace
 
@@ -376,6 +381,21 @@ int qwsg_time_dslash(qwsg_lattice* h, int prec, int reps, double* ms) {
 }
 int qwsg_set_dslash_kernel(qwsg_lattice* h, int k) {
   return guarded([&] { h->lat->set_dslash_kernel(k); });
+}
+int qwsg_autotune_solve(qwsg_lattice* h, void* x, const void* b, const qwsg_solve_params* p, const int* nsap,
+                        int nnsap, const int* nm, int nnm, qwsg_solve_params* best, double* seconds) {
+  return guarded([&] {
+    if (!p || !best || !nsap || !nm || nnsap < 1 || nnm < 1) throw std::invalid_argument("autotune_solve: bad arguments");
+    const qws::SolveParams sp = solve_params(p);
+    std::vector<double> sec;
+    const qws::SolveParams b2 = h->lat->autotune_solve((double*)x, (const double*)b, sp,
+                                                       std::vector<int>(nsap, nsap + nnsap),
+                                                       std::vector<int>(nm, nm + nnm), sec);
+    *best = *p;
+    best->nsap = b2.nsap;
+    best->nm = b2.nm;
+    if (seconds) std::copy(sec.begin(), sec.end(), seconds);
+  });
 }
 int qwsg_autotune_dslash(qwsg_lattice* h, int reps, int* chosen) {
   return guarded([&] {
@@ -841,6 +861,7 @@ static bool dd_mix_body(const char* where, scd_t* x, const scd_t* b, const doubl
     p.nsap = *nsap;
     p.nm = *nm;
     p.sap_prec = sap_prec;
+    p.escalate = 0;  // the reference's method throughout
     // bicgstab_dd_mix.cc:165 runs iter = 1..maxiter-1 and reports the last iter;
     // bicgstab_dd_mix2_hf.cc:169 runs iter = 0..maxiter-1 and reports iter only on convergence
     if (sap_prec == qws::P16) p.maxiter = *maxiter + 1;

@@ -1176,7 +1176,7 @@ int Lattice::bicgstab_precdd_h(float* x, const float* b, double tol, int maxiter
 // reordering.cc:108-204), for the RHS of B with live[k] != 0 (B.active holds the
 // same flags on the device); sc = the outer scalars (nrm is set here); the inner
 // iterations are added to inner[k].
-void Lattice::approx_inv(const Batch& B, Scalars* sc, const std::vector<int>& live, double* u, const double* p,
+bool Lattice::approx_inv(const Batch& B, Scalars* sc, const std::vector<int>& live, double* u, const double* p,
                          const SolveParams& prm, std::vector<int>& inner) {
   const long long n = g_.V * 12;
   const int nb = B.n;
@@ -1193,13 +1193,25 @@ void Lattice::approx_inv(const Batch& B, Scalars* sc, const std::vector<int>& li
   if (prm.sap_prec == P16) {
     // approx_inv_dirac_op_hf (bicgstab_dd_mix2_hf.cc:88-106): the inner solution is x itself
     bicg_t<float>(Batch{nb, g_.V * 24, d_inner_mask_}, xs, bs, prm.tol_s, prm.maxiter_s, prm.nsap, prm.nm, it, true);
+    for (int k = 0; k < nb; ++k)
+      if (live[k]) inner[k] += it[k];
+    if (prm.escalate && prm.tol_s > 0) {
+      // the inner solve's last residual relative to its (unit) source
+      read_scalars(sc_in_, nb);
+      for (int k = 0; k < nb; ++k)
+        if (live[k]) {
+          const double rel = std::sqrt(sc_host_[k].rnorm / sc_host_[k].bnorm);
+          if (!(rel < 1.0)) return true;  // failed: no u; the caller switches to the FP32 SAP
+        }
+    }
   } else {
     bicg_t<float>(Batch{nb, g_.V * 24, d_inner_mask_}, ts, bs, prm.tol_s, prm.maxiter_s, prm.nsap, prm.nm, it, false);
     prec_s_t<float>(B, xs, ts, prm.nsap, prm.nm);
+    for (int k = 0; k < nb; ++k)
+      if (live[k]) inner[k] += it[k];
   }
   launch_s2d_scaled<float>(B, g_.V, B.stride, B.stride, u, xs, sc, st_);
-  for (int k = 0; k < nb; ++k)
-    if (live[k]) inner[k] += it[k];
+  return false;
 }
 
 void check_sap_params(int nsap, int nm) {
@@ -1210,6 +1222,7 @@ void check_sap_params(int nsap, int nm) {
 void OuterBatch::set_params(const SolveParams& p) {
   check_sap_params(p.nsap, p.nm);
   prm_ = p;
+  escalated_ = false;
 }
 
 OuterBatch::OuterBatch(Lattice& L, int capacity, const SolveParams& p) : L_(L), cap_(capacity), prm_(p) {
@@ -1459,6 +1472,34 @@ int Lattice::autotune_dslash(int reps) {
     }
   }
   g_.dslash_kernel = best;
+  return best;
+}
+
+SolveParams Lattice::autotune_solve(double* x, const double* b, const SolveParams& prm, const std::vector<int>& nsaps,
+                                    const std::vector<int>& nms, std::vector<double>& seconds) {
+  if (nsaps.empty() || nms.empty()) throw std::invalid_argument("autotune_solve: empty candidate lists");
+  for (int v : nsaps) check_sap_params(v, 1);
+  for (int v : nms) check_sap_params(1, v);
+  autotune_dslash(20);
+  seconds.assign(nsaps.size() * nms.size(), -1.0);
+  SolveParams best = prm;
+  double best_s = 1e300;
+  double* xs[1] = {x};
+  const double* bs[1] = {b};
+  for (size_t i = 0; i < nsaps.size(); ++i)
+    for (size_t j = 0; j < nms.size(); ++j) {
+      SolveParams p = prm;
+      p.nsap = nsaps[i], p.nm = nms[j];
+      QWS_HIP(hipMemsetAsync(x, 0, spinor_bytes(P64), st_));
+      sync();
+      const auto t0 = std::chrono::steady_clock::now();
+      const SolveStats st = solve_batch(xs, bs, 1, p)[0];
+      const double sec = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+      if (prm.true_tol > 0 && !(st.true_residual <= prm.true_tol)) continue;
+      seconds[i * nms.size() + j] = sec;
+      if (sec < best_s) best_s = sec, best = p;
+    }
+  if (best_s == 1e300) throw std::runtime_error("autotune_solve: no candidate met the true-residual target");
   return best;
 }
 

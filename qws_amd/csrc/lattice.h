@@ -38,6 +38,9 @@ namespace qws {
 // (nin = 1: copy): the fastest of three launch shapes, each averaged over reps
 // launches; allocates and frees its buffers
 double measure_stream(long long bytes, int nin, int reps);
+// the multi-RHS clover product on the vector ALUs and on the matrix cores, timed
+// (mfma_trial.hip; measurement only): ms[0] VALU, ms[1] MFMA per launch
+void time_clover_rhs(long long V, int nrep, int reps, double ms[2], double* maxdiff);
 
 struct HipError : std::runtime_error {
   using std::runtime_error::runtime_error;
@@ -62,6 +65,11 @@ struct SolveParams {
   double true_tol = -1;
   int max_restarts = 0;
   int sap_prec = P32;
+  // with sap_prec = P16 and tol_s > 0: switch to the FP32 SAP (with nsap - 1, the same
+  // number of sweeps) when an inner solve ends with a relative residual >= 1 (near the
+  // critical kappa the FP16-stored SAP can make the inner BiCGStab diverge); 0 keeps
+  // the FP16 method throughout, as the reference
+  int escalate = 1;
 };
 
 // nsap and nm of QWS's SAP must not exceed 10 (prec_ddd_s_ exits otherwise,
@@ -71,6 +79,9 @@ void check_sap_params(int nsap, int nm);
 struct SolveStats {
   int outer_iters = 0, inner_iters = 0, restarts = 0;
   double true_residual = -1, seconds = 0;
+  // the FP16-stored SAP method (sap_prec = P16) switched to the FP32 SAP during the solve
+  // because an inner solve ended with a residual no smaller than its source's
+  int sap_escalated = 0;
 };
 
 class Lattice;
@@ -98,6 +109,8 @@ class OuterBatch {
   double* x(int k) const { return x_ + (size_t)k * stride_; }
   double* b(int k) const { return b_ + (size_t)k * stride_; }
   void release(int k) { state_[k] = EMPTY; }
+  // the batch switched from the FP16-stored SAP to the FP32 SAP (set_params resets it)
+  bool escalated() const { return escalated_; }
   int active_count() const;
 
  private:
@@ -115,6 +128,7 @@ class OuterBatch {
   Scalars* sc_;
   int* d_mask_;
   std::vector<int> state_, iters_, inner_;
+  bool escalated_ = false;
 };
 
 class Lattice {
@@ -207,6 +221,13 @@ class Lattice {
   void set_dslash_kernel(int k);
   int dslash_kernel() const { return g_.dslash_kernel; }
   int autotune_dslash(int reps);
+  // The solver's SAP settings tuned on this lattice: one solve of b from x = 0 (x is
+  // overwritten) for every (nsap, nm) of the candidate lists with prm's other settings,
+  // timed on the host clock; returns prm with the fastest pair whose FP64 true residual
+  // meets prm.true_tol (all of them when true_tol <= 0), seconds[i * nms + j] per pair
+  // (negative: did not meet it).  Also picks the full-operator kernel (autotune_dslash).
+  SolveParams autotune_solve(double* x, const double* b, const SolveParams& prm, const std::vector<int>& nsaps,
+                             const std::vector<int>& nms, std::vector<double>& seconds);
   void sync() { QWS_HIP(hipStreamSynchronize(st_)); }
 
   // ---- even/odd family (deo_*, mtilde) on x-compressed parity fields of V/2 sites
@@ -242,7 +263,9 @@ class Lattice {
 
   // ---- batched building blocks (used by OuterBatch)
   void dslash_batch(const Batch& B, double* out, const double* in);
-  void approx_inv(const Batch& B, Scalars* sc, const std::vector<int>& live, double* u, const double* p,
+  // returns true when the FP16-storage method's inner solve failed (some RHS ended with
+  // a relative residual >= 1 or not finite): the caller switches to the FP32 SAP
+  bool approx_inv(const Batch& B, Scalars* sc, const std::vector<int>& live, double* u, const double* p,
                   const SolveParams& prm, std::vector<int>& inner);
   Scalars* host_scalars() { return sc_host_; }
   void read_scalars(const Scalars* dev, int n);

@@ -89,6 +89,14 @@ void read_all(const std::vector<std::pair<Lattice*, std::pair<const Scalars*, in
   for (hipStream_t s : streams) QWS_HIP(hipStreamSynchronize(s));
 }
 
+// the FP32-SAP settings that replace a failed FP16-stored SAP: the same sweeps
+SolveParams escalated(const SolveParams& p) {
+  SolveParams q = p;
+  q.sap_prec = P32;
+  q.nsap = p.nsap > 1 ? p.nsap - 1 : 1;
+  return q;
+}
+
 bool any_of(const std::vector<int>& v) {
   for (int x : v)
     if (x) return true;
@@ -337,7 +345,15 @@ void Fused::bicg(std::vector<InnerJob>& J, double tol, int maxiter, int nsap, in
 // The FP16-storage method runs lattice after lattice.
 void Fused::approx_inv(std::vector<AIJob>& jobs, const SolveParams& prm, SapFusion* fz) {
   if (prm.sap_prec == P16 || !fz || jobs.size() == 1) {
-    for (AIJob& j : jobs) j.L->approx_inv(j.B, j.sc, *j.live, j.u, j.p, prm, *j.inner);
+    for (AIJob& j : jobs) {
+      if (!j.L->approx_inv(j.B, j.sc, *j.live, j.u, j.p, prm, *j.inner)) continue;
+      // the FP16-stored SAP's inner solve failed: this batch goes on with the FP32 SAP,
+      // starting with this preconditioner application, with the same number of sweeps
+      // (the FP16 method runs nsap - 1 of them; near the critical kappa the Jacobi
+      // block solves amplify, so an extra sweep would hurt)
+      j.escalate = true;
+      j.L->approx_inv(j.B, j.sc, *j.live, j.u, j.p, escalated(prm), *j.inner);
+    }
     return;
   }
   const size_t np = jobs.size();
@@ -404,15 +420,34 @@ int Fused::outer_step(std::vector<OuterBatch*>& obs, SapFusion* fz) {
       if (any_of(live[i])) v.push_back(i);
     return v;
   };
-  std::vector<AIJob> ai;
+  // u = M v for the batches in `which` (v = p or r), those with the same SAP precision
+  // together; a batch whose FP16-stored SAP failed continues with the FP32 SAP
+  auto apply_m = [&](const std::vector<size_t>& which, bool of_p) {
+    for (int pass = 0; pass < 2; ++pass) {
+      std::vector<AIJob> ai;
+      std::vector<size_t> idx;
+      for (size_t i : which) {
+        OuterBatch& o = *obs[i];
+        if ((o.prm_.sap_prec == P16) != (pass == 0)) continue;
+        ai.push_back({&o.L_, batch(i), o.sc_, &live[i], o.u_, of_p ? o.p_ : o.r_, &o.inner_, false});
+        idx.push_back(i);
+      }
+      if (ai.empty()) continue;
+      approx_inv(ai, obs[idx[0]]->prm_, fz);
+      for (size_t j = 0; j < ai.size(); ++j)
+        if (ai[j].escalate) {
+          obs[idx[j]]->prm_ = escalated(obs[idx[j]]->prm_);
+          obs[idx[j]]->escalated_ = true;
+        }
+    }
+  };
   for (size_t i : act) {
     OuterBatch& o = *obs[i];
     o.upload_mask(live[i]);
     for (int k = 0; k < o.cap_; ++k) o.iters_[k] += live[i][k];
-    ai.push_back({&o.L_, batch(i), o.sc_, &live[i], o.u_, o.p_, &o.inner_});
   }
   // u = M p ; q = D u ; alpha ; x += alpha u ; r -= alpha q
-  approx_inv(ai, obs[act[0]]->prm_, fz);
+  apply_m(act, true);
   for (size_t i : act) {
     OuterBatch& o = *obs[i];
     Lattice& L = o.L_;
@@ -429,12 +464,7 @@ int Fused::outer_step(std::vector<OuterBatch*>& obs, SapFusion* fz) {
   std::vector<size_t> act2 = still(act);
   if (!act2.empty()) {
     // u = M r ; t = D u ; omega ; x += omega u ; r -= omega t ; rho, beta
-    ai.clear();
-    for (size_t i : act2) {
-      OuterBatch& o = *obs[i];
-      ai.push_back({&o.L_, batch(i), o.sc_, &live[i], o.u_, o.r_, &o.inner_});
-    }
-    approx_inv(ai, obs[act2[0]]->prm_, fz);
+    apply_m(act2, false);
     for (size_t i : act2) {
       OuterBatch& o = *obs[i];
       Lattice& L = o.L_;
@@ -532,6 +562,7 @@ std::vector<std::vector<SolveStats>> Fused::solve(const std::vector<Lattice*>& l
     if (x[i].empty()) continue;
     Lattice& L = *lats[i];
     OuterBatch& ob = *L.batch_;
+    for (size_t k = 0; k < x[i].size(); ++k) out[i][k].sap_escalated = ob.escalated() ? 1 : 0;
     for (size_t k = 0; k < x[i].size(); ++k)
       if (ob.running((int)k) || ob.finished((int)k))
         QWS_HIP(hipMemcpyAsync(x[i][k], ob.x((int)k), L.spinor_bytes(P64), hipMemcpyDeviceToDevice, L.st_));

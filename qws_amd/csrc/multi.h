@@ -111,6 +111,7 @@ struct Fused {
     double* u;
     const double* p;
     std::vector<int>* inner;
+    bool escalate;  // out: the FP16-stored SAP failed and the FP32 SAP redid the job
   };
   static void approx_inv(std::vector<AIJob>& jobs, const SolveParams& prm, SapFusion* fz);
   // one outer iteration of every running slot of every batch (each on its own
