@@ -134,8 +134,9 @@ PRECISION_WORKLOAD = dict(L=[48, 48, 48, 96], B=[4, 4, 4, 4], eps=0.25, seed=3, 
 
 def precision_variant(Q, kappa=None, budget_s=240.0):
     """48^3x96: FP16-stored SAP (FP32 arithmetic), FP32 inner, FP64 outer to a 1e-10 FP64
-    true residual in one solve (near the critical kappa the FP16-stored SAP can make an
-    inner solve diverge; the solve then goes on with the FP32 SAP: sap_escalated).  Its
+    true residual in one solve, the outer iteration restarted every 4 iterations (near
+    the critical kappa the FP16-stored SAP can make an inner solve diverge; the solve
+    then goes on with the FP32 SAP: sap_escalated).  Its
     error against an FP64-refined continuation: the FP32-SAP mixed solve started from
     the result and driven to a 1e-13 true residual (the preconditioner's precision
     bounds only the convergence rate, not the accuracy of the FP64-refined solution).
@@ -144,7 +145,9 @@ def precision_variant(Q, kappa=None, budget_s=240.0):
     lat, b = setup(Q, w)
     progress("48^3x96 fields ready")
     x = lat.spinor(Q.F64)
-    prm = dict(SOLVE, sap_precision=Q.F16, nsap=3, nm=6, maxiter=200, maxiter_s=200)
+    # outer BiCGStab restarted from x every 4 iterations (maxiter 5), as long as the true
+    # residual misses 1e-10; a switch to the FP32 SAP holds for the rest of the solve
+    prm = dict(SOLVE, sap_precision=Q.F16, nsap=3, nm=6, maxiter=5, max_restarts=60, maxiter_s=200)
     lat.solve(x, b, **dict(prm, maxiter=2, true_tol=-1.0, max_restarts=0))  # warm-up
     x.zero()
     lat.sync()
@@ -155,7 +158,7 @@ def precision_variant(Q, kappa=None, budget_s=240.0):
              % (st["outer_iters"], st["inner_iters"], st["true_residual"], st["sap_escalated"], sec))
     xr = lat.spinor(Q.F64, x.download())
     t1 = time.perf_counter()
-    ref = lat.solve(xr, b, **dict(SOLVE, tol=1e-14, maxiter=200, maxiter_s=200, true_tol=1e-13))
+    ref = lat.solve(xr, b, **dict(SOLVE, tol=1e-14, maxiter=5, max_restarts=60, maxiter_s=200, true_tol=1e-13))
     ref_sec = time.perf_counter() - t1
     progress("FP64-refined continuation: true residual %.2e, %.0f s" % (ref["true_residual"], ref_sec))
     a, r = x.download(), xr.download()
@@ -287,7 +290,7 @@ def dropin_solve(Q, reps=2):
 SERVE_VOLUMES = ([8, 8, 8, 8], [16, 16, 16, 32], [24, 24, 24, 48], [32, 32, 32, 64])
 
 
-def uneven_batch(Q, params):
+def uneven_batch(Q, params, fused_only=0):
     """16^3x32, 24^3x48 and 32^3x64 (links exp(0.3 iH), seeds 9-11; Gaussian sources with
     seeds 4-6; kappa drawn from U[0.130, 0.137]) solved at once with every SAP half-sweep
     fused over the three lattices (qwsg_solve_multi), against the same three solves one
@@ -320,6 +323,15 @@ def uneven_batch(Q, params):
             lat.sync()
         return time.perf_counter() - t0, st
 
+    if fused_only:
+        # only the fused solves (for kernel traces): warm-up, then `fused_only` timed solves
+        fused()
+        ts = [fused()[0] for _ in range(fused_only)]
+        st = fused()[1]
+        for lat in lats:
+            lat.close()
+        return {"fused_seconds": ts, "kappa": kap, "inner_iters": [s[0]["inner_iters"] for s in st],
+                "outer_iters": [s[0]["outer_iters"] for s in st]}
     fused()  # warm-up (buffers, descriptor tables, graphs)
     separate()
     t_sep, st_sep = separate()
@@ -429,6 +441,8 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--uneven-only", type=int, default=0,
+                    help="only N fused solves of the uneven batch (16^3x32, 24^3x48, 32^3x64), for kernel traces")
     ap.add_argument("--mfma-trial", type=int, default=0,
                     help="time the multi-RHS clover product on the VALU and on MFMA (N launches each)")
     ap.add_argument("--dslash-only", type=int, default=0, help="only time N dslash launches (for counter runs)")
@@ -484,9 +498,12 @@ def main():
             print(json.dumps({"rate_req_per_s": rate, **serve_stats(res, tr)}), flush=True)
         srv.close()
         return
+    if a.uneven_only:
+        print(json.dumps(uneven_batch(Q, SOLVE, fused_only=a.uneven_only)), flush=True)
+        return
     if a.mfma_trial:
         # the multi-RHS clover product, VALU vs MFMA, on 2^20 sites x 16 RHS
-        for nrep in (1, 4, 16, 64):
+        for nrep in (1, 4, 7, 16, 64):
             ms_v, ms_m, diff = Q.time_clover_rhs(1 << 20, nrep, a.mfma_trial)
             useful = (1 << 20) * 16 * nrep * 2 * 144 * 2  # real-form 12x12 products, flops
             print(json.dumps({"nrep": nrep, "valu_ms": ms_v, "mfma_ms": ms_m, "mfma_speedup": ms_v / ms_m,

@@ -10,7 +10,8 @@
 // SAP's Jacobi steps apply the clover to data in registers and LDS, so the trial is
 // arithmetic-bound as the SAP kernel is):
 //   k_clover_rhs_valu: one lane per (site, RHS), 64 lanes = 4 sites x 16 RHS, FP32 FMAs;
-//   k_clover_rhs_mfma: one wave per site, v_mfma_f32_16x16x16f16 with A = the site's
+//   k_clover_rhs_mfma: one wave per site at a time (a grid-stride loop over the sites),
+//     v_mfma_f32_16x16x16f16 with A = the site's
 //     block (12x12 padded to 16x16), B = its 16 RHS (K = 12 of 16): lane l holds
 //     A[l % 16][4 (l / 16) .. +3] and B[4 (l / 16) .. +3][l % 16], and receives
 //     D[4 (l / 16) .. +3][l % 16] -- the B layout of the next product.
@@ -19,6 +20,7 @@
 #include <hip/hip_fp16.h>
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <stdexcept>
 
 #include "lattice.h"
@@ -39,62 +41,62 @@ __host__ __device__ inline int herm_pair(int i, int j) {
   return i * 5 - i * (i - 1) / 2 + (j - i - 1);
 }
 
-// the real-form entry A[row][k] (row, k < 12) of a packed block c (36 reals)
-__device__ __forceinline__ float real_form(const __half* c, int row, int k) {
-  if (row >= 12 || k >= 12) return 0.0f;
+// The A operand entry this lane supplies, A[row][k]: packed position and sign, or -1
+// for the zero padding; the same for every site.
+__device__ __forceinline__ void real_form_entry(int row, int k, int& pos, float& sign) {
+  pos = -1, sign = 1.0f;
+  if (row >= 12 || k >= 12) return;
   const int i = row % 6, j = k % 6;
   const bool im_row = row >= 6, im_col = k >= 6;
-  float re, im;
+  const bool re_part = im_row == im_col;  // [[Re, -Im], [Im, Re]]
   if (i == j) {
-    re = __half2float(c[i]);
-    im = 0.0f;
-  } else if (i < j) {
-    const int p = 6 + 2 * herm_pair(i, j);
-    re = __half2float(c[p]);
-    im = __half2float(c[p + 1]);
-  } else {
-    const int p = 6 + 2 * herm_pair(j, i);
-    re = __half2float(c[p]);
-    im = -__half2float(c[p + 1]);
+    if (re_part) pos = i;  // the diagonal is real: its imaginary blocks vanish
+    return;
   }
-  // [[Re, -Im], [Im, Re]]
-  if (!im_row && !im_col) return re;
-  if (!im_row && im_col) return -im;
-  if (im_row && !im_col) return im;
-  return re;
+  const int p = 6 + 2 * (i < j ? herm_pair(i, j) : herm_pair(j, i));
+  pos = re_part ? p : p + 1;
+  // imaginary part of A[i][j]: +im for i < j, -im below the diagonal (conjugate); the
+  // upper-right block carries a minus sign
+  float s_im = i < j ? 1.0f : -1.0f;
+  if (!re_part && !im_row) s_im = -s_im;
+  sign = re_part ? 1.0f : s_im;
 }
 
+// One wave per site at a time, looping over the sites (grid-stride): the lanes' A
+// entries are mapped once, then per site 8 half loads for A, one 8-byte load per
+// chirality for B, 2 MFMAs per repetition, one 8-byte store per chirality.
 __global__ void __launch_bounds__(256) k_clover_rhs_mfma(long long V, int nrep, const __half* __restrict__ C,
                                                          const __half* __restrict__ x, __half* __restrict__ y) {
   const int lane = threadIdx.x & 63;
-  const long long s = (long long)blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
-  if (s >= V) return;  // whole waves
   const int r = lane & 15, g = lane >> 4;
-  h4 a[2], b[2];
+  int pos[4];
+  float sgn[4];
 #pragma unroll
-  for (int chi = 0; chi < 2; ++chi) {
-    const __half* c = C + (s * 2 + chi) * 36;
-#pragma unroll
-    for (int j = 0; j < 4; ++j) a[chi][j] = (_Float16)real_form(c, r, 4 * g + j);
-    const __half* xs = x + ((s * NR + r) * 2 + chi) * 12;
-#pragma unroll
-    for (int j = 0; j < 4; ++j) b[chi][j] = 4 * g + j < 12 ? (_Float16)__half2float(xs[4 * g + j]) : (_Float16)0.0f;
-  }
-  for (int n = 0; n < nrep; ++n) {
+  for (int j = 0; j < 4; ++j) real_form_entry(r, 4 * g + j, pos[j], sgn[j]);
+  const long long nw = (long long)gridDim.x * (blockDim.x / 64);
+  for (long long s = (long long)blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6); s < V; s += nw) {
+    h4 a[2], b[2];
 #pragma unroll
     for (int chi = 0; chi < 2; ++chi) {
-      const f4 zero = {0.0f, 0.0f, 0.0f, 0.0f};
-      const f4 d = __builtin_amdgcn_mfma_f32_16x16x16f16(a[chi], b[chi], zero, 0, 0, 0);
+      const __half* c = C + (s * 2 + chi) * 36;
 #pragma unroll
-      for (int j = 0; j < 4; ++j) b[chi][j] = (_Float16)d[j];
+      for (int j = 0; j < 4; ++j) a[chi][j] = pos[j] >= 0 ? (_Float16)(sgn[j] * __half2float(c[pos[j]])) : (_Float16)0.0f;
+      if (g < 3) b[chi] = *reinterpret_cast<const h4*>(x + ((s * NR + r) * 2 + chi) * 12 + 4 * g);
+      else b[chi] = h4{0, 0, 0, 0};
     }
-  }
+    for (int n = 0; n < nrep; ++n) {
 #pragma unroll
-  for (int chi = 0; chi < 2; ++chi) {
-    __half* ys = y + ((s * NR + r) * 2 + chi) * 12;
+      for (int chi = 0; chi < 2; ++chi) {
+        const f4 zero = {0.0f, 0.0f, 0.0f, 0.0f};
+        const f4 d = __builtin_amdgcn_mfma_f32_16x16x16f16(a[chi], b[chi], zero, 0, 0, 0);
 #pragma unroll
-    for (int j = 0; j < 4; ++j)
-      if (4 * g + j < 12) ys[4 * g + j] = __float2half((float)b[chi][j]);
+        for (int j = 0; j < 4; ++j) b[chi][j] = (_Float16)d[j];
+      }
+    }
+    if (g < 3) {
+#pragma unroll
+      for (int chi = 0; chi < 2; ++chi) *reinterpret_cast<h4*>(y + ((s * NR + r) * 2 + chi) * 12 + 4 * g) = b[chi];
+    }
   }
 }
 
@@ -225,7 +227,7 @@ void time_clover_rhs(long long V, int nrep, int reps, double ms[2], double* maxd
     QWS_HIP(hipEventCreate(&e0));
     QWS_HIP(hipEventCreate(&e1));
     hipLaunchKernelGGL(k_clover_rhs_init, dim3((unsigned)((V + 255) / 256)), dim3(256), 0, st, V, C, x);
-    const dim3 gv((unsigned)((V * NR + 255) / 256)), gm((unsigned)((V + 3) / 4));
+    const dim3 gv((unsigned)((V * NR + 255) / 256)), gm((unsigned)std::min<long long>((V + 3) / 4, 4096));
     for (int which = 0; which < 2; ++which) {
       __half* y = which ? y1 : y0;
       auto launch = [&] {
