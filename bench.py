@@ -441,6 +441,8 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--dslash-sweep", type=int, default=0,
+                    help="time the full operator for every block order and kernel variant (N launches each)")
     ap.add_argument("--uneven-only", type=int, default=0,
                     help="only N fused solves of the uneven batch (16^3x32, 24^3x48, 32^3x64), for kernel traces")
     ap.add_argument("--mfma-trial", type=int, default=0,
@@ -488,9 +490,9 @@ def main():
         print(json.dumps(precision_variant(Q, a.precision_kappa, a.precision_budget)), flush=True)
         return
     if a.serve:
-        srv = make_server(Q)
-        srv.run(serve_trace(1000.0, 4, seed=1))  # warm-up
-        # 1 and 2 req/s lie below the server's capacity (~2-4 requests/s of this mix);
+        srv = make_server(Q, slots=8)
+        srv.run(serve_trace(1000.0, 8, seed=1))  # warm-up (fills the slot pool)
+        # 1 and 2 req/s lie below the server's capacity (~2.5 requests/s of this mix);
         # 5..200 is the configured sweep, all above it
         for rate in (1, 2, 5, 10, 20, 50, 100, 200):
             tr = serve_trace(rate, a.serve_requests)
@@ -515,6 +517,18 @@ def main():
         return
     lat, b = setup(Q)
     V = lat.V
+    if a.dslash_sweep:
+        # block order x kernel variant of the full operator (FP32), and FP64 D per order
+        for order in (0, 1):
+            lat.set_dslash_order(order)
+            for k in (0, 3, 4):
+                lat.set_dslash_kernel(k)
+                ms = lat.time_dslash(Q.F32, a.dslash_sweep)
+                print(json.dumps({"order": order, "kernel": k, "f32_ms": ms,
+                                  "GBs": V * BYTES_PER_SITE_F32_12 / (ms * 1e-3) / 1e9}), flush=True)
+            ms = lat.time_dslash(Q.F64, a.dslash_sweep)
+            print(json.dumps({"order": order, "f64_ms": ms, "GBs": V * 1536 / (ms * 1e-3) / 1e9}), flush=True)
+        return
     if a.dslash_only:
         ms = lat.time_dslash(Q.F32, a.dslash_only)
         print(json.dumps({"dslash_ms": ms, "GBs": V * BYTES_PER_SITE_F32_12 / (ms * 1e-3) / 1e9}))

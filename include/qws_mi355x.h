@@ -471,6 +471,13 @@ int qwsg_time_dslash(qwsg_lattice* lat, int prec, int reps, double* ms_per_call)
  * it in *chosen.  Tuning only; no counterpart in the reference. */
 int qwsg_set_dslash_kernel(qwsg_lattice* lat, int kernel);
 int qwsg_autotune_dslash(qwsg_lattice* lat, int reps, int* chosen);
+/* Block order of the launches over every block (tuning only): 0 index order, 1 t-z
+ * pencils inside each XCD's range of workgroups (t and z neighbours close in time, so
+ * their faces are read from that XCD's L2).  Results do not depend on it.
+ * qwsg_autotune_dslash times both orders with every kernel variant and keeps the
+ * fastest pair. */
+int qwsg_set_dslash_order(qwsg_lattice* lat, int order);
+int qwsg_get_dslash_order(qwsg_lattice* lat, int* order);
 /* The solver's SAP settings tuned on this lattice (no counterpart in the reference):
  * one solve of b from x = 0 (x is overwritten) for every (nsap[i], nm[j]) with p's
  * other settings, timed; *best = p with the fastest pair whose FP64 true residual meets

@@ -97,6 +97,8 @@ def lib():
         L.qwsg_time_dslash.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
         L.qwsg_set_dslash_kernel.argtypes = [vp, ctypes.c_int]
         L.qwsg_autotune_dslash.argtypes = [vp, ctypes.c_int, ip]
+        L.qwsg_set_dslash_order.argtypes = [vp, ctypes.c_int]
+        L.qwsg_get_dslash_order.argtypes = [vp, ip]
         L.qwsg_autotune_solve.argtypes = [vp, vp, vp, ctypes.POINTER(SolveParams), vp, ctypes.c_int, vp, ctypes.c_int,
                                           ctypes.POINTER(SolveParams), vp]
         L.qwsg_time_sap.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
@@ -500,6 +502,15 @@ class Lattice:
                                         ctypes.byref(best), sec))
         out = dict(base, nsap=best.nsap, nm=best.nm)
         return out, [[sec[i * len(nms) + j] for j in range(len(nms))] for i in range(len(nsaps))]
+
+    def set_dslash_order(self, order):
+        """Block order of the launches over every block: 0 index, 1 t-z pencils per XCD."""
+        check(lib().qwsg_set_dslash_order(self.h, int(order)))
+
+    def dslash_order(self):
+        o = ctypes.c_int()
+        check(lib().qwsg_get_dslash_order(self.h, ctypes.byref(o)))
+        return o.value
 
     def autotune_dslash(self, reps=20):
         """Time the full-operator kernels on this lattice, keep and return the fastest."""

@@ -397,6 +397,12 @@ int qwsg_autotune_solve(qwsg_lattice* h, void* x, const void* b, const qwsg_solv
     if (seconds) std::copy(sec.begin(), sec.end(), seconds);
   });
 }
+int qwsg_set_dslash_order(qwsg_lattice* h, int order) {
+  return guarded([&] { h->lat->set_dslash_order(order); });
+}
+int qwsg_get_dslash_order(qwsg_lattice* h, int* order) {
+  return guarded([&] { *order = h->lat->dslash_order(); });
+}
 int qwsg_autotune_dslash(qwsg_lattice* h, int reps, int* chosen) {
   return guarded([&] {
     const int k = h->lat->autotune_dslash(reps);

@@ -99,6 +99,24 @@ Lattice::Lattice(const int L[4], const int B[4], int tbc, int links32) {
   for (int b = 0; b < nblk; ++b) order[b] = b;
   d_order_ = dalloc<int>(nblk + 1);
   QWS_HIP(hipMemcpy(d_order_, order.data(), order.size() * sizeof(int), hipMemcpyHostToDevice));
+  {
+    // XCD x runs the workgroups [x q + min(x, r), ...) of a launch over every block
+    // (xcd_remap); inside each such range sort the blocks with t fastest, then z, x, y
+    std::vector<int> pen(order);
+    const int q = nblk / NUM_XCD, r = nblk % NUM_XCD;
+    auto key = [&](int b) {
+      int c[4], rem = b;
+      for (int m = 0; m < 4; ++m) c[m] = rem % g_.NB[m], rem /= g_.NB[m];
+      return (((long long)c[1] * g_.NB[0] + c[0]) * g_.NB[2] + c[2]) * g_.NB[3] + c[3];
+    };
+    for (int x = 0, lo = 0; x < NUM_XCD; ++x) {
+      const int n = q + (x < r ? 1 : 0);
+      std::stable_sort(pen.begin() + lo, pen.begin() + lo + n, [&](int a, int b) { return key(a) < key(b); });
+      lo += n;
+    }
+    d_order_pencil_ = dalloc<int>(nblk + 1);
+    QWS_HIP(hipMemcpy(d_order_pencil_, pen.data(), pen.size() * sizeof(int), hipMemcpyHostToDevice));
+  }
   // block lists per colour (colour = (bx+by+bz+bt)%2, qws.cc:275-276)
   std::vector<int> lists[2];
   for (int b : order) {
@@ -129,7 +147,7 @@ Lattice::~Lattice() {
   if (!owns_gauge_) Ud_ = nullptr, Us_ = nullptr;
   for (auto* p : {(void*)Ud_, (void*)Cd_, (void*)Us_, (void*)Cs_, (void*)Uh_, (void*)Ch_, (void*)part_, (void*)sc_in_,
                   (void*)sc_one_, (void*)sc_h_, (void*)sc_eo_, (void*)d_inner_mask_, (void*)d_blocks_[0], (void*)d_blocks_[1],
-                  (void*)d_order_})
+                  (void*)d_order_, (void*)d_order_pencil_})
     if (p) (void)hipFree(p);
   for (auto& w : work_) (void)hipFree(w.second.first);
   if (sc_host_) (void)hipHostFree(sc_host_);
@@ -407,7 +425,7 @@ template <> Lattice::FieldSet<__half> Lattice::fields<__half>() const {
 }
 
 BlockSel Lattice::sel(int colour) const {
-  if (colour < 0) return {d_order_, g_.nblk};
+  if (colour < 0) return {dslash_order_ == 1 ? d_order_pencil_ : d_order_, g_.nblk};
   return {d_blocks_[colour & 1], nblk_colour_[colour & 1]};
 }
 
@@ -1459,19 +1477,28 @@ void Lattice::set_dslash_kernel(int k) {
   g_.dslash_kernel = k;
 }
 
+void Lattice::set_dslash_order(int order) {
+  if (order != 0 && order != 1) throw std::invalid_argument("dslash order: 0 (index) or 1 (t-z pencils per XCD)");
+  dslash_order_ = order;
+}
+
 int Lattice::autotune_dslash(int reps) {
   if (reps < 1) throw std::invalid_argument("autotune_dslash: reps must be positive");
-  int best = g_.dslash_kernel;
+  int best = g_.dslash_kernel, best_order = dslash_order_;
   double best_ms = 1e30;
-  for (int k : {0, 3, 4}) {
-    g_.dslash_kernel = k;
-    const double ms = time_dslash(P32, reps);
-    if (ms < best_ms) {
-      best_ms = ms;
-      best = k;
+  for (int order : {0, 1})
+    for (int k : {0, 3, 4}) {
+      g_.dslash_kernel = k;
+      dslash_order_ = order;
+      const double ms = time_dslash(P32, reps);
+      if (ms < best_ms) {
+        best_ms = ms;
+        best = k;
+        best_order = order;
+      }
     }
-  }
   g_.dslash_kernel = best;
+  dslash_order_ = best_order;
   return best;
 }
 

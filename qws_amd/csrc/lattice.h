@@ -219,6 +219,12 @@ class Lattice {
   // or as a separate pass; both give the same results bit for bit
   void set_sap_source_update(bool on) { fold_pupd_ = on; }
   void set_dslash_kernel(int k);
+  // Block order of the launches over every block (the full operator D among them):
+  // 0 = index order (x fastest); 1 = inside each XCD's contiguous range of workgroups,
+  // the blocks sorted with t fastest, then z, x, y -- so that the t and z neighbours of
+  // a block are read while they are still in that XCD's L2.  Results do not depend on it.
+  void set_dslash_order(int order);
+  int dslash_order() const { return dslash_order_; }
   int dslash_kernel() const { return g_.dslash_kernel; }
   int autotune_dslash(int reps);
   // The solver's SAP settings tuned on this lattice: one solve of b from x = 0 (x is
@@ -307,7 +313,9 @@ class Lattice {
   double kappa_ = 0;
   hipStream_t st_ = nullptr;
   int* d_blocks_[2] = {nullptr, nullptr};
-  int* d_order_ = nullptr;  // every block, Morton order
+  int* d_order_ = nullptr;  // every block, in index order
+  int* d_order_pencil_ = nullptr;  // every block, t-z pencils inside each XCD range
+  int dslash_order_ = 0;
   int nblk_colour_[2] = {0, 0};
   // fields
   double *Ud_ = nullptr, *Cd_ = nullptr;

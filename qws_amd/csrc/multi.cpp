@@ -423,12 +423,16 @@ int Fused::outer_step(std::vector<OuterBatch*>& obs, SapFusion* fz) {
   // u = M v for the batches in `which` (v = p or r), those with the same SAP precision
   // together; a batch whose FP16-stored SAP failed continues with the FP32 SAP
   auto apply_m = [&](const std::vector<size_t>& which, bool of_p) {
+    // each batch's SAP precision as the half-step starts (a batch that switches during
+    // the FP16 pass has its preconditioner application redone there, not again below)
+    std::vector<char> f16(obs.size(), 0);
+    for (size_t i : which) f16[i] = obs[i]->prm_.sap_prec == P16;
     for (int pass = 0; pass < 2; ++pass) {
       std::vector<AIJob> ai;
       std::vector<size_t> idx;
       for (size_t i : which) {
         OuterBatch& o = *obs[i];
-        if ((o.prm_.sap_prec == P16) != (pass == 0)) continue;
+        if ((f16[i] != 0) != (pass == 0)) continue;
         ai.push_back({&o.L_, batch(i), o.sc_, &live[i], o.u_, of_p ? o.p_ : o.r_, &o.inner_, false});
         idx.push_back(i);
       }

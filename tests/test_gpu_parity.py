@@ -90,23 +90,33 @@ def test_dslash_fp64_fp32(prob, hops, colour):
 @pytest.mark.parametrize("su3", [True, False])
 def test_dslash_kernel_variants(su3):
     """The FP32 full-operator kernels on 256-site blocks (global gathers, LDS-staged
-    for 3 or 4 waves) agree with the oracle and with each other bit for bit (same hop
-    order and arithmetic); the autotuner keeps one of them."""
+    for 3 or 4 waves), in either block order, agree with the oracle and with each other
+    bit for bit (same hop order and arithmetic); the autotuner keeps one pair."""
     p = Problem([16, 8, 8, 8], [4, 4, 4, 4], kappa=0.125, seed=12, su3=su3)
     src = p.b.astype(np.float32)
     inp = p.lat.spinor(Q.F32, src)
     want = p.orc.dslash(src, Q.HOPS_ALL, -1, out=np.zeros_like(src))
     outs = {}
-    for k in (0, 3, 4):
-        p.lat.set_dslash_kernel(k)
-        out = p.lat.spinor(Q.F32, np.zeros_like(p.b))
-        p.lat.dslash(out, inp, Q.HOPS_ALL, -1)
-        outs[k] = out.download()
-        assert rel(outs[k], want) < TOL32, k
-    assert all(np.array_equal(outs[0], outs[k]) for k in (3, 4))
-    assert p.lat.autotune_dslash(3) in (0, 3, 4)
+    for order in (0, 1):  # index order, or t-z pencils inside each XCD's range
+        p.lat.set_dslash_order(order)
+        for k in (0, 3, 4):
+            p.lat.set_dslash_kernel(k)
+            out = p.lat.spinor(Q.F32, np.zeros_like(p.b))
+            p.lat.dslash(out, inp, Q.HOPS_ALL, -1)
+            outs[order, k] = out.download()
+            assert rel(outs[order, k], want) < TOL32, (order, k)
+    assert all(np.array_equal(outs[0, 0], v) for v in outs.values())
+    # the FP64 operator in pencil order as well
+    p.lat.set_dslash_order(1)
+    o64 = p.lat.spinor(Q.F64)
+    p.lat.dslash(o64, p.lat.spinor(Q.F64, p.b), Q.HOPS_ALL, -1)
+    assert rel(o64.download(), p.orc.dslash(p.b, Q.HOPS_ALL, -1, out=np.zeros_like(p.b))) < TOL64
+    assert p.lat.autotune_dslash(3) in (0, 3, 4) and p.lat.dslash_order() in (0, 1)
     with pytest.raises(Q.QwsError):
         p.lat.set_dslash_kernel(7)
+    with pytest.raises(Q.QwsError):
+        p.lat.set_dslash_order(2)
+    p.lat.set_dslash_order(0)
 
 
 @pytest.mark.parametrize("colour", [-1, 0, 1])
