@@ -632,6 +632,12 @@ def main():
     line["served"]["c2_equiv_vs_plain"] = line["served"]["c2_equiv_solves_per_s"] / line["value"]
     srv.close()
     progress("served trace done")
+    # the precision variant's lattice (48^3x96, eps 0.25, csw 1.1) with the FP16-stored SAP at
+    # kappa = 0.13; kappa = 0.137 (near critical, minutes) runs with --precision
+    line["precision"] = precision_variant(Q, 0.13)
+    line["precision"]["note"] = ("kappa = 0.137 of the precision configuration: bench.py --precision "
+                                 "(profiles/precision_48c96.md)")
+    progress("precision variant (kappa 0.13) done")
     if not a.no_cpu_baseline:
         progress("timing the CPU reference")
         with tempfile.TemporaryDirectory() as td:
