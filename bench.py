@@ -569,8 +569,12 @@ def main():
     ms_s = lat.time_sap(Q.F32, 10, SOLVE["nsap"], SOLVE["nm"])
     flops_s = sap_flops_per_site(WORKLOAD["B"], SOLVE["nsap"], SOLVE["nm"]) * V / 2
     achieved_s = flops_s / (ms_s * 1e-3) / 1e12
-    # practical HBM ceiling: streaming kernels with D's read:write mix (6:1) and a copy
-    stream = {"read6_write1_GBs": Q.stream_bandwidth(256 << 20, 6, 20), "copy_GBs": Q.stream_bandwidth(1 << 30, 1, 10)}
+    # practical HBM ceiling: streaming kernels with D's read:write mix (6:1) over six arrays
+    # and over one array read region by region, and a copy; D is compared with the best
+    stream = {"read6_write1_GBs": Q.stream_bandwidth(256 << 20, 6, 20),
+              "read6_write1_one_stream_GBs": Q.stream_bandwidth(256 << 20, 6, 20, one_stream=True),
+              "copy_GBs": Q.stream_bandwidth(1 << 30, 1, 10)}
+    stream_best = max(stream.values())
 
     def traffic(name):
         tfile = os.path.join(ROOT, "profiles", name)
@@ -599,7 +603,7 @@ def main():
                        "kernel": ("k_dslash<float,12>" if d_kernel == 0 else "k_dslash_lds<12,%d>" % d_kernel)
                                  + " (FP32 D, 672 B/site algorithmic; kernel chosen by autotune_dslash)",
                        "ms_per_launch": ms_d, "stream_ceiling": stream,
-                       "frac_of_stream": achieved / stream["read6_write1_GBs"]},
+                       "frac_of_stream": achieved / stream_best},
         "cpu_baseline": None,
     }
     line["error_vs_fp64"] = error_vs_fp64(Q, lat, x, b)

@@ -276,6 +276,11 @@ int qwsg_device_count(int* n);
  * (nin + 1) * bytes of device memory for the call.  Measurement only; no
  * counterpart in the reference. */
 int qwsg_stream_bandwidth(long long bytes, int nin, int reps, double* gbs);
+/* The same with a single input stream: each 4 KB output chunk is the sum of nin
+ * consecutive 4 KB chunks of one input array of nin * bytes (a read-mostly kernel
+ * whose workgroups each read one contiguous region).  Measurement only; no
+ * counterpart in the reference. */
+int qwsg_stream_bandwidth_one(long long bytes, int nin, int reps, double* gbs);
 /* Measurement only, no counterpart in the reference: the multi-RHS clover product (the
  * two Hermitian 6x6 blocks of the clover inverse, clover_def.h:88-284, applied to 16
  * right-hand sides per site, FP16 storage, FP32 accumulation, nrep products on data in

@@ -108,6 +108,8 @@ def lib():
                                            ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]
         L.qwsg_stream_bandwidth.argtypes = [ctypes.c_longlong, ctypes.c_int, ctypes.c_int,
                                             ctypes.POINTER(ctypes.c_double)]
+        L.qwsg_stream_bandwidth_one.argtypes = [ctypes.c_longlong, ctypes.c_int, ctypes.c_int,
+                                                ctypes.POINTER(ctypes.c_double)]
         L.qwsg_generate_gauge.argtypes = [vp, ctypes.c_double, ctypes.c_ulonglong]
         L.qwsg_compute_clover.argtypes = [vp, ctypes.c_double]
         L.qwsg_gauge_download.argtypes = [vp, vp]
@@ -525,10 +527,13 @@ class Lattice:
         return ms.value
 
 
-def stream_bandwidth(nbytes, nin=1, reps=20):
-    """Measured GB/s of a streaming kernel summing nin arrays of nbytes into one."""
+def stream_bandwidth(nbytes, nin=1, reps=20, one_stream=False):
+    """Measured GB/s of a streaming kernel summing nin arrays of nbytes into one
+    (one_stream: the nin inputs are consecutive chunks of one array, read region by
+    region)."""
     g = ctypes.c_double()
-    check(lib().qwsg_stream_bandwidth(int(nbytes), int(nin), int(reps), ctypes.byref(g)))
+    f = lib().qwsg_stream_bandwidth_one if one_stream else lib().qwsg_stream_bandwidth
+    check(f(int(nbytes), int(nin), int(reps), ctypes.byref(g)))
     return g.value
 
 

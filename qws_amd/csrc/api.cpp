@@ -44,6 +44,9 @@ int qwsg_time_clover_rhs(long long V, int nrep, int reps, double* ms, double* ma
 int qwsg_stream_bandwidth(long long bytes, int nin, int reps, double* gbs) {
   return guarded([&] { *gbs = qws::measure_stream(bytes, nin, reps); });
 }
+int qwsg_stream_bandwidth_one(long long bytes, int nin, int reps, double* gbs) {
+  return guarded([&] { *gbs = qws::measure_stream(bytes, nin, reps, true); });
+}
 
 int qwsg_create(const int* L, const int* B, int tbc, int links32, qwsg_lattice** out) {
   return guarded([&] {

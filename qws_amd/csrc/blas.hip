@@ -489,9 +489,34 @@ __global__ void __launch_bounds__(256) k_stream(long long n4, int nin, const flo
   }
 }
 
-// cfg 0: one element per thread; 1: 4 per thread; 2: grid-stride over 2048 workgroups
+// One input stream: output chunk c (256 float4) is the sum of the nin consecutive
+// 256-float4 chunks c*nin .. c*nin+nin-1 of `in`, so that every workgroup reads one
+// contiguous region (a read-mostly kernel over a single array).
+__global__ void __launch_bounds__(256) k_stream_one(long long nchunk, int nin, const float4* __restrict__ in,
+                                                    float4* __restrict__ out) {
+  for (long long c = blockIdx.x; c < nchunk; c += gridDim.x) {
+    const float4* p = in + c * nin * 256 + threadIdx.x;
+    float4 s = p[0];
+    for (int k = 1; k < nin; ++k) {
+      const float4 v = p[k * 256];
+      s.x += v.x;
+      s.y += v.y;
+      s.z += v.z;
+      s.w += v.w;
+    }
+    out[c * 256 + threadIdx.x] = s;
+  }
+}
+
+// cfg 0: one element per thread; 1: 4 per thread; 2: grid-stride over 2048 workgroups;
+// 3, 4: one input stream (k_stream_one; n4 a multiple of 256), one workgroup per output
+// chunk or grid-stride over 2048 workgroups
 void launch_stream(int cfg, long long n4, int nin, const float4* in, float4* out, hipStream_t st) {
-  if (cfg == 0) {
+  if (cfg >= 3) {
+    const long long nchunk = n4 / 256;
+    const unsigned grid = cfg == 3 ? (unsigned)nchunk : 2048u;
+    hipLaunchKernelGGL(k_stream_one, dim3(grid), dim3(256), 0, st, nchunk, nin, in, out);
+  } else if (cfg == 0) {
     hipLaunchKernelGGL(k_stream<1>, dim3((unsigned)((n4 + 255) / 256)), dim3(256), 0, st, n4, nin, in, out);
   } else if (cfg == 1) {
     hipLaunchKernelGGL(k_stream<4>, dim3((unsigned)((n4 + 1023) / 1024)), dim3(256), 0, st, n4, nin, in, out);

@@ -105,7 +105,9 @@ void launch_scale_rms(const Batch& B, long long V, long long so, long long si, S
 // *out ^= order-independent digest of nwords 8-byte words (clear *out first)
 void launch_digest(long long nwords, const void* data, unsigned long long* out, hipStream_t st);
 // HBM probe: out = sum of the nin consecutive arrays of n4 float4 in `in` (nin = 1: copy);
-// cfg 0/1/2 = launch shape (1 or 4 elements per thread, or grid-stride)
+// cfg 0/1/2 = launch shape (1 or 4 elements per thread, or grid-stride); cfg 3/4: out
+// chunk c (256 float4) = sum of the consecutive input chunks c*nin .. c*nin+nin-1
+// (one input stream; n4 a multiple of 256), one workgroup per chunk or grid-stride
 void launch_stream(int cfg, long long n4, int nin, const float4* in, float4* out, hipStream_t st);
 // out = a - b (real arrays of length n)
 template <typename R> void launch_sub(long long n, R* out, const R* a, const R* b, hipStream_t st);

@@ -1432,9 +1432,9 @@ double Lattice::time_sap_t(int reps, int nsap, int nm) {
   return ms / (reps * (2.0 * nsap + 2));
 }
 
-double measure_stream(long long bytes, int nin, int reps) {
+double measure_stream(long long bytes, int nin, int reps, bool one) {
   if (bytes < (1 << 20) || nin < 1 || nin > 16 || reps < 1) throw std::invalid_argument("measure_stream: bad arguments");
-  const long long n4 = bytes / 16;
+  const long long n4 = bytes / 16 / 256 * 256;
   float4 *in = nullptr, *out = nullptr;
   QWS_HIP(hipMalloc((void**)&in, 16 * n4 * nin));
   if (hipMalloc((void**)&out, 16 * n4) != hipSuccess) {
@@ -1448,7 +1448,7 @@ double measure_stream(long long bytes, int nin, int reps) {
     QWS_HIP(hipEventCreate(&e0));
     QWS_HIP(hipEventCreate(&e1));
     ms = 1e30f;
-    for (int cfg = 0; cfg < 3; ++cfg) {  // the fastest launch shape
+    for (int cfg = one ? 3 : 0; cfg < (one ? 5 : 3); ++cfg) {  // the fastest launch shape
       float t = 0;
       launch_stream(cfg, n4, nin, in, out, nullptr);  // warm up
       QWS_HIP(hipEventRecord(e0, nullptr));

@@ -36,8 +36,9 @@ namespace qws {
 
 // GB/s (reads + writes) of a streaming kernel summing nin arrays of `bytes` into one
 // (nin = 1: copy): the fastest of three launch shapes, each averaged over reps
-// launches; allocates and frees its buffers
-double measure_stream(long long bytes, int nin, int reps);
+// launches; allocates and frees its buffers.  one: the nin inputs are consecutive
+// chunks of one array, read region by region (a single input stream).
+double measure_stream(long long bytes, int nin, int reps, bool one = false);
 // the multi-RHS clover product on the vector ALUs and on the matrix cores, timed
 // (mfma_trial.hip; measurement only): ms[0] VALU, ms[1] MFMA per launch
 void time_clover_rhs(long long V, int nrep, int reps, double ms[2], double* maxdiff);

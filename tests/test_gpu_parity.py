@@ -508,10 +508,13 @@ def test_stream_bandwidth_probe():
     """The HBM probe runs (copy and a 6:1 read-mostly sum) and reports a plausible
     rate; bad arguments are refused."""
     for nin in (1, 6):
-        g = Q.stream_bandwidth(64 << 20, nin, 3)
-        assert 100 < g < 20000, (nin, g)
+        for one in (False, True):
+            g = Q.stream_bandwidth(64 << 20, nin, 3, one_stream=one)
+            assert 100 < g < 20000, (nin, one, g)
     with pytest.raises(Q.QwsError):
         Q.stream_bandwidth(1024, 1, 1)
+    with pytest.raises(Q.QwsError):
+        Q.stream_bandwidth(64 << 20, 17, 1, one_stream=True)
 
 
 def test_inner_bicgstab(prob):
