@@ -458,6 +458,8 @@ def main():
     ap.add_argument("--serve", action="store_true",
                     help="serving sweep: Poisson rates 1..200 req/s over four resident fields (p50/p99 latency)")
     ap.add_argument("--serve-requests", type=int, default=24)
+    ap.add_argument("--serve-slots", type=int, default=4, help="concurrent requests of the server (1..16)")
+    ap.add_argument("--serve-rates", default="1,2,5,10,20,50,100,200", help="offered rates of the sweep (req/s)")
     ap.add_argument("--only-plain", action="store_true", help="only the plain solves and the D roofline (profiling)")
     ap.add_argument("--precision", action="store_true",
                     help="the 48^3x96 FP16-SAP precision variant and its error against an FP64-refined solution")
@@ -490,14 +492,16 @@ def main():
         print(json.dumps(precision_variant(Q, a.precision_kappa, a.precision_budget)), flush=True)
         return
     if a.serve:
-        srv = make_server(Q, slots=8)
-        srv.run(serve_trace(1000.0, 8, seed=1))  # warm-up (fills the slot pool)
-        # 1 and 2 req/s lie below the server's capacity (~2.5 requests/s of this mix);
-        # 5..200 is the configured sweep, all above it
-        for rate in (1, 2, 5, 10, 20, 50, 100, 200):
+        srv = make_server(Q, slots=a.serve_slots)
+        # warm-up: the same requests (the trace's shapes do not depend on the rate) at once,
+        # which fills the slot pool
+        srv.run(serve_trace(1000.0, a.serve_requests))
+        # 1 and 2 req/s lie below the server's capacity (~4.2 requests/s of this mix with
+        # 4 slots); 5..200 is the configured sweep, at or above it
+        for rate in [float(v) for v in a.serve_rates.split(",")]:
             tr = serve_trace(rate, a.serve_requests)
             res = srv.run(tr)
-            print(json.dumps({"rate_req_per_s": rate, **serve_stats(res, tr)}), flush=True)
+            print(json.dumps({"rate_req_per_s": rate, "slots": a.serve_slots, **serve_stats(res, tr)}), flush=True)
         srv.close()
         return
     if a.uneven_only:
@@ -626,12 +630,12 @@ def main():
     line["uneven"] = uneven_batch(Q, SOLVE)
     progress("uneven batch done")
     # serving: four resident fields of different volume, continuous batching
-    srv = make_server(Q, slots=8)
-    srv.run(serve_trace(1000.0, 8, seed=1))  # warm-up (fills the slot pool)
+    srv = make_server(Q, slots=4)
+    srv.run(serve_trace(1000.0, 16))  # warm-up: the same requests at once (fills the slot pool)
     tr = serve_trace(20.0, 16)
     res = srv.run(tr)
     line["served"] = {"workload": "Poisson 20 req/s, 16 requests, volumes 8^4/16^3x32/24^3x48/32^3x64, "
-                                  "kappa U[0.125,0.137], 1-12 RHS, up to 8 requests in the running batch",
+                                  "kappa U[0.125,0.137], 1-12 RHS, up to 4 requests in the running batch",
                       **serve_stats(res, tr)}
     line["served"]["c2_equiv_vs_plain"] = line["served"]["c2_equiv_solves_per_s"] / line["value"]
     srv.close()

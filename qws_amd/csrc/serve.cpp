@@ -24,6 +24,7 @@ struct Server::Slot {
   std::unique_ptr<OuterBatch> ob;
   unsigned long long* dig = nullptr;  // device word for the solutions' digest
   bool busy = false;
+  long long used = 0;  // when the slot last took a request (eviction order)
   // the request being served
   int req = -1, nrhs = 0;
   std::vector<SolveStats> st;
@@ -88,18 +89,27 @@ std::vector<RequestResult> Server::run(const std::vector<Request>& trace) {
   const auto t0 = clock::now();
   auto since = [&] { return std::chrono::duration<double>(clock::now() - t0).count(); };
 
-  // a free pooled slot for (field, capacity), a new one if it fits the budget (giving
-  // back idle slots of other shapes if needed), or nullptr for now
+  // a free pooled slot on the field with room for the request (the smallest such), a
+  // new one if it fits the budget (giving back idle slots, least recently used first),
+  // or nullptr for now
+  long long uses = 0;
   auto acquire = [&](int field, int cap, double bytes) -> Slot* {
+    Slot* best = nullptr;
     for (auto& s : pool_)
-      if (!s->busy && s->field == field && s->cap == cap) return s.get();
-    auto used = [&] {
+      if (!s->busy && s->field == field && s->cap >= cap && (!best || s->cap < best->cap)) best = s.get();
+    if (best) {
+      best->used = ++uses;
+      return best;
+    }
+    auto in_use = [&] {
       double u = 0;
       for (auto& s : pool_) u += s->bytes;
       return u;
     };
-    while (used() + bytes > cfg_.mem_budget) {
-      auto it = std::find_if(pool_.begin(), pool_.end(), [](const std::unique_ptr<Slot>& s) { return !s->busy; });
+    while (in_use() + bytes > cfg_.mem_budget) {
+      auto it = pool_.end();
+      for (auto j = pool_.begin(); j != pool_.end(); ++j)
+        if (!(*j)->busy && (it == pool_.end() || (*j)->used < (*it)->used)) it = j;
       if (it == pool_.end()) return nullptr;
       QWS_HIP(hipStreamSynchronize(st_));
       pool_.erase(it);
@@ -113,6 +123,7 @@ std::vector<RequestResult> Server::run(const std::vector<Request>& trace) {
     s->lat->set_stream(st_);
     s->ob.reset(new OuterBatch(*s->lat, cap, cfg_.prm));
     QWS_HIP(hipMalloc((void**)&s->dig, sizeof(unsigned long long)));
+    s->used = ++uses;
     pool_.push_back(std::move(s));
     return pool_.back().get();
   };

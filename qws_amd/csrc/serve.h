@@ -55,7 +55,8 @@ class Server {
   ~Server();
   // Replays the trace: each request is released at its arrival time and admitted in
   // arrival order when fewer than `slots` requests run and a pooled slot for it is free
-  // or fits the budget (idle slots of other shapes are given back to make room).
+  // (on its field, with room for its right-hand sides: the smallest such) or a new one
+  // fits the budget (idle slots are given back to make room, least recently used first).
   std::vector<RequestResult> run(const std::vector<Request>& trace);
   // device memory of a slot for nrhs right-hand sides on V sites: clover inverse (FP64 +
   // FP32), the outer solver's 8 FP64 vectors, the inner solver's, preconditioner's and
