@@ -344,6 +344,9 @@ def uneven_batch(Q, params, fused_only=0):
            / t_fused,
            "true_residual_max": max(s[0]["true_residual"] for s in st),
            "same_iterations_as_separate": all(a[0]["inner_iters"] == b[0]["inner_iters"] for a, b in zip(st, st_sep))}
+    # the fused solutions against all-FP64 solves of the same sources (outside the timing)
+    out["error_vs_fp64"] = {"x".join(map(str, lat.L)): error_vs_fp64(Q, lat, xs[0], bs[0])["error"]
+                            for lat, xs, bs in parts}
     for lat in lats:
         lat.close()
     return out
