@@ -229,6 +229,10 @@ void check_residual_s_(const scs_t* x, const scs_t* b, const int* iter, const fl
  * x-halves (extents must divide nx/2, ny, nz, nt).  For callers that write
  * glud/glus/clvd/clvs themselves or want a GPU-sized SAP block. */
 void qws_mi355x_fields_modified_(void);
+/* No counterpart in the reference: upload glud/glus/clvd/clvs to the GPU now (the
+ * explicit form of qws_mi355x_fields_modified_(); exits like QWS's fatal errors when
+ * qws_init_ has not been called). */
+void qws_gpu_sync_fields_(void);
 void qws_mi355x_set_sap_blocks_(int* block_extents);
 /* No counterpart as a call in the reference (QWS chooses it at compile time: its
  * default SU3_RECONSTRUCT_S = 18 at qws.h:61, the 12-real layout at qws.h:164-166 is

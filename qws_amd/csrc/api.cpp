@@ -700,6 +700,14 @@ void qws_mi355x_fields_modified_(void) {
   mark_fields_modified();
 }
 
+void qws_gpu_sync_fields_(void) {
+  drop_in(__func__, [&] {
+    mark_fields_modified();
+    lat_sync("qws_gpu_sync_fields_");
+    QWS_HIP(hipStreamSynchronize(g_qws.lat->stream()));
+  });
+}
+
 void qws_mi355x_set_sap_blocks_(int* b) {
   if (!g_qws.lat) fatal(__func__, "qws_init_ has not been called");
   int B[4] = {b[0], b[1], b[2], b[3]};

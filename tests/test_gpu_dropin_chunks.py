@@ -72,3 +72,53 @@ def test_multi_chunk_roundtrip_and_operator(monkeypatch):
         assert np.linalg.norm(got - want) / np.linalg.norm(want) < 1e-13
     finally:
         L.qws_finalize_()
+
+
+def test_gpu_sync_fields_uploads_the_host_arrays():
+    """Fields written through QWS's global pointers after a first operator call reach
+    the GPU only when the caller says so: without a call the operator keeps the old
+    fields; after qws_gpu_sync_fields_() ddd_d_ matches the oracle on the new ones."""
+    L = Q.lib()
+    nloc = [32, 4, 4, 4]  # QWS's FP32 layout needs nx/2 to be a multiple of 16
+    V = int(np.prod(nloc))
+    rng = np.random.default_rng(33)
+    U1, C1 = K.random_su3_like(V, rng)
+    U2, C2 = K.random_su3_like(V, rng)
+    b = rng.standard_normal((V, 3, 4, 2))
+    ci = lambda v: ctypes.byref(ctypes.c_int(v))
+    keep = [(ctypes.c_int * 4)(1, 1, 1, 1), (ctypes.c_int * 4)(1, 1, 1, -1), ctypes.c_int(0), ctypes.c_int(1),
+            (ctypes.c_int * 4)(1, 2, 2, 2)]
+    L.qws_init_(*(ci(n) for n in nloc), keep[0], keep[1], ctypes.byref(keep[2]), ctypes.byref(keep[3]), keep[4])
+    try:
+        vold = ctypes.c_int.in_dll(L, "vold").value
+
+        def glob(name, ct, n):
+            return np.ctypeslib.as_array((ct * n).from_address(ctypes.c_void_p.in_dll(L, name).value))
+
+        def write(U, C):
+            glob("glud", ctypes.c_double, vold * 8 * 144)[:] = RW.links_to_aosoa(U, nloc, 8).ravel()
+            glob("clvd", ctypes.c_double, vold * 2 * 576)[:] = RW.clover_to_aosoa(C, nloc, 8).ravel()
+
+        kap = 0.11
+        ctypes.c_double.in_dll(L, "kappa").value = kap
+        ctypes.c_double.in_dll(L, "mkappa").value = -kap
+        ctypes.c_double.in_dll(L, "kappa2").value = kap * kap
+        blocks = [nloc[0] // 2] + nloc[1:]
+        bd = RW.spinor_to_aosoa(b, nloc, 8)
+        out = np.zeros_like(bd)
+        write(U1, C1)
+        L.qws_gpu_sync_fields_()
+        L.ddd_d_(_ptr(out), _ptr(bd))
+        want1 = O.Oracle(nloc, blocks, -1, kap, U1, C1).dslash(b)
+        got = RW.spinor_from_aosoa(out, nloc, 8)
+        assert np.linalg.norm(got - want1) / np.linalg.norm(want1) < 1e-13
+        write(U2, C2)  # not yet announced: the GPU still holds U1, C1
+        L.ddd_d_(_ptr(out), _ptr(bd))
+        assert np.array_equal(RW.spinor_from_aosoa(out, nloc, 8), got)
+        L.qws_gpu_sync_fields_()
+        L.ddd_d_(_ptr(out), _ptr(bd))
+        want2 = O.Oracle(nloc, blocks, -1, kap, U2, C2).dslash(b)
+        got2 = RW.spinor_from_aosoa(out, nloc, 8)
+        assert np.linalg.norm(got2 - want2) / np.linalg.norm(want2) < 1e-13
+    finally:
+        L.qws_finalize_()
