@@ -132,22 +132,24 @@ PRECISION_WORKLOAD = dict(L=[48, 48, 48, 96], B=[4, 4, 4, 4], eps=0.25, seed=3, 
                           links32=12)
 
 
-def precision_variant(Q, kappa=None, budget_s=240.0):
-    """48^3x96: FP16-stored SAP (FP32 arithmetic), FP32 inner, FP64 outer to a 1e-10 FP64
-    true residual in one solve, the outer iteration restarted every 4 iterations (near
-    the critical kappa the FP16-stored SAP can make an inner solve diverge; the solve
-    then goes on with the FP32 SAP: sap_escalated).  Its
-    error against an FP64-refined continuation: the FP32-SAP mixed solve started from
-    the result and driven to a 1e-13 true residual (the preconditioner's precision
-    bounds only the convergence rate, not the accuracy of the FP64-refined solution).
-    budget_s is unused (the solve runs to completion)."""
+def precision_variant(Q, kappa=None):
+    """48^3x96: FP16-stored SAP (FP32 arithmetic; nsap = 4, i.e. 3 sweeps, nm = 6), FP32
+    inner (at most 40 iterations), FP64 outer to a 1e-10 FP64 true residual in one solve,
+    the outer iteration restarted every 4 iterations (near the critical kappa the
+    FP16-stored SAP can make an inner solve diverge; the solve then goes on with the FP32
+    SAP and its same 3 sweeps: sap_escalated).  Settings from a sweep at kappa = 0.137:
+    maxiter_s 200 / 100 / 50 / 40 / 30 took 204 (nsap 3) / 107 / 76 / 53 / 56 s; the inner
+    solves end at 0.1-0.8 of their source at any of these caps.  Its error against an
+    FP64-refined continuation: the FP32-SAP mixed solve started from the result and driven
+    to a 1e-13 true residual (the preconditioner's precision bounds only the convergence
+    rate, not the accuracy of the FP64-refined solution)."""
     w = dict(PRECISION_WORKLOAD, kappa=kappa if kappa is not None else PRECISION_WORKLOAD["kappa"])
     lat, b = setup(Q, w)
     progress("48^3x96 fields ready")
     x = lat.spinor(Q.F64)
     # outer BiCGStab restarted from x every 4 iterations (maxiter 5), as long as the true
     # residual misses 1e-10; a switch to the FP32 SAP holds for the rest of the solve
-    prm = dict(SOLVE, sap_precision=Q.F16, nsap=3, nm=6, maxiter=5, max_restarts=60, maxiter_s=200)
+    prm = dict(SOLVE, sap_precision=Q.F16, nsap=4, nm=6, maxiter=5, max_restarts=60, maxiter_s=40)
     lat.solve(x, b, **dict(prm, maxiter=2, true_tol=-1.0, max_restarts=0))  # warm-up
     x.zero()
     lat.sync()
@@ -158,7 +160,8 @@ def precision_variant(Q, kappa=None, budget_s=240.0):
              % (st["outer_iters"], st["inner_iters"], st["true_residual"], st["sap_escalated"], sec))
     xr = lat.spinor(Q.F64, x.download())
     t1 = time.perf_counter()
-    ref = lat.solve(xr, b, **dict(SOLVE, tol=1e-14, maxiter=5, max_restarts=60, maxiter_s=200, true_tol=1e-13))
+    ref = lat.solve(xr, b, **dict(SOLVE, nsap=3, nm=6, tol=1e-14, maxiter=5, max_restarts=60, maxiter_s=40,
+                                  true_tol=1e-13))
     ref_sec = time.perf_counter() - t1
     progress("FP64-refined continuation: true residual %.2e, %.0f s" % (ref["true_residual"], ref_sec))
     a, r = x.download(), xr.download()
@@ -467,8 +470,6 @@ def main():
     ap.add_argument("--precision", action="store_true",
                     help="the 48^3x96 FP16-SAP precision variant and its error against an FP64-refined solution")
     ap.add_argument("--precision-kappa", type=float, default=None, help="kappa of the precision variant")
-    ap.add_argument("--precision-budget", type=float, default=240.0,
-                    help="seconds for each of the precision variant's two solves")
     ap.add_argument("--copy-calib", type=int, default=0,
                     help="only run N device-to-device copies of 1 GiB (HBM counter calibration)")
     a = ap.parse_args()
@@ -492,7 +493,7 @@ def main():
     import qws_amd as Q
 
     if a.precision:
-        print(json.dumps(precision_variant(Q, a.precision_kappa, a.precision_budget)), flush=True)
+        print(json.dumps(precision_variant(Q, a.precision_kappa)), flush=True)
         return
     if a.serve:
         srv = make_server(Q, slots=a.serve_slots)
@@ -644,11 +645,11 @@ def main():
     line["served"]["c2_equiv_vs_plain"] = line["served"]["c2_equiv_solves_per_s"] / line["value"]
     srv.close()
     progress("served trace done")
-    # the precision variant's lattice (48^3x96, eps 0.25, csw 1.1) with the FP16-stored SAP at
-    # kappa = 0.13; kappa = 0.137 (near critical, minutes) runs with --precision
-    line["precision"] = precision_variant(Q, 0.13)
-    line["precision"]["note"] = ("kappa = 0.137 of the precision configuration: bench.py --precision "
-                                 "(profiles/precision_48c96.md)")
+    # the precision variant (48^3x96, eps 0.25, csw 1.1, FP16-stored SAP) at its kappa = 0.137
+    # (near critical: about a minute, and the FP64-refined continuation) and at kappa = 0.13
+    line["precision"] = precision_variant(Q)
+    progress("precision variant (kappa 0.137) done")
+    line["precision_kappa_0.13"] = precision_variant(Q, 0.13)
     progress("precision variant (kappa 0.13) done")
     if not a.no_cpu_baseline:
         progress("timing the CPU reference")
