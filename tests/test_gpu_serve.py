@@ -62,3 +62,31 @@ def test_memory_admission():
     iv = sorted((m["admit"], m["finish"]) for m in res)
     assert all(iv[k + 1][0] >= iv[k][1] - 1e-6 for k in range(len(iv) - 1))
     assert all(m["true_residual_max"] <= 1e-10 for m in res)
+
+
+def test_request_in_a_larger_pooled_slot():
+    """A request served from a free pooled slot of larger RHS capacity (the smallest
+    free one on its field with room) gives the same result as from a slot of its own
+    size, and takes no new memory: with a budget for one 8-RHS slot, a 2-RHS request
+    after an 8-RHS one on the same field reuses that slot."""
+    f = FIELDS[2]
+    V = int(np.prod(f[0]))
+    small = dict(field=2, kappa=0.12, nrhs=2, seed=501, arrival=0.0)
+    big = dict(field=2, kappa=0.118, nrhs=8, seed=502, arrival=0.0)
+    s1 = Q.Server(FIELDS, links32=12, slots=4, **PRM)
+    try:
+        own = s1.run([small])[0]  # a fresh pool: a 2-RHS slot
+    finally:
+        s1.close()
+    budget = Q.Server.request_bytes(V, 8) + 1.0
+    s2 = Q.Server(FIELDS, links32=12, slots=4, mem_budget=budget, **PRM)
+    try:
+        first = s2.run([big])[0]
+        again = s2.run([small])[0]
+    finally:
+        s2.close()
+    assert first["true_residual_max"] <= 1e-10
+    assert again["bytes"] == first["bytes"] > own["bytes"]  # served by the 8-RHS slot
+    assert again["digest"] == own["digest"]
+    assert again["inner_iters"] == own["inner_iters"] and again["outer_iters"] == own["outer_iters"]
+    assert again["true_residual_max"] <= 1e-10
