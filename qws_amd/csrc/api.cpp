@@ -63,106 +63,106 @@ int qwsg_destroy(qwsg_lattice* h) {
   });
 }
 int qwsg_set_sap_fused(qwsg_lattice* h, int on) {
-  return guarded([&] { h->lat->set_fused_sap(on != 0); });
+  return guarded([&] { qws::lat_of(h)->set_fused_sap(on != 0); });
 }
 int qwsg_set_sap_face_buffer(qwsg_lattice* h, int on) {
-  return guarded([&] { h->lat->set_sap_face_buffer(on != 0); });
+  return guarded([&] { qws::lat_of(h)->set_sap_face_buffer(on != 0); });
 }
 int qwsg_set_sap_source_update(qwsg_lattice* h, int on) {
-  return guarded([&] { h->lat->set_sap_source_update(on != 0); });
+  return guarded([&] { qws::lat_of(h)->set_sap_source_update(on != 0); });
 }
 int qwsg_set_kappa(qwsg_lattice* h, double k) {
-  return guarded([&] { h->lat->set_kappa(k); });
+  return guarded([&] { qws::lat_of(h)->set_kappa(k); });
 }
 int qwsg_volume(qwsg_lattice* h, long long* V) {
-  return guarded([&] { *V = h->lat->volume(); });
+  return guarded([&] { *V = qws::lat_of(h)->volume(); });
 }
 int qwsg_load_gauge(qwsg_lattice* h, const double* U) {
-  return guarded([&] { h->lat->load_gauge(U, 0, 1, 3); });
+  return guarded([&] { qws::lat_of(h)->load_gauge(U, 0, 1, 3); });
 }
 int qwsg_load_clover(qwsg_lattice* h, const double* C) {
-  return guarded([&] { h->lat->load_clover(C, 0, 1, 3); });
+  return guarded([&] { qws::lat_of(h)->load_clover(C, 0, 1, 3); });
 }
 int qwsg_load_gauge_f32(qwsg_lattice* h, const float* U) {
-  return guarded([&] { h->lat->load_gauge(U, 0, 0, 2); });
+  return guarded([&] { qws::lat_of(h)->load_gauge(U, 0, 0, 2); });
 }
 int qwsg_load_clover_f32(qwsg_lattice* h, const float* C) {
-  return guarded([&] { h->lat->load_clover(C, 0, 0, 2); });
+  return guarded([&] { qws::lat_of(h)->load_clover(C, 0, 0, 2); });
 }
 int qwsg_generate_gauge(qwsg_lattice* h, double eps, unsigned long long seed) {
-  return guarded([&] { h->lat->generate_gauge(eps, seed); });
+  return guarded([&] { qws::lat_of(h)->generate_gauge(eps, seed); });
 }
 int qwsg_compute_clover(qwsg_lattice* h, double csw) {
-  return guarded([&] { h->lat->compute_clover(csw); });
+  return guarded([&] { qws::lat_of(h)->compute_clover(csw); });
 }
 int qwsg_gauge_download(qwsg_lattice* h, double* U) {
-  return guarded([&] { h->lat->export_field(1, U); });
+  return guarded([&] { qws::lat_of(h)->export_field(1, U); });
 }
 int qwsg_clover_download(qwsg_lattice* h, double* C) {
-  return guarded([&] { h->lat->export_field(2, C); });
+  return guarded([&] { qws::lat_of(h)->export_field(2, C); });
 }
 int qwsg_spinor_gaussian(qwsg_lattice* h, int prec, void* dev, unsigned long long seed) {
-  return guarded([&] { h->lat->gaussian_spinor(prec, dev, seed); });
+  return guarded([&] { qws::lat_of(h)->gaussian_spinor(prec, dev, seed); });
 }
 int qwsg_spinor_point(qwsg_lattice* h, int prec, void* dev, const int* site, int spin, int colour) {
-  return guarded([&] { h->lat->point_spinor(prec, dev, site, spin, colour); });
+  return guarded([&] { qws::lat_of(h)->point_spinor(prec, dev, site, spin, colour); });
 }
 int qwsg_spinor_alloc(qwsg_lattice* h, int prec, void** dev) {
-  return guarded([&] { *dev = h->lat->alloc_spinor(prec); });
+  return guarded([&] { *dev = qws::lat_of(h)->alloc_spinor(prec); });
 }
 int qwsg_spinor_free(void* dev) {
   return guarded([&] { QWS_HIP(hipFree(dev)); });
 }
 int qwsg_spinor_upload(qwsg_lattice* h, int prec, void* dev, const double* host) {
-  return guarded([&] { h->lat->upload_spinor(prec, dev, host, 0, 1); });
+  return guarded([&] { qws::lat_of(h)->upload_spinor(prec, dev, host, 0, 1); });
 }
 int qwsg_spinor_download(qwsg_lattice* h, int prec, const void* dev, double* host) {
-  return guarded([&] { h->lat->download_spinor(prec, dev, host, 0, 1); });
+  return guarded([&] { qws::lat_of(h)->download_spinor(prec, dev, host, 0, 1); });
 }
 int qwsg_spinor_zero(qwsg_lattice* h, int prec, void* dev) {
   return guarded([&] {
-    QWS_HIP(hipMemsetAsync(dev, 0, h->lat->spinor_bytes(prec), h->lat->stream()));
+    QWS_HIP(hipMemsetAsync(dev, 0, qws::lat_of(h)->spinor_bytes(prec), qws::lat_of(h)->stream()));
   });
 }
 int qwsg_dslash(qwsg_lattice* h, int prec, void* out, const void* in, int hops, int colour) {
-  return guarded([&] { h->lat->dslash(prec, out, in, hops, colour); });
+  return guarded([&] { qws::lat_of(h)->dslash(prec, out, in, hops, colour); });
 }
 int qwsg_face(qwsg_lattice* h, int prec, void* out, const void* in, double factor, int colour) {
-  return guarded([&] { h->lat->face(prec, out, in, factor, colour); });
+  return guarded([&] { qws::lat_of(h)->face(prec, out, in, factor, colour); });
 }
 int qwsg_clover(qwsg_lattice* h, int prec, void* out, const void* in) {
-  return guarded([&] { h->lat->clover(prec, out, in); });
+  return guarded([&] { qws::lat_of(h)->clover(prec, out, in); });
 }
 int qwsg_jinv(qwsg_lattice* h, int prec, void* x, const void* b, int colour, int nm) {
-  return guarded([&] { h->lat->jinv(prec, x, b, colour, nm); });
+  return guarded([&] { qws::lat_of(h)->jinv(prec, x, b, colour, nm); });
 }
 int qwsg_prec_ddd(qwsg_lattice* h, int prec, void* out, const void* in, int nsap, int nm) {
-  return guarded([&] { h->lat->prec_ddd(prec, out, in, nsap, nm); });
+  return guarded([&] { qws::lat_of(h)->prec_ddd(prec, out, in, nsap, nm); });
 }
 int qwsg_prec_s(qwsg_lattice* h, int prec, void* out, const void* in, int nsap, int nm) {
-  return guarded([&] { h->lat->prec_s(prec, out, in, nsap, nm); });
+  return guarded([&] { qws::lat_of(h)->prec_s(prec, out, in, nsap, nm); });
 }
 int qwsg_bicgstab_precdd(qwsg_lattice* h, int prec, void* x, const void* b, double tol, int maxiter, int nsap, int nm,
                          int* iters) {
   return guarded([&] {
-    int it = h->lat->bicgstab_precdd(prec, x, b, tol, maxiter, nsap, nm);
+    int it = qws::lat_of(h)->bicgstab_precdd(prec, x, b, tol, maxiter, nsap, nm);
     if (iters) *iters = it;
   });
 }
 int qwsg_msap_h(qwsg_lattice* h, void* out, const void* in, int nsap, int nm) {
-  return guarded([&] { h->lat->msap_h((float*)out, (const float*)in, nsap, nm); });
+  return guarded([&] { qws::lat_of(h)->msap_h((float*)out, (const float*)in, nsap, nm); });
 }
 int qwsg_bicgstab_precdd_h(qwsg_lattice* h, void* x, const void* b, double tol, int maxiter, int nsap, int nm,
                            int* iters) {
   return guarded([&] {
-    int it = h->lat->bicgstab_precdd_h((float*)x, (const float*)b, tol, maxiter, nsap, nm);
+    int it = qws::lat_of(h)->bicgstab_precdd_h((float*)x, (const float*)b, tol, maxiter, nsap, nm);
     if (iters) *iters = it;
   });
 }
 int qwsg_bicgstab_dd(qwsg_lattice* h, int prec, void* x, const void* b, int maxiter, double tol, int* iters) {
   return guarded([&] {
     if (prec != qws::P64 && prec != qws::P32) throw std::invalid_argument("bicgstab_dd: FP64 or FP32 vectors");
-    const int it = h->lat->bicgstab_dd(prec, x, b, maxiter, tol);
+    const int it = qws::lat_of(h)->bicgstab_dd(prec, x, b, maxiter, tol);
     if (iters) *iters = it;
   });
 }
@@ -195,7 +195,7 @@ void put_stats(const qws::SolveStats& s, qwsg_solve_stats* st) {
 int qwsg_solve(qwsg_lattice* h, void* x, const void* b, const qwsg_solve_params* p, qwsg_solve_stats* st) {
   return guarded([&] {
     const qws::SolveParams sp = solve_params(p);
-    qws::SolveStats s = h->lat->solve((double*)x, (const double*)b, sp);
+    qws::SolveStats s = qws::lat_of(h)->solve((double*)x, (const double*)b, sp);
     if (st) put_stats(s, st);
   });
 }
@@ -205,7 +205,7 @@ int qwsg_solve_batch(qwsg_lattice* h, int n, void* const* x, const void* const* 
     if (n < 1 || n > qws::MAX_BATCH) throw std::invalid_argument("batch size must be in 1..64");
     const qws::SolveParams sp = solve_params(p);
     std::vector<qws::SolveStats> s =
-        h->lat->solve_batch((double* const*)x, (const double* const*)b, n, sp);
+        qws::lat_of(h)->solve_batch((double* const*)x, (const double* const*)b, n, sp);
     for (int k = 0; k < n && st; ++k) put_stats(s[k], st + k);
   });
 }
@@ -249,36 +249,36 @@ int qwsg_solve_multi(int nlat, qwsg_lattice* const* lats, const int* nrhs, void*
   });
 }
 int qwsg_eo_load_gauge(qwsg_lattice* h, const double* U) {
-  return guarded([&] { h->lat->eo_load_gauge(U, 0, 1, 1, 3); });
+  return guarded([&] { qws::lat_of(h)->eo_load_gauge(U, 0, 1, 1, 3); });
 }
 int qwsg_eo_load_clover(qwsg_lattice* h, const double* C) {
-  return guarded([&] { h->lat->eo_load_clover(C, 0, 1, 1, 3); });
+  return guarded([&] { qws::lat_of(h)->eo_load_clover(C, 0, 1, 1, 3); });
 }
 int qwsg_eo_from_lattice(qwsg_lattice* h) {
-  return guarded([&] { h->lat->eo_fields_from_lattice(); });
+  return guarded([&] { qws::lat_of(h)->eo_fields_from_lattice(); });
 }
 int qwsg_eo_spinor_alloc(qwsg_lattice* h, int prec, void** dev) {
   return guarded([&] {
     if (prec != qws::P64 && prec != qws::P32) throw std::invalid_argument("even/odd spinors are FP64 or FP32");
     *dev = nullptr;
-    QWS_HIP(hipMalloc(dev, h->lat->eo_spinor_bytes(prec)));
-    QWS_HIP(hipMemset(*dev, 0, h->lat->eo_spinor_bytes(prec)));
+    QWS_HIP(hipMalloc(dev, qws::lat_of(h)->eo_spinor_bytes(prec)));
+    QWS_HIP(hipMemset(*dev, 0, qws::lat_of(h)->eo_spinor_bytes(prec)));
   });
 }
 int qwsg_eo_spinor_upload(qwsg_lattice* h, int prec, void* dev, const double* host) {
-  return guarded([&] { h->lat->eo_upload_spinor(prec, dev, host, 0, 1, 1); });
+  return guarded([&] { qws::lat_of(h)->eo_upload_spinor(prec, dev, host, 0, 1, 1); });
 }
 int qwsg_eo_spinor_download(qwsg_lattice* h, int prec, const void* dev, double* host) {
-  return guarded([&] { h->lat->eo_download_spinor(prec, dev, host, 0, 1, 1); });
+  return guarded([&] { qws::lat_of(h)->eo_download_spinor(prec, dev, host, 0, 1, 1); });
 }
 int qwsg_deo(qwsg_lattice* h, int prec, int pe, int set, int dag, int clv, int mode, double fac, void* out,
              const void* in, const void* in0) {
-  return guarded([&] { h->lat->deo(prec, pe, set, dag, clv, mode, fac, out, in, in0); });
+  return guarded([&] { qws::lat_of(h)->deo(prec, pe, set, dag, clv, mode, fac, out, in, in0); });
 }
 int qwsg_mtilde(qwsg_lattice* h, int prec, int pce, int which, void* out, const void* in) {
   return guarded([&] {
-    const double k = h->lat->kappa();
-    h->lat->mtilde(prec, pce, which, -k * k, out, in);
+    const double k = qws::lat_of(h)->kappa();
+    qws::lat_of(h)->mtilde(prec, pce, which, -k * k, out, in);
   });
 }
 struct qwsg_server {
@@ -318,6 +318,8 @@ int qwsg_server_create(const qwsg_field_spec* fields, int nfields, int links32, 
 }
 int qwsg_server_run(qwsg_server* h, const qwsg_request* reqs, int nreq, qwsg_request_result* out) {
   return guarded([&] {
+    if (!h || !h->srv) throw std::invalid_argument("null server handle");
+    if (nreq < 0 || (nreq > 0 && (!reqs || !out))) throw std::invalid_argument("server_run: bad request array");
     std::vector<qws::Request> tr(nreq);
     for (int i = 0; i < nreq; ++i) tr[i] = {reqs[i].field, reqs[i].kappa, reqs[i].nrhs, reqs[i].seed, reqs[i].arrival};
     std::vector<qws::RequestResult> r = h->srv->run(tr);
@@ -350,40 +352,40 @@ int qwsg_server_destroy(qwsg_server* h) {
 }
 int qwsg_eo_bicgstab(qwsg_lattice* h, void* x, const void* b, int maxiter, double tol, int pce, int* iters) {
   return guarded([&] {
-    const double k = h->lat->kappa();
-    const int it = h->lat->bicgstab_eo((double*)x, (const double*)b, maxiter, tol, pce, -k * k);
+    const double k = qws::lat_of(h)->kappa();
+    const int it = qws::lat_of(h)->bicgstab_eo((double*)x, (const double*)b, maxiter, tol, pce, -k * k);
     if (iters) *iters = it;
   });
 }
 int qwsg_eo_cg(qwsg_lattice* h, void* x, const void* b, int maxiter, double tol, int pce, int* iters) {
   return guarded([&] {
-    const double k = h->lat->kappa();
-    const int it = h->lat->cg_eo((double*)x, (const double*)b, maxiter, tol, pce, -k * k);
+    const double k = qws::lat_of(h)->kappa();
+    const int it = qws::lat_of(h)->cg_eo((double*)x, (const double*)b, maxiter, tol, pce, -k * k);
     if (iters) *iters = it;
   });
 }
 int qwsg_eo_mcg(qwsg_lattice* h, void* const* xs, const void* b, int nshift, const double* sigma, int maxiter,
                 double tol, int pce, int* iters) {
   return guarded([&] {
-    const double k = h->lat->kappa();
-    const int it = h->lat->mcg_eo((double* const*)xs, (const double*)b, nshift, sigma, maxiter, tol, pce, -k * k);
+    const double k = qws::lat_of(h)->kappa();
+    const int it = qws::lat_of(h)->mcg_eo((double* const*)xs, (const double*)b, nshift, sigma, maxiter, tol, pce, -k * k);
     if (iters) *iters = it;
   });
 }
 int qwsg_norm2(qwsg_lattice* h, int prec, const void* a, double* out) {
-  return guarded([&] { *out = h->lat->norm2(prec, a); });
+  return guarded([&] { *out = qws::lat_of(h)->norm2(prec, a); });
 }
 int qwsg_spinor_digest(qwsg_lattice* h, int prec, const void* a, unsigned long long* out) {
-  return guarded([&] { *out = h->lat->digest(prec, a); });
+  return guarded([&] { *out = qws::lat_of(h)->digest(prec, a); });
 }
 int qwsg_synchronize(qwsg_lattice* h) {
-  return guarded([&] { h->lat->sync(); });
+  return guarded([&] { qws::lat_of(h)->sync(); });
 }
 int qwsg_time_dslash(qwsg_lattice* h, int prec, int reps, double* ms) {
-  return guarded([&] { *ms = h->lat->time_dslash(prec, reps); });
+  return guarded([&] { *ms = qws::lat_of(h)->time_dslash(prec, reps); });
 }
 int qwsg_set_dslash_kernel(qwsg_lattice* h, int k) {
-  return guarded([&] { h->lat->set_dslash_kernel(k); });
+  return guarded([&] { qws::lat_of(h)->set_dslash_kernel(k); });
 }
 int qwsg_autotune_solve(qwsg_lattice* h, void* x, const void* b, const qwsg_solve_params* p, const int* nsap,
                         int nnsap, const int* nm, int nnm, qwsg_solve_params* best, double* seconds) {
@@ -391,7 +393,7 @@ int qwsg_autotune_solve(qwsg_lattice* h, void* x, const void* b, const qwsg_solv
     if (!p || !best || !nsap || !nm || nnsap < 1 || nnm < 1) throw std::invalid_argument("autotune_solve: bad arguments");
     const qws::SolveParams sp = solve_params(p);
     std::vector<double> sec;
-    const qws::SolveParams b2 = h->lat->autotune_solve((double*)x, (const double*)b, sp,
+    const qws::SolveParams b2 = qws::lat_of(h)->autotune_solve((double*)x, (const double*)b, sp,
                                                        std::vector<int>(nsap, nsap + nnsap),
                                                        std::vector<int>(nm, nm + nnm), sec);
     *best = *p;
@@ -401,19 +403,19 @@ int qwsg_autotune_solve(qwsg_lattice* h, void* x, const void* b, const qwsg_solv
   });
 }
 int qwsg_set_dslash_order(qwsg_lattice* h, int order) {
-  return guarded([&] { h->lat->set_dslash_order(order); });
+  return guarded([&] { qws::lat_of(h)->set_dslash_order(order); });
 }
 int qwsg_get_dslash_order(qwsg_lattice* h, int* order) {
-  return guarded([&] { *order = h->lat->dslash_order(); });
+  return guarded([&] { *order = qws::lat_of(h)->dslash_order(); });
 }
 int qwsg_autotune_dslash(qwsg_lattice* h, int reps, int* chosen) {
   return guarded([&] {
-    const int k = h->lat->autotune_dslash(reps);
+    const int k = qws::lat_of(h)->autotune_dslash(reps);
     if (chosen) *chosen = k;
   });
 }
 int qwsg_time_sap(qwsg_lattice* h, int prec, int reps, int nsap, int nm, double* ms) {
-  return guarded([&] { *ms = h->lat->time_sap(prec, reps, nsap, nm); });
+  return guarded([&] { *ms = qws::lat_of(h)->time_sap(prec, reps, nsap, nm); });
 }
 
 // ============================================================================ QWS drop-in

@@ -16,6 +16,12 @@ namespace qws {
 // the calling thread's last error message (qwsg_last_error)
 std::string& last_error();
 
+// the lattice of a handle (invalid_argument for a null handle)
+inline Lattice* lat_of(qwsg_lattice* h) {
+  if (!h || !h->lat) throw std::invalid_argument("null lattice handle");
+  return h->lat;
+}
+
 // 0 on success; 1 invalid argument, 2 HIP runtime error, 3 any other failure
 template <typename F> int guarded(F&& f) {
   try {

@@ -19,6 +19,24 @@ def test_library_exports_every_declared_function():
     assert not missing, missing
 
 
+def test_null_handles_are_refused():
+    """Entry points given a null lattice or server handle return 1 (invalid argument)
+    with a message, before any device work."""
+    L = Q.lib()
+    L.qwsg_set_kappa.argtypes = [ctypes.c_void_p, ctypes.c_double]
+    L.qwsg_volume.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_longlong)]
+    L.qwsg_set_sap_fused.argtypes = [ctypes.c_void_p, ctypes.c_int]
+    L.qwsg_server_run.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]
+    L.qwsg_last_error.restype = ctypes.c_char_p
+    v = ctypes.c_longlong(-1)
+    for rc in (L.qwsg_set_kappa(None, 0.1), L.qwsg_volume(None, ctypes.byref(v)), L.qwsg_set_sap_fused(None, 1)):
+        assert rc == 1
+        assert b"null lattice handle" in L.qwsg_last_error()
+    assert v.value == -1
+    assert L.qwsg_server_run(None, None, 0, None) == 1
+    assert b"null server handle" in L.qwsg_last_error()
+
+
 def test_library_exports_qws_data_symbols():
     L = Q.lib()
     txt = open(Q.HEADER).read()
