@@ -102,65 +102,77 @@ int qwsg_clover_download(qwsg_lattice* h, double* C) {
   return guarded([&] { qws::lat_of(h)->export_field(2, C); });
 }
 int qwsg_spinor_gaussian(qwsg_lattice* h, int prec, void* dev, unsigned long long seed) {
-  return guarded([&] { qws::lat_of(h)->gaussian_spinor(prec, dev, seed); });
+  return guarded([&] { qws::need("qwsg_spinor_gaussian", dev); qws::lat_of(h)->gaussian_spinor(prec, dev, seed); });
 }
 int qwsg_spinor_point(qwsg_lattice* h, int prec, void* dev, const int* site, int spin, int colour) {
-  return guarded([&] { qws::lat_of(h)->point_spinor(prec, dev, site, spin, colour); });
+  return guarded([&] {
+    qws::need("qwsg_spinor_point", dev);
+    qws::lat_of(h)->point_spinor(prec, dev, site, spin, colour);
+  });
 }
 int qwsg_spinor_alloc(qwsg_lattice* h, int prec, void** dev) {
-  return guarded([&] { *dev = qws::lat_of(h)->alloc_spinor(prec); });
+  return guarded([&] { qws::need("qwsg_spinor_alloc", dev); *dev = qws::lat_of(h)->alloc_spinor(prec); });
 }
 int qwsg_spinor_free(void* dev) {
   return guarded([&] { QWS_HIP(hipFree(dev)); });
 }
 int qwsg_spinor_upload(qwsg_lattice* h, int prec, void* dev, const double* host) {
-  return guarded([&] { qws::lat_of(h)->upload_spinor(prec, dev, host, 0, 1); });
+  return guarded([&] {
+    qws::need("qwsg_spinor_upload", dev, host);
+    qws::lat_of(h)->upload_spinor(prec, dev, host, 0, 1);
+  });
 }
 int qwsg_spinor_download(qwsg_lattice* h, int prec, const void* dev, double* host) {
-  return guarded([&] { qws::lat_of(h)->download_spinor(prec, dev, host, 0, 1); });
+  return guarded([&] {
+    qws::need("qwsg_spinor_download", dev, host);
+    qws::lat_of(h)->download_spinor(prec, dev, host, 0, 1);
+  });
 }
 int qwsg_spinor_zero(qwsg_lattice* h, int prec, void* dev) {
-  return guarded([&] {
+  return guarded([&] { qws::need("qwsg_spinor_zero", dev);
     QWS_HIP(hipMemsetAsync(dev, 0, qws::lat_of(h)->spinor_bytes(prec), qws::lat_of(h)->stream()));
   });
 }
 int qwsg_dslash(qwsg_lattice* h, int prec, void* out, const void* in, int hops, int colour) {
-  return guarded([&] { qws::lat_of(h)->dslash(prec, out, in, hops, colour); });
+  return guarded([&] { qws::need("qwsg_dslash", out, in); qws::lat_of(h)->dslash(prec, out, in, hops, colour); });
 }
 int qwsg_face(qwsg_lattice* h, int prec, void* out, const void* in, double factor, int colour) {
-  return guarded([&] { qws::lat_of(h)->face(prec, out, in, factor, colour); });
+  return guarded([&] { qws::need("qwsg_face", out, in); qws::lat_of(h)->face(prec, out, in, factor, colour); });
 }
 int qwsg_clover(qwsg_lattice* h, int prec, void* out, const void* in) {
-  return guarded([&] { qws::lat_of(h)->clover(prec, out, in); });
+  return guarded([&] { qws::need("qwsg_clover", out, in); qws::lat_of(h)->clover(prec, out, in); });
 }
 int qwsg_jinv(qwsg_lattice* h, int prec, void* x, const void* b, int colour, int nm) {
-  return guarded([&] { qws::lat_of(h)->jinv(prec, x, b, colour, nm); });
+  return guarded([&] { qws::need("qwsg_jinv", x, b); qws::lat_of(h)->jinv(prec, x, b, colour, nm); });
 }
 int qwsg_prec_ddd(qwsg_lattice* h, int prec, void* out, const void* in, int nsap, int nm) {
-  return guarded([&] { qws::lat_of(h)->prec_ddd(prec, out, in, nsap, nm); });
+  return guarded([&] { qws::need("qwsg_prec_ddd", out, in); qws::lat_of(h)->prec_ddd(prec, out, in, nsap, nm); });
 }
 int qwsg_prec_s(qwsg_lattice* h, int prec, void* out, const void* in, int nsap, int nm) {
-  return guarded([&] { qws::lat_of(h)->prec_s(prec, out, in, nsap, nm); });
+  return guarded([&] { qws::need("qwsg_prec_s", out, in); qws::lat_of(h)->prec_s(prec, out, in, nsap, nm); });
 }
 int qwsg_bicgstab_precdd(qwsg_lattice* h, int prec, void* x, const void* b, double tol, int maxiter, int nsap, int nm,
                          int* iters) {
-  return guarded([&] {
+  return guarded([&] { qws::need("qwsg_bicgstab_precdd", x, b);
     int it = qws::lat_of(h)->bicgstab_precdd(prec, x, b, tol, maxiter, nsap, nm);
     if (iters) *iters = it;
   });
 }
 int qwsg_msap_h(qwsg_lattice* h, void* out, const void* in, int nsap, int nm) {
-  return guarded([&] { qws::lat_of(h)->msap_h((float*)out, (const float*)in, nsap, nm); });
+  return guarded([&] {
+    qws::need("qwsg_msap_h", out, in);
+    qws::lat_of(h)->msap_h((float*)out, (const float*)in, nsap, nm);
+  });
 }
 int qwsg_bicgstab_precdd_h(qwsg_lattice* h, void* x, const void* b, double tol, int maxiter, int nsap, int nm,
                            int* iters) {
-  return guarded([&] {
+  return guarded([&] { qws::need("qwsg_bicgstab_precdd_h", x, b);
     int it = qws::lat_of(h)->bicgstab_precdd_h((float*)x, (const float*)b, tol, maxiter, nsap, nm);
     if (iters) *iters = it;
   });
 }
 int qwsg_bicgstab_dd(qwsg_lattice* h, int prec, void* x, const void* b, int maxiter, double tol, int* iters) {
-  return guarded([&] {
+  return guarded([&] { qws::need("qwsg_bicgstab_dd", x, b);
     if (prec != qws::P64 && prec != qws::P32) throw std::invalid_argument("bicgstab_dd: FP64 or FP32 vectors");
     const int it = qws::lat_of(h)->bicgstab_dd(prec, x, b, maxiter, tol);
     if (iters) *iters = it;
@@ -193,7 +205,7 @@ void put_stats(const qws::SolveStats& s, qwsg_solve_stats* st) {
 }  // namespace
 
 int qwsg_solve(qwsg_lattice* h, void* x, const void* b, const qwsg_solve_params* p, qwsg_solve_stats* st) {
-  return guarded([&] {
+  return guarded([&] { qws::need("qwsg_solve", x, b);
     const qws::SolveParams sp = solve_params(p);
     qws::SolveStats s = qws::lat_of(h)->solve((double*)x, (const double*)b, sp);
     if (st) put_stats(s, st);
@@ -201,7 +213,7 @@ int qwsg_solve(qwsg_lattice* h, void* x, const void* b, const qwsg_solve_params*
 }
 int qwsg_solve_batch(qwsg_lattice* h, int n, void* const* x, const void* const* b, const qwsg_solve_params* p,
                      qwsg_solve_stats* st) {
-  return guarded([&] {
+  return guarded([&] { qws::need("qwsg_solve_batch", x, b);
     if (n < 1 || n > qws::MAX_BATCH) throw std::invalid_argument("batch size must be in 1..64");
     const qws::SolveParams sp = solve_params(p);
     std::vector<qws::SolveStats> s =
@@ -211,7 +223,7 @@ int qwsg_solve_batch(qwsg_lattice* h, int n, void* const* x, const void* const* 
 }
 int qwsg_solve_multi(int nlat, qwsg_lattice* const* lats, const int* nrhs, void* const* x, const void* const* b,
                      const qwsg_solve_params* p, qwsg_solve_stats* st) {
-  return guarded([&] {
+  return guarded([&] { qws::need("qwsg_solve_multi", x, b);
     if (nlat < 1 || nlat > 64) throw std::invalid_argument("solve_multi: 1..64 lattices");
     const qws::SolveParams sp = solve_params(p);
     std::vector<Lattice*> L(nlat);
@@ -258,7 +270,7 @@ int qwsg_eo_from_lattice(qwsg_lattice* h) {
   return guarded([&] { qws::lat_of(h)->eo_fields_from_lattice(); });
 }
 int qwsg_eo_spinor_alloc(qwsg_lattice* h, int prec, void** dev) {
-  return guarded([&] {
+  return guarded([&] { qws::need("qwsg_eo_spinor_alloc", dev);
     if (prec != qws::P64 && prec != qws::P32) throw std::invalid_argument("even/odd spinors are FP64 or FP32");
     *dev = nullptr;
     QWS_HIP(hipMalloc(dev, qws::lat_of(h)->eo_spinor_bytes(prec)));
@@ -266,17 +278,26 @@ int qwsg_eo_spinor_alloc(qwsg_lattice* h, int prec, void** dev) {
   });
 }
 int qwsg_eo_spinor_upload(qwsg_lattice* h, int prec, void* dev, const double* host) {
-  return guarded([&] { qws::lat_of(h)->eo_upload_spinor(prec, dev, host, 0, 1, 1); });
+  return guarded([&] {
+    qws::need("qwsg_eo_spinor_upload", dev, host);
+    qws::lat_of(h)->eo_upload_spinor(prec, dev, host, 0, 1, 1);
+  });
 }
 int qwsg_eo_spinor_download(qwsg_lattice* h, int prec, const void* dev, double* host) {
-  return guarded([&] { qws::lat_of(h)->eo_download_spinor(prec, dev, host, 0, 1, 1); });
+  return guarded([&] {
+    qws::need("qwsg_eo_spinor_download", dev, host);
+    qws::lat_of(h)->eo_download_spinor(prec, dev, host, 0, 1, 1);
+  });
 }
 int qwsg_deo(qwsg_lattice* h, int prec, int pe, int set, int dag, int clv, int mode, double fac, void* out,
              const void* in, const void* in0) {
-  return guarded([&] { qws::lat_of(h)->deo(prec, pe, set, dag, clv, mode, fac, out, in, in0); });
+  return guarded([&] {
+    qws::need("qwsg_deo", out, in);
+    qws::lat_of(h)->deo(prec, pe, set, dag, clv, mode, fac, out, in, in0);
+  });
 }
 int qwsg_mtilde(qwsg_lattice* h, int prec, int pce, int which, void* out, const void* in) {
-  return guarded([&] {
+  return guarded([&] { qws::need("qwsg_mtilde", out, in);
     const double k = qws::lat_of(h)->kappa();
     qws::lat_of(h)->mtilde(prec, pce, which, -k * k, out, in);
   });
@@ -351,14 +372,14 @@ int qwsg_server_destroy(qwsg_server* h) {
   });
 }
 int qwsg_eo_bicgstab(qwsg_lattice* h, void* x, const void* b, int maxiter, double tol, int pce, int* iters) {
-  return guarded([&] {
+  return guarded([&] { qws::need("qwsg_eo_bicgstab", x, b);
     const double k = qws::lat_of(h)->kappa();
     const int it = qws::lat_of(h)->bicgstab_eo((double*)x, (const double*)b, maxiter, tol, pce, -k * k);
     if (iters) *iters = it;
   });
 }
 int qwsg_eo_cg(qwsg_lattice* h, void* x, const void* b, int maxiter, double tol, int pce, int* iters) {
-  return guarded([&] {
+  return guarded([&] { qws::need("qwsg_eo_cg", x, b);
     const double k = qws::lat_of(h)->kappa();
     const int it = qws::lat_of(h)->cg_eo((double*)x, (const double*)b, maxiter, tol, pce, -k * k);
     if (iters) *iters = it;
@@ -366,17 +387,18 @@ int qwsg_eo_cg(qwsg_lattice* h, void* x, const void* b, int maxiter, double tol,
 }
 int qwsg_eo_mcg(qwsg_lattice* h, void* const* xs, const void* b, int nshift, const double* sigma, int maxiter,
                 double tol, int pce, int* iters) {
-  return guarded([&] {
+  return guarded([&] { qws::need("qwsg_eo_mcg", xs, b);
     const double k = qws::lat_of(h)->kappa();
-    const int it = qws::lat_of(h)->mcg_eo((double* const*)xs, (const double*)b, nshift, sigma, maxiter, tol, pce, -k * k);
+    const int it =
+        qws::lat_of(h)->mcg_eo((double* const*)xs, (const double*)b, nshift, sigma, maxiter, tol, pce, -k * k);
     if (iters) *iters = it;
   });
 }
 int qwsg_norm2(qwsg_lattice* h, int prec, const void* a, double* out) {
-  return guarded([&] { *out = qws::lat_of(h)->norm2(prec, a); });
+  return guarded([&] { qws::need("qwsg_norm2", a); *out = qws::lat_of(h)->norm2(prec, a); });
 }
 int qwsg_spinor_digest(qwsg_lattice* h, int prec, const void* a, unsigned long long* out) {
-  return guarded([&] { *out = qws::lat_of(h)->digest(prec, a); });
+  return guarded([&] { qws::need("qwsg_spinor_digest", a); *out = qws::lat_of(h)->digest(prec, a); });
 }
 int qwsg_synchronize(qwsg_lattice* h) {
   return guarded([&] { qws::lat_of(h)->sync(); });
@@ -389,7 +411,7 @@ int qwsg_set_dslash_kernel(qwsg_lattice* h, int k) {
 }
 int qwsg_autotune_solve(qwsg_lattice* h, void* x, const void* b, const qwsg_solve_params* p, const int* nsap,
                         int nnsap, const int* nm, int nnm, qwsg_solve_params* best, double* seconds) {
-  return guarded([&] {
+  return guarded([&] { qws::need("qwsg_autotune_solve", x, b);
     if (!p || !best || !nsap || !nm || nnsap < 1 || nnm < 1) throw std::invalid_argument("autotune_solve: bad arguments");
     const qws::SolveParams sp = solve_params(p);
     std::vector<double> sec;

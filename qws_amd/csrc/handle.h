@@ -16,6 +16,13 @@ namespace qws {
 // the calling thread's last error message (qwsg_last_error)
 std::string& last_error();
 
+// invalid_argument when a pointer argument of entry point `what` is null
+inline void need(const char*) {}
+template <typename P, typename... R> void need(const char* what, P p, R... rest) {
+  if (!p) throw std::invalid_argument(std::string(what) + ": null pointer argument");
+  need(what, rest...);
+}
+
 // the lattice of a handle (invalid_argument for a null handle)
 inline Lattice* lat_of(qwsg_lattice* h) {
   if (!h || !h->lat) throw std::invalid_argument("null lattice handle");

@@ -35,6 +35,11 @@ def test_null_handles_are_refused():
     assert v.value == -1
     assert L.qwsg_server_run(None, None, 0, None) == 1
     assert b"null server handle" in L.qwsg_last_error()
+    # null spinor pointers are refused before the handle is used
+    L.qwsg_dslash.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
+                              ctypes.c_int]
+    assert L.qwsg_dslash(None, 0, None, None, 0, -1) == 1
+    assert b"qwsg_dslash: null pointer argument" in L.qwsg_last_error()
 
 
 def test_library_exports_qws_data_symbols():
