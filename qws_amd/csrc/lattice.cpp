@@ -1292,7 +1292,9 @@ void OuterBatch::admit(int k, const double* b, const double* x0) {
   upload_mask(only);
   const Batch B{cap_, stride_, d_mask_};
   double* part = L_.reduction_partials(cap_);
-  L_.dslash_batch(B, q_, x_);
+  // q = D x; for x = 0 that is 0, and r = b - q = b exactly without the operator
+  if (x0) L_.dslash_batch(B, q_, x_);
+  else QWS_HIP(hipMemsetAsync(q_ + (size_t)k * stride_, 0, bytes, st));
   launch_bicg_init_outer<double>(B, L_.volume() * 12, r_, p_, r0_, b_, q_, part, st);
   launch_finalize(B, part, 2, sc_, FIN_INIT_OUTER, st);
   state_[k] = RUNNING;
