@@ -10,6 +10,9 @@ solution of the reference's solve the same way).
 The 12 spin-colour point sources at the origin on the same field, one batched solve:
 every solution's residual recomputed by the oracle, and two of them equal to their
 single solves bit for bit.
+
+The 48^3x96 precision variant (FP16-stored SAP at kappa = 0.137): its true residual and
+its error against an FP64-refined continuation.
 """
 import numpy as np
 import pytest
@@ -170,3 +173,33 @@ def test_c5_served_mix_at_full_volumes(server):
         seen.add(q["field"])
         print(f"[configs] served: field {q['field']} nrhs {q['nrhs']} kappa {q['kappa']:.4f}", flush=True)
         check_request_with_oracle(q, mix[i], 2)
+
+
+def test_c3_precision_variant_full_size():
+    """48^3x96 (links exp(0.25 i H), seed 3, kappa = 0.137, c_sw = 1.1) with the FP16-stored
+    SAP: near the critical kappa an FP16-SAP inner solve diverges, the solve switches to
+    the FP32 SAP (sap_escalated) and still reaches an FP64 true residual of 1e-10; its
+    error against an FP64-refined continuation (the FP32-SAP solve from the result, driven
+    to 1e-13) stays at the level of that residual."""
+    print("[configs] 48^3x96 fields", flush=True)
+    lat = Q.Lattice([48, 48, 48, 96], [4, 4, 4, 4], -1, 12, kappa=0.137)
+    try:
+        lat.generate_gauge(0.25, 3)
+        lat.compute_clover(1.1)
+        b = lat.gaussian(Q.F64, 3)
+        x = lat.spinor(Q.F64)
+        print("[configs] 48^3x96 FP16-SAP solve", flush=True)
+        st = lat.solve(x, b, nsap=4, nm=6, tol=1e-11, maxiter=5, tol_s=1e-3, maxiter_s=40, true_tol=1e-10,
+                       max_restarts=60, sap_precision=Q.F16)
+        print("[configs] %s" % st, flush=True)
+        assert st["true_residual"] <= 1e-10 and st["sap_escalated"] == 1, st
+        xr = lat.spinor(Q.F64, x.download())
+        print("[configs] 48^3x96 FP64-refined continuation", flush=True)
+        ref = lat.solve(xr, b, nsap=3, nm=6, tol=1e-14, maxiter=5, tol_s=1e-3, maxiter_s=40, true_tol=1e-13,
+                        max_restarts=60)
+        assert ref["true_residual"] <= 1e-13, ref
+        err = rel(x.download(), xr.download())
+        print("[configs] error vs the FP64-refined continuation %.3e" % err, flush=True)
+        assert err <= 1e-9, err
+    finally:
+        lat.close()
