@@ -178,6 +178,7 @@ int qwsg_bicgstab_dd(qwsg_lattice* h, int prec, void* x, const void* b, int maxi
     if (iters) *iters = it;
   });
 }
+}  // extern "C"
 namespace {
 qws::SolveParams solve_params(const qwsg_solve_params* p) {
   if (!p) throw std::invalid_argument("no solve parameters");
@@ -203,6 +204,7 @@ void put_stats(const qws::SolveStats& s, qwsg_solve_stats* st) {
   st->sap_escalated = s.sap_escalated;
 }
 }  // namespace
+extern "C" {
 
 int qwsg_solve(qwsg_lattice* h, void* x, const void* b, const qwsg_solve_params* p, qwsg_solve_stats* st) {
   return guarded([&] { qws::need("qwsg_solve", x, b);
