@@ -210,6 +210,50 @@ def eo_operator(Q, lat, reps=20):
     return out
 
 
+def eo_solvers(Q, lat, iters=200):
+    """The even/odd solvers (bicgstab_vm_, cg_vm_, mcg_vm_ with 4 shifts) on the parity fields
+    of the 32^3x64 lattice (FP64, 18-real links), each run for a fixed number of iterations
+    (tol 0) from a Gaussian source: ms per iteration, the bandwidth of the mtilde applications
+    that dominate it (two mtilde-sized operator passes per iteration for all three: M p and
+    M s; mtilde and its adjoint; 4416 B per even site each, as in eo_operator) and the
+    relative residual reached, recomputed with mtilde on the downloaded solution."""
+    vh = lat.V // 2
+    rng = np.random.default_rng(5)
+    bh = rng.standard_normal((vh, 24))
+    b = lat.eo_spinor(Q.F64, bh)
+    out = {"workload": "even/odd solvers on the %d even sites of 32^3x64, FP64, %d iterations each from a Gaussian "
+                       "source" % (vh, iters)}
+    op_bytes = 2 * 4416 * vh
+
+    def resid(x, which):
+        r = lat.eo_spinor(Q.F64)
+        lat.mtilde(r, x, which=which)
+        d = r.download()
+        return float(np.linalg.norm(bh - d) / np.linalg.norm(bh))
+
+    def timed(f):
+        lat.sync()
+        t0 = time.perf_counter()
+        it = f()
+        lat.sync()
+        return time.perf_counter() - t0, it
+
+    x = lat.eo_spinor(Q.F64)
+    sec, it = timed(lambda: lat.eo_bicgstab(x, b, maxiter=iters, tol=0.0))
+    out["bicgstab_vm"] = {"ms_per_iteration": sec * 1e3 / it, "iterations": it,
+                          "operator_GBs": op_bytes * it / sec / 1e9, "relative_residual": resid(x, 0)}
+    x = lat.eo_spinor(Q.F64)
+    sec, it = timed(lambda: lat.eo_cg(x, b, maxiter=iters, tol=0.0))
+    out["cg_vm"] = {"ms_per_iteration": sec * 1e3 / it, "iterations": it,
+                    "operator_GBs": op_bytes * it / sec / 1e9, "relative_residual": resid(x, 2)}
+    shifts = [0.0, 0.01, 0.1, 1.0]
+    res = {}
+    sec, (xs, it) = timed(lambda: lat.eo_mcg(b, shifts, maxiter=iters, tol=0.0))
+    out["mcg_vm"] = {"shifts": shifts, "ms_per_iteration": sec * 1e3 / it, "iterations": it,
+                     "operator_GBs": op_bytes * it / sec / 1e9, "relative_residual_shift0": resid(xs[0], 2)}
+    return out
+
+
 def qws_host_layout(a, kind, L, vlen):
     """Plain fields (site x + Lx (y + Ly (z + Lz t))) -> QWS's host AoSoA (qws.h:111-192,
     main.cc:426-429,485-488): x split into two domains of Lx/2, vector i = xv + nxv (y + Ly
@@ -457,7 +501,7 @@ def main():
     ap.add_argument("--dropin", action="store_true",
                     help="the workload through bicgstab_dd_mix_ with host arrays (PCIe-inclusive rate)")
     ap.add_argument("--eo-only", type=int, default=0,
-                    help="only time N applications of the even/odd operator mtilde (FP64, FP32)")
+                    help="only the even/odd measurements: N applications of mtilde (FP64, FP32), the solvers")
     ap.add_argument("--sap-only", type=int, default=0,
                     help="only time N preconditioner applications of fused SAP launches (for counter runs)")
     ap.add_argument("--sweep", action="store_true", help="print solve statistics for a grid of solver settings")
@@ -542,7 +586,7 @@ def main():
         print(json.dumps({"dslash_ms": ms, "GBs": V * BYTES_PER_SITE_F32_12 / (ms * 1e-3) / 1e9}))
         return
     if a.eo_only:
-        print(json.dumps(eo_operator(Q, lat, a.eo_only)))
+        print(json.dumps({"eo": eo_operator(Q, lat, a.eo_only), "eo_solvers": eo_solvers(Q, lat)}), flush=True)
         return
     if a.sap_only:
         ms = lat.time_sap(Q.F32, a.sap_only, SOLVE["nsap"], SOLVE["nm"])
@@ -621,7 +665,8 @@ def main():
     line["error_vs_fp64"] = error_vs_fp64(Q, lat, x, b)
     progress("plain solves, dslash and the all-FP64 comparison done")
     line["eo"] = eo_operator(Q, lat)
-    progress("even/odd operator done")
+    line["eo_solvers"] = eo_solvers(Q, lat)
+    progress("even/odd operator and solvers done")
     # batched: the 12 point sources on the same field, one batch
     dt_b, st_b, xs_b, bs_b = batched_point_sources(Q, lat, SOLVE)
     line["batched"] = {"workload": "12 spin-colour point sources at the origin, one batched solve, 32^3x64",
