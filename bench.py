@@ -511,6 +511,8 @@ def main():
     ap.add_argument("--serve-slots", type=int, default=4, help="concurrent requests of the server (1..16)")
     ap.add_argument("--serve-rates", default="1,2,5,10,20,50,100,200", help="offered rates of the sweep (req/s)")
     ap.add_argument("--only-plain", action="store_true", help="only the plain solves and the D roofline (profiling)")
+    ap.add_argument("--blocks", action="store_true",
+                    help="time one solve of the workload per SAP block shape (qwsg_autotune_blocks)")
     ap.add_argument("--precision", action="store_true",
                     help="the 48^3x96 FP16-SAP precision variant and its error against an FP64-refined solution")
     ap.add_argument("--precision-kappa", type=float, default=None, help="kappa of the precision variant")
@@ -569,6 +571,12 @@ def main():
         return
     lat, b = setup(Q)
     V = lat.V
+    if a.blocks:
+        # 256-site (two workgroups per CU) and 512-site blocks; one untimed solve each first
+        shapes = [[4, 4, 4, 4], [8, 4, 4, 2], [8, 4, 2, 4], [8, 4, 4, 4], [4, 4, 4, 8]]
+        best, sec = lat.autotune_blocks(b, shapes, **SOLVE)
+        print(json.dumps({"block_shapes": shapes, "seconds_per_solve": sec, "best": best}), flush=True)
+        return
     if a.dslash_sweep:
         # block order x kernel variant of the full operator (FP32), and FP64 D per order
         for order in (0, 1):

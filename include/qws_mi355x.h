@@ -495,6 +495,17 @@ int qwsg_get_dslash_order(qwsg_lattice* lat, int* order);
  * (qwsg_autotune_dslash). */
 int qwsg_autotune_solve(qwsg_lattice* lat, void* x, const void* b, const qwsg_solve_params* p, const int* nsap,
                         int nnsap, const int* nm, int nnm, qwsg_solve_params* best, double* seconds);
+/* The SAP block shape tuned for this lattice's extents, FP64 fields and kappa: for
+ * each candidate shape (shapes[4i..4i+3], x y z t; nshape of them) a lattice of that
+ * shape is built with copies of the fields and solves b (FP64, this lattice's layout)
+ * once untimed, then once timed on the host clock with *p.  *best is the index of the
+ * fastest; seconds[i] (optional, nshape entries) is negative where the shape does not
+ * tile the extents or the timed solve missed p->true_tol.  The lattice itself keeps its
+ * blocks: create the production lattice with the chosen shape.  No counterpart in the
+ * reference, whose domains are the two x-halves of each process's local lattice
+ * (qws.cc:131, 622-631). */
+int qwsg_autotune_blocks(qwsg_lattice* lat, const void* b, const qwsg_solve_params* p, const int* shapes, int nshape,
+                         int* best, double* seconds);
 /* average device time (ms) of one fused SAP half-sweep launch (FP32 or FP16
  * storage), from `reps` times the 2*nsap+2 launches of one preconditioner
  * application on a Gaussian source, measured with HIP events on the lattice's

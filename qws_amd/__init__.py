@@ -101,6 +101,7 @@ def lib():
         L.qwsg_get_dslash_order.argtypes = [vp, ip]
         L.qwsg_autotune_solve.argtypes = [vp, vp, vp, ctypes.POINTER(SolveParams), vp, ctypes.c_int, vp, ctypes.c_int,
                                           ctypes.POINTER(SolveParams), vp]
+        L.qwsg_autotune_blocks.argtypes = [vp, vp, ctypes.POINTER(SolveParams), vp, ctypes.c_int, ip, vp]
         L.qwsg_time_sap.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                     ctypes.POINTER(ctypes.c_double)]
         L.qwsg_device_count.argtypes = [ip]
@@ -504,6 +505,21 @@ class Lattice:
                                         ctypes.byref(best), sec))
         out = dict(base, nsap=best.nsap, nm=best.nm)
         return out, [[sec[i * len(nms) + j] for j in range(len(nms))] for i in range(len(nsaps))]
+
+    def autotune_blocks(self, b, shapes, **params):
+        """Time one solve of b on a copy of this lattice for each candidate block shape;
+        returns (fastest shape, seconds per shape, negative where the shape does not tile
+        the lattice or missed the true residual).  This lattice keeps its blocks."""
+        base = dict(dict(nsap=4, nm=2, tol=-1.0, maxiter=6, tol_s=-1.0, maxiter_s=50, true_tol=-1.0, max_restarts=0,
+                         sap_precision=F32), **params)
+        p = SolveParams(base["nsap"], base["nm"], base["tol"], base["maxiter"], base["tol_s"], base["maxiter_s"],
+                        base["true_tol"], base["max_restarts"], base["sap_precision"])
+        n = len(shapes)
+        sh = (ctypes.c_int * (4 * n))(*[int(v) for s in shapes for v in s])
+        sec = (ctypes.c_double * n)()
+        best = ctypes.c_int()
+        check(lib().qwsg_autotune_blocks(self.h, b.ptr, ctypes.byref(p), sh, n, ctypes.byref(best), sec))
+        return list(shapes[best.value]), [sec[i] for i in range(n)]
 
     def set_dslash_order(self, order):
         """Block order of the launches over every block: 0 index, 1 t-z pencils per XCD."""

@@ -426,6 +426,16 @@ int qwsg_autotune_solve(qwsg_lattice* h, void* x, const void* b, const qwsg_solv
     if (seconds) std::copy(sec.begin(), sec.end(), seconds);
   });
 }
+int qwsg_autotune_blocks(qwsg_lattice* h, const void* b, const qwsg_solve_params* p, const int* shapes, int nshape,
+                         int* best, double* seconds) {
+  return guarded([&] { qws::need("qwsg_autotune_blocks", b, p, shapes, best);
+    if (nshape < 1) throw std::invalid_argument("autotune_blocks: no candidate shapes");
+    std::vector<double> sec;
+    *best = qws::lat_of(h)->autotune_blocks((const double*)b, solve_params(p),
+                                            std::vector<int>(shapes, shapes + 4 * (size_t)nshape), sec);
+    if (seconds) std::copy(sec.begin(), sec.end(), seconds);
+  });
+}
 int qwsg_set_dslash_order(qwsg_lattice* h, int order) {
   return guarded([&] { qws::lat_of(h)->set_dslash_order(order); });
 }

@@ -1532,6 +1532,51 @@ SolveParams Lattice::autotune_solve(double* x, const double* b, const SolveParam
   return best;
 }
 
+int Lattice::autotune_blocks(const double* b, const SolveParams& prm, const std::vector<int>& shapes,
+                             std::vector<double>& seconds) {
+  if (shapes.empty() || shapes.size() % 4 != 0) throw std::invalid_argument("autotune_blocks: shapes holds 4 ints each");
+  if (!Ud_ || !Cd_) throw std::runtime_error("autotune_blocks: FP64 gauge/clover fields are not loaded");
+  // the fields and b leave the device in the plain site order, which every block shape reads
+  std::vector<double> U((size_t)g_.V * 72), C((size_t)g_.V * 72), bh((size_t)g_.V * 24);
+  export_field(1, U.data());
+  export_field(2, C.data());
+  download_spinor(P64, b, bh.data(), 0, 1);
+  sync();
+  const size_t n = shapes.size() / 4;
+  seconds.assign(n, -1.0);
+  int best = -1;
+  for (size_t i = 0; i < n; ++i) {
+    std::unique_ptr<Lattice> c;
+    try {
+      c.reset(new Lattice(g_.L, &shapes[4 * i], g_.tbc, nl32_));
+    } catch (const std::invalid_argument&) {
+      continue;  // not a tiling this solver admits
+    }
+    c->set_kappa(kappa_);
+    c->load_gauge(U.data(), 0, 1, 3);
+    c->load_clover(C.data(), 0, 1, 3);
+    c->autotune_dslash(20);
+    double* x = (double*)c->alloc_spinor(P64);
+    double* bc = (double*)c->alloc_spinor(P64);
+    c->upload_spinor(P64, bc, bh.data(), 0, 1);
+    double* xs[1] = {x};
+    const double* bs[1] = {bc};
+    c->solve_batch(xs, bs, 1, prm);  // untimed: the work buffers are allocated here
+    QWS_HIP(hipMemsetAsync(x, 0, c->spinor_bytes(P64), c->st_));
+    c->sync();
+    const auto t0 = std::chrono::steady_clock::now();
+    const SolveStats st = c->solve_batch(xs, bs, 1, prm)[0];
+    const double sec = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    QWS_HIP(hipFree(x));
+    QWS_HIP(hipFree(bc));
+    if (prm.true_tol > 0 && !(st.true_residual <= prm.true_tol)) continue;
+    seconds[i] = sec;
+    if (best < 0 || sec < seconds[best]) best = (int)i;
+  }
+  if (best < 0) throw std::runtime_error("autotune_blocks: no admissible shape met the true-residual target");
+  return best;
+}
+
 double Lattice::time_sap(int prec, int reps, int nsap, int nm) {
   if (reps < 1 || nsap < 1 || nm < 1) throw std::invalid_argument("time_sap: reps, nsap and nm must be positive");
   if (prec == P32) return time_sap_t<float>(reps, nsap, nm);

@@ -235,6 +235,14 @@ class Lattice {
   // (negative: did not meet it).  Also picks the full-operator kernel (autotune_dslash).
   SolveParams autotune_solve(double* x, const double* b, const SolveParams& prm, const std::vector<int>& nsaps,
                              const std::vector<int>& nms, std::vector<double>& seconds);
+  // The SAP block shape tuned for this lattice's extents, FP64 fields and kappa: for each
+  // candidate (shapes[4 i .. 4 i + 3], x y z t) a lattice of that shape is built with
+  // copies of the fields, solves b (FP64, this lattice's layout) once untimed and once
+  // timed on the host clock with prm; seconds[i] is negative where the shape is not
+  // admissible (does not tile the extents, odd block counts) or the timed solve missed
+  // prm.true_tol.  Returns the index of the fastest shape.  This lattice is not changed.
+  int autotune_blocks(const double* b, const SolveParams& prm, const std::vector<int>& shapes,
+                      std::vector<double>& seconds);
   void sync() { QWS_HIP(hipStreamSynchronize(st_)); }
 
   // ---- even/odd family (deo_*, mtilde) on x-compressed parity fields of V/2 sites
