@@ -40,6 +40,10 @@ def test_null_handles_are_refused():
                               ctypes.c_int]
     assert L.qwsg_dslash(None, 0, None, None, 0, -1) == 1
     assert b"qwsg_dslash: null pointer argument" in L.qwsg_last_error()
+    best = ctypes.c_int(-1)
+    assert L.qwsg_autotune_blocks(None, None, None, None, 1, ctypes.byref(best), None) == 1
+    assert b"qwsg_autotune_blocks: null pointer argument" in L.qwsg_last_error()
+    assert best.value == -1
 
 
 def test_library_exports_qws_data_symbols():
