@@ -137,28 +137,40 @@ k_norm(long long n, long long stride, const int* active, const R* a, double* par
 }
 
 // x += c v1 ; r -= c v2 ; |r|^2 [, <r0,r>]   (bicgstab_precdd_s.cc:281-295,427-440)
-// c = alpha (coef 0) or omega (coef 1) of this RHS.
-template <typename R, bool WITH_R0>
+// c = alpha (coef 0) or omega (coef 1) of this RHS.  MODE (XR_*): XR_FULL as above;
+// XR_R_ONLY leaves x alone (x and v1 unused); XR_DEFERRED first adds alpha v0 to x, the
+// x update of the previous half-step, then c v1 (the same roundings as two passes).
+template <typename R, bool WITH_R0, int MODE>
 __global__ void __launch_bounds__(RED_THREADS)
-k_update_xr(long long n, long long stride, const int* active, R* x, const R* v1, R* r, const R* v2, const Scalars* sc,
-            int coef, const R* r0, double* part) {
+k_update_xr(long long n, long long stride, const int* active, R* x, const R* v0, const R* v1, R* r, const R* v2,
+            const Scalars* sc, int coef, const R* r0, double* part) {
   using T = typename V2<R>::T;
   if (rhs_off(active)) return;
   const Scalars& S = sc[blockIdx.y];
   const double* c = coef == 0 ? S.alpha : S.omega;
   const R cr = (R)c[0], ci = (R)c[1];
-  T *X = RHS(x, T), *Rr = RHS(r, T);
-  const T *A = RHS(v1, const T), *Qv = RHS(v2, const T), *Z = WITH_R0 ? RHS(r0, const T) : nullptr;
+  const R ar = (R)S.alpha[0], ai = (R)S.alpha[1];
+  T *X = MODE == XR_R_ONLY ? nullptr : RHS(x, T), *Rr = RHS(r, T);
+  const T *A = MODE == XR_R_ONLY ? nullptr : RHS(v1, const T), *Qv = RHS(v2, const T);
+  const T *P0 = MODE == XR_DEFERRED ? RHS(v0, const T) : nullptr, *Z = WITH_R0 ? RHS(r0, const T) : nullptr;
   double acc[WITH_R0 ? 3 : 1];
 #pragma unroll
   for (int k = 0; k < (WITH_R0 ? 3 : 1); ++k) acc[k] = 0;
   GRID_STRIDE(i, n) {
-    T xv = X[i], a = A[i], rv = Rr[i], q = Qv[i];
-    xv.x = xv.x + cr * a.x - ci * a.y;
-    xv.y = xv.y + cr * a.y + ci * a.x;
+    T rv = Rr[i], q = Qv[i];
+    if constexpr (MODE != XR_R_ONLY) {
+      T xv = X[i], a = A[i];
+      if constexpr (MODE == XR_DEFERRED) {
+        const T a0 = P0[i];
+        xv.x = xv.x + ar * a0.x - ai * a0.y;
+        xv.y = xv.y + ar * a0.y + ai * a0.x;
+      }
+      xv.x = xv.x + cr * a.x - ci * a.y;
+      xv.y = xv.y + cr * a.y + ci * a.x;
+      X[i] = xv;
+    }
     rv.x = rv.x - cr * q.x + ci * q.y;
     rv.y = rv.y - cr * q.y - ci * q.x;
-    X[i] = xv;
     Rr[i] = rv;
     acc[0] += (double)rv.x * rv.x + (double)rv.y * rv.y;
     if (WITH_R0) {
@@ -168,6 +180,26 @@ k_update_xr(long long n, long long stride, const int* active, R* x, const R* v1,
     }
   }
   block_reduce_store<WITH_R0 ? 3 : 1>(acc, part);
+}
+
+// x += alpha v0: the x update of a half-step whose residual met the tolerance, where
+// that update is otherwise deferred to the next half-step (XR_DEFERRED)
+template <typename R>
+__global__ void __launch_bounds__(RED_THREADS)
+k_update_x(long long n, long long stride, const int* active, R* x, const R* v0, const Scalars* sc) {
+  using T = typename V2<R>::T;
+  if (rhs_off(active)) return;
+  const Scalars& S = sc[blockIdx.y];
+  const R ar = (R)S.alpha[0], ai = (R)S.alpha[1];
+  T* X = RHS(x, T);
+  const T* P0 = RHS(v0, const T);
+  GRID_STRIDE(i, n) {
+    T xv = X[i];
+    const T a0 = P0[i];
+    xv.x = xv.x + ar * a0.x - ai * a0.y;
+    xv.y = xv.y + ar * a0.y + ai * a0.x;
+    X[i] = xv;
+  }
 }
 
 // p = r + beta (p - omega q)   (bicgstab_precdd_s.cc:529-543)
@@ -414,9 +446,20 @@ void launch_norm_dot(const Batch& B, long long n, const R* t, const R* r, double
 }
 template <typename R>
 void launch_update_xr(const Batch& B, long long n, R* x, const R* v1, R* r, const R* v2, const Scalars* sc, int coef,
-                      const R* r0, double* part, hipStream_t st) {
-  if (r0) LAUNCH_B((k_update_xr<R, true>), B.n, n, B.stride, B.active, x, v1, r, v2, sc, coef, r0, part);
-  else LAUNCH_B((k_update_xr<R, false>), B.n, n, B.stride, B.active, x, v1, r, v2, sc, coef, r0, part);
+                      const R* r0, double* part, hipStream_t st, int mode, const R* v0) {
+  if (mode == XR_DEFERRED && (!r0 || !v0)) throw std::invalid_argument("update_xr: the deferred update needs r0 and v0");
+  if (mode == XR_R_ONLY && r0) throw std::invalid_argument("update_xr: the r-only update takes no r0");
+  if (mode == XR_DEFERRED)
+    LAUNCH_B((k_update_xr<R, true, XR_DEFERRED>), B.n, n, B.stride, B.active, x, v0, v1, r, v2, sc, coef, r0, part);
+  else if (mode == XR_R_ONLY)
+    LAUNCH_B((k_update_xr<R, false, XR_R_ONLY>), B.n, n, B.stride, B.active, x, v0, v1, r, v2, sc, coef, r0, part);
+  else if (r0)
+    LAUNCH_B((k_update_xr<R, true, XR_FULL>), B.n, n, B.stride, B.active, x, v0, v1, r, v2, sc, coef, r0, part);
+  else LAUNCH_B((k_update_xr<R, false, XR_FULL>), B.n, n, B.stride, B.active, x, v0, v1, r, v2, sc, coef, r0, part);
+}
+template <typename R>
+void launch_update_x(const Batch& B, long long n, R* x, const R* v0, const Scalars* sc, hipStream_t st) {
+  LAUNCH_B((k_update_x<R>), B.n, n, B.stride, B.active, x, v0, sc);
 }
 template <typename R>
 void launch_update_p(const Batch& B, long long n, R* p, const R* r, const R* q, const Scalars* sc, hipStream_t st) {
@@ -532,7 +575,8 @@ void launch_stream(int cfg, long long n4, int nin, const float4* in, float4* out
   template void launch_dot<R>(const Batch&, long long, const R*, const R*, double*, hipStream_t);                     \
   template void launch_norm_dot<R>(const Batch&, long long, const R*, const R*, double*, hipStream_t);                \
   template void launch_update_xr<R>(const Batch&, long long, R*, const R*, R*, const R*, const Scalars*, int,         \
-                                    const R*, double*, hipStream_t);                                                  \
+                                    const R*, double*, hipStream_t, int, const R*);                                   \
+  template void launch_update_x<R>(const Batch&, long long, R*, const R*, const Scalars*, hipStream_t);              \
   template void launch_update_p<R>(const Batch&, long long, R*, const R*, const R*, const Scalars*, hipStream_t);     \
   template void launch_norm<R>(const Batch&, long long, const R*, double*, hipStream_t);                              \
   template void launch_sub<R>(long long, R*, const R*, const R*, hipStream_t);                                      \

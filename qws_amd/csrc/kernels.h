@@ -77,10 +77,15 @@ void launch_bicg_init_outer(const Batch& B, long long n, R* r, R* p, R* r0, cons
 template <typename R> void launch_dot(const Batch& B, long long n, const R* a, const R* b, double* part, hipStream_t st);
 template <typename R>
 void launch_norm_dot(const Batch& B, long long n, const R* t, const R* r, double* part, hipStream_t st);
-// x += c v1 ; r -= c v2 ; red |r|^2 (and <r0,r> if r0); c = alpha (coef 0) or omega (coef 1)
+// x += c v1 ; r -= c v2 ; red |r|^2 (and <r0,r> if r0); c = alpha (coef 0) or omega (coef 1).
+// mode XR_R_ONLY: r only (r0 must be null); XR_DEFERRED: x += alpha v0 first (r0 required)
+enum { XR_FULL = 0, XR_R_ONLY = 1, XR_DEFERRED = 2 };
 template <typename R>
 void launch_update_xr(const Batch& B, long long n, R* x, const R* v1, R* r, const R* v2, const Scalars* sc, int coef,
-                      const R* r0, double* part, hipStream_t st);
+                      const R* r0, double* part, hipStream_t st, int mode = XR_FULL, const R* v0 = nullptr);
+// x += alpha v0 (alpha of each RHS)
+template <typename R>
+void launch_update_x(const Batch& B, long long n, R* x, const R* v0, const Scalars* sc, hipStream_t st);
 // p = r + beta (p - omega q)
 template <typename R>
 void launch_update_p(const Batch& B, long long n, R* p, const R* r, const R* q, const Scalars* sc, hipStream_t st);

@@ -209,6 +209,7 @@ void Fused::bicg(std::vector<InnerJob>& J, double tol, int maxiter, int nsap, in
     double* part;
     std::vector<int> live;
     std::vector<double> bnorm;
+    std::vector<int> fix;  // RHS leaving after the first half-step (host copy of the mask)
     long long n;
     Batch B;
     SapSrcUpdate pupd;
@@ -304,10 +305,27 @@ void Fused::bicg(std::vector<InnerJob>& J, double tol, int maxiter, int nsap, in
       hipStream_t st = J[i].L->st_;
       launch_dot<float>(s.B, s.n, s.r0, s.q, s.part, st);
       launch_finalize(s.B, s.part, 2, s.sc, FIN_ALPHA, st);
-      launch_update_xr<float>(s.B, s.n, J[i].x, s.p, s.r, s.q, s.sc, 0, nullptr, s.part, st);
+      // r -= alpha q; x += alpha p is deferred to the second half-step's update (two
+      // spinor passes fewer), or done below for the RHS that leave here
+      launch_update_xr<float>(s.B, s.n, J[i].x, s.p, s.r, s.q, s.sc, 0, nullptr, s.part, st, XR_R_ONLY);
       launch_finalize(s.B, s.part, 1, s.sc, FIN_RNORM, st);
     }
-    if (!converge(act)) break;
+    std::vector<std::vector<int>> before(np);
+    for (size_t i : act) before[i] = S[i].live;
+    const bool more = converge(act);
+    for (size_t i : act) {
+      St& s = S[i];
+      Lattice& L = *J[i].L;
+      const int nb = J[i].Bin.n;
+      s.fix.assign(nb, 0);
+      bool any = false;
+      for (int k = 0; k < nb; ++k) any |= (s.fix[k] = before[i][k] && !s.live[k]) != 0;
+      if (!any) continue;
+      int* dfix = (int*)L.work("bicg_fix", sizeof(int) * nb);
+      QWS_HIP(hipMemcpyAsync(dfix, s.fix.data(), sizeof(int) * nb, hipMemcpyHostToDevice, L.st_));
+      launch_update_x<float>(Batch{nb, s.B.stride, dfix}, s.n, J[i].x, s.p, s.sc, L.st_);
+    }
+    if (!more) break;
     act = live_parts();
     jobs.clear();
     for (size_t i : act) jobs.push_back({J[i].L, S[i].B, S[i].t, S[i].r, nullptr});
@@ -317,7 +335,7 @@ void Fused::bicg(std::vector<InnerJob>& J, double tol, int maxiter, int nsap, in
       hipStream_t st = J[i].L->st_;
       launch_norm_dot<float>(s.B, s.n, s.t, s.r, s.part, st);
       launch_finalize(s.B, s.part, 3, s.sc, FIN_OMEGA, st);
-      launch_update_xr<float>(s.B, s.n, J[i].x, s.r, s.r, s.t, s.sc, 1, s.r0, s.part, st);
+      launch_update_xr<float>(s.B, s.n, J[i].x, s.r, s.r, s.t, s.sc, 1, s.r0, s.part, st, XR_DEFERRED, s.p);
       launch_finalize(s.B, s.part, 3, s.sc, FIN_RHO, st);
     }
     if (!converge(act)) break;
