@@ -527,6 +527,30 @@ def test_inner_bicgstab(prob):
     assert rel(x.download(), want) < 1e-4
 
 
+def test_inner_bicgstab_stops_at_either_half_step():
+    """The inner solve with a tolerance (bicgstab_precdd_s.cc:594) ends after the half-step
+    whose residual meets it, the first or the second of an iteration, and x then holds that
+    half-step's update: equal to the oracle at both kinds of exit (which kind each tolerance
+    gives is read off the oracle: an exit after the second half-step equals the fixed-count
+    run of the same iterations)."""
+    p = Problem([16, 8, 8, 8], [4, 4, 4, 4])
+    lat = p.lat
+    bs = (p.b / np.sqrt(O.norm2(p.b))).astype(np.float32)
+    halves = set()
+    for tol in (1e-2, 1e-3, 1e-5):
+        want, it_o = p.orc.bicgstab_precdd_s(bs, tol, 50, 4, 2)
+        fixed, _ = p.orc.bicgstab_precdd_s(bs, -1.0, it_o, 4, 2)
+        halves.add((it_o, 2 if np.array_equal(want, fixed) else 1))
+        x = lat.spinor(Q.F32)
+        it = lat.bicgstab_precdd(x, lat.spinor(Q.F32, bs), tol, 50, 4, 2)
+        print(f"[inner] tol {tol:g}: {it} / {it_o} iterations", flush=True)
+        assert it == it_o
+        assert rel(x.download(), want) < 1e-4
+    assert {h for _, h in halves} == {1, 2}, halves  # both exits exercised
+    assert any(i > 1 and h == 1 for i, h in halves)  # a first-half exit after a full iteration
+    lat.close()
+
+
 def test_links12_reconstruction():
     """12-real FP32 links (third row rebuilt) equal 18-real storage for SU(3) links."""
     p = Problem([8, 8, 4, 8], [4, 4, 2, 4], su3=True, seed=11)
