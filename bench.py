@@ -66,6 +66,21 @@ def sap_flops_per_site(B, nsap, nm):
     return total / n
 
 
+# Compulsory HBM bytes per site of the lattice for one FP32 inner BiCGStab iteration
+# (fields read once, outputs written once): two preconditioner applications, each
+# 2 nsap + 2 fused SAP launches over V/2 sites at 960 B (links 192, clover 288, source or
+# iterate 96, the other colour's face data 96, the residual update's source 96, two
+# stored spinors 192) and one final face-term launch over V/2 sites at 624 B (clover 288,
+# output read and written 192, face slots 96, the backward hops' own links 48); the
+# vector updates (r only: 3 spinor passes; x and r: 7) and the dot products (4 passes)
+# at 96 B per spinor pass.
+def inner_iteration_bytes_per_site(nsap):
+    sap = 2 * (2 * nsap + 2) * 0.5 * 960
+    face = 2 * 0.5 * 624
+    vectors = (3 + 7 + 4) * 96
+    return sap + face + vectors
+
+
 def progress(msg):
     """A progress line on stderr (stdout carries only the JSON result)."""
     print("[bench] " + msg, file=sys.stderr, flush=True)
@@ -651,6 +666,10 @@ def main():
         "iterations": {"outer": stats[-1]["outer_iters"], "inner": stats[-1]["inner_iters"],
                        "restarts": stats[-1]["restarts"]},
         "dslash_GBs": achieved,
+        # compulsory bytes of the inner solve's launches per wall second (the FP64 outer
+        # iterations' traffic is not counted: a lower bound)
+        "effective_GBs": stats[-1]["inner_iters"] * V * inner_iteration_bytes_per_site(SOLVE["nsap"]) / (
+            ms_per * 1e-3) / 1e9,
         "roofline": {"bound": "valu", "achieved": achieved_s, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved_s / FP32_PEAK_TFLOPS, "traffic": traffic("sap_f32_32c64_traffic.json"),
                      "kernel": "k_sap_block<float,12> (fused SAP half-sweep, FP32, %.0f flop/site algorithmic, "
