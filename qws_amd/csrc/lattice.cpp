@@ -1105,6 +1105,7 @@ void Lattice::bicg_t(const Batch& Bin, S* x, const S* b, double tol, int maxiter
   S* t = (S*)work("bicg_t", bytes);
   S* r0 = (S*)work("bicg_r0", bytes);
   S* w = (S*)work("bicg_w", bytes);
+  S* w2 = (S*)work("bicg_w2", bytes);
   const float mk = (float)(-kappa_);
   if (nb > sc_in_n_) {
     if (sc_in_) QWS_HIP(hipFree(sc_in_));
@@ -1156,14 +1157,26 @@ void Lattice::bicg_t(const Batch& Bin, S* x, const S* b, double tol, int maxiter
     hop<S>(sel(-1), B, HOPS_ALL, mk, q, w, nullptr, nullptr, w);
     launch_dot<S>(B, n, r0, q, part, st_);
     launch_finalize(B, part, 2, sc, FIN_ALPHA, st_);
-    launch_update_xr<S>(B, n, x, w, r, q, sc, 0, nullptr, part, st_);
+    // r -= alpha q; x += alpha M p is deferred to the second half-step's update, or done
+    // here for the RHS that leave after this half-step
+    launch_update_xr<S>(B, n, x, w, r, q, sc, 0, nullptr, part, st_, XR_R_ONLY);
     launch_finalize(B, part, 1, sc, FIN_RNORM, st_);
-    if (!converge()) break;
-    msap_h_t(B, w, r, nsap, nm);
-    hop<S>(sel(-1), B, HOPS_ALL, mk, t, w, nullptr, nullptr, w);
+    const std::vector<int> before(live);
+    const bool more = converge();
+    fix_.assign(nb, 0);
+    bool anyfix = false;
+    for (int k = 0; k < nb; ++k) anyfix |= (fix_[k] = before[k] && !live[k]) != 0;
+    if (anyfix) {
+      int* dfix = (int*)work("bicg_fix", sizeof(int) * nb);
+      QWS_HIP(hipMemcpyAsync(dfix, fix_.data(), sizeof(int) * nb, hipMemcpyHostToDevice, st_));
+      launch_update_x<S>(Batch{nb, B.stride, dfix}, n, x, w, sc, st_);
+    }
+    if (!more) break;
+    msap_h_t(B, w2, r, nsap, nm);
+    hop<S>(sel(-1), B, HOPS_ALL, mk, t, w2, nullptr, nullptr, w2);
     launch_norm_dot<S>(B, n, t, r, part, st_);
     launch_finalize(B, part, 3, sc, FIN_OMEGA, st_);
-    launch_update_xr<S>(B, n, x, w, r, t, sc, 1, r0, part, st_);
+    launch_update_xr<S>(B, n, x, w2, r, t, sc, 1, r0, part, st_, XR_DEFERRED, w);
     launch_finalize(B, part, 3, sc, FIN_RHO, st_);
     if (!converge()) break;
     launch_update_p<S>(B, n, p, r, q, sc, st_);  // p = r + beta (p - omega q)

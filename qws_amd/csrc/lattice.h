@@ -340,6 +340,7 @@ class Lattice {
   int part_rhs_ = 0;
   Scalars* sc_in_ = nullptr;
   int sc_in_n_ = 0;
+  std::vector<int> fix_;  // the FP16-SAP inner solve: RHS leaving after a first half-step
   Scalars* sc_one_ = nullptr;
   Scalars* sc_host_ = nullptr;  // pinned, room for MAX_BATCH
   // host <-> device copies of caller memory (pageable) through two pinned chunks
