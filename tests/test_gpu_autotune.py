@@ -63,3 +63,26 @@ def test_autotune_blocks_solves_the_same_problem_on_every_shape():
     assert err < 1e-6  # |r|/|b| <= 1e-10 times the operator's condition number
     c.close()
     lat.close()
+
+
+def test_autotune_blocks_with_18_real_links_and_bad_candidates():
+    """With 18-real FP32 links the 512-site shapes run the unfused SAP (the fused kernel's
+    LDS holds 12-real links at that size): still admissible and converging.  An empty
+    candidate list and a lattice without FP64 fields are refused."""
+    L = [16, 8, 8, 8]
+    lat = Q.Lattice(L, [4, 4, 4, 4], -1, 18, kappa=0.12)
+    lat.generate_gauge(0.3, 23)
+    lat.compute_clover(1.0)
+    b = lat.gaussian(Q.F64, 7)
+    prm = dict(tol=1e-11, maxiter=40, tol_s=1e-3, maxiter_s=80, true_tol=1e-10, max_restarts=4)
+    best, sec = lat.autotune_blocks(b, [[4, 4, 4, 4], [8, 4, 4, 4]], **prm)
+    print("[autotune_blocks 18-real] seconds", [round(s, 4) for s in sec], "best", best, flush=True)
+    assert all(s > 0 for s in sec)
+    with pytest.raises(Q.QwsError):
+        lat.autotune_blocks(b, [], **prm)
+    bare = Q.Lattice(L, [4, 4, 4, 4], -1, 18, kappa=0.12)
+    bare.load_gauge_f32(lat.gauge().astype(np.float32))
+    with pytest.raises(Q.QwsError):
+        bare.autotune_blocks(bare.gaussian(Q.F64, 7), [[4, 4, 4, 4]], **prm)
+    bare.close()
+    lat.close()
