@@ -185,6 +185,73 @@ void launch_dslash_lds(const Geo& g, BlockSel sel, const int* active, float gamm
 #undef DSL_LDS
 }
 
+// One FP64 hop of the LDS-staged full operator: the arithmetic of one_hop<double, 18>,
+// the neighbour's spinor from LDS when it lies in the block.
+template <int MU, int DIR>
+__device__ __forceinline__ void hop_staged_d(const Geo& g, long long s, const SiteC& c, int t, const double2* lx,
+                                             const double* in, const double* U, Cx<double> acc[12]) {
+  if (!(g.hopmask & (1 << (2 * MU + (DIR > 0 ? 0 : 1))))) return;
+  bool inside;
+  const long long n = neighbour(g, s, c, MU, DIR, inside);
+  Cx<double> v[12], a[6], b[6], u[9];
+  if (inside) {
+    const int tn = t + DIR * g.sB[MU];
+#pragma unroll
+    for (int k = 0; k < 12; ++k) {
+      const double2 q = lx[k * HOP_THREADS + tn];
+      v[k] = {q.x, q.y};
+    }
+  } else {
+    SpinorIO<double>::load(in, n, g.V, v);
+  }
+  project<double, MU, DIR>(v, a);
+  load_link<double, 18>(U, MU, DIR > 0 ? s : n, g.V, u);
+  link_mult<double, DIR>(u, a, b);
+  double tf = 2.0;
+  if (MU == 3) {
+    const int tt = c.b[3] * g.B[3] + c.l[3];
+    if ((DIR > 0 && tt == g.L[3] - 1) || (DIR < 0 && tt == 0)) tf = double(2 * g.tbc);
+  }
+  reconstruct<double, MU, DIR>(b, tf, acc);
+}
+
+// FP64 full operator (the outer solver's D, ddd_d_) on 256-site blocks with the block's
+// spinor staged in LDS (12 double2 planes, 48 KB): bit for bit k_dslash<double, 18>.
+template <int W>
+__global__ void __launch_bounds__(HOP_THREADS) __attribute__((amdgpu_waves_per_eu(1, W)))
+k_dslash_lds_d(Geo g, const int* __restrict__ order, const int* __restrict__ active, double gamma,
+               double* __restrict__ out, const double* __restrict__ in, const double* __restrict__ U,
+               const double* __restrict__ C) {
+  __shared__ double2 lx[12 * HOP_THREADS];
+  if (active && !*active) return;
+  const int t = threadIdx.x;
+  const long long s = (long long)order[xcd_remap(g, blockIdx.x, gridDim.x)] * HOP_THREADS + t;
+  const double2* iv = reinterpret_cast<const double2*>(in);
+#pragma unroll
+  for (int k = 0; k < 12; ++k) lx[k * HOP_THREADS + t] = iv[(long long)k * g.V + s];
+  __syncthreads();
+  SiteC sc;
+  site_coords(g, s, sc);
+  Cx<double> acc[12];
+#pragma unroll
+  for (int k = 0; k < 12; ++k) acc[k] = {0, 0};
+  hop_staged_d<0, +1>(g, s, sc, t, lx, in, U, acc);
+  hop_staged_d<0, -1>(g, s, sc, t, lx, in, U, acc);
+  hop_staged_d<1, +1>(g, s, sc, t, lx, in, U, acc);
+  hop_staged_d<1, -1>(g, s, sc, t, lx, in, U, acc);
+  hop_staged_d<2, +1>(g, s, sc, t, lx, in, U, acc);
+  hop_staged_d<2, -1>(g, s, sc, t, lx, in, U, acc);
+  hop_staged_d<3, +1>(g, s, sc, t, lx, in, U, acc);
+  hop_staged_d<3, -1>(g, s, sc, t, lx, in, U, acc);
+  clover_apply<double>(C, s, g.V, acc);
+  double2* ov = reinterpret_cast<double2*>(out);
+#pragma unroll
+  for (int k = 0; k < 12; ++k) {
+    const double2 o = lx[k * HOP_THREADS + t];
+    ov[(long long)k * g.V + s] = make_double2(fmar(gamma, acc[k].re, o.x), fmar(gamma, acc[k].im, o.y));
+  }
+}
+
 template <typename S>
 __global__ void __launch_bounds__(HOP_THREADS)
 k_zero_blocks(Geo g, const int* list, int wg_per_blk, int nrhs, long long stride, const int* active, S* out) {
@@ -211,6 +278,20 @@ void launch_hop(const Geo& g, BlockSel sel, const Batch& B, int set, typename St
         for (int r = 0; r < B.n; ++r)
           launch_dslash_lds<NL>(g, sel, B.active ? B.active + r : nullptr, gamma, out + r * B.stride,
                                 in + r * B.stride, U, C, st);
+        return;
+      }
+    }
+    if constexpr (std::is_same<S, double>::value && NL == 18) {
+      if (g.Vb == HOP_THREADS && g.dslash_kernel_d != 0) {
+        for (int r = 0; r < B.n; ++r) {
+          const int* act = B.active ? B.active + r : nullptr;
+          if (g.dslash_kernel_d == 2)
+            hipLaunchKernelGGL((k_dslash_lds_d<2>), dim3(sel.n), dim3(HOP_THREADS), 0, st, g, sel.list, act, gamma,
+                               out + r * B.stride, in + r * B.stride, U, C);
+          else
+            hipLaunchKernelGGL((k_dslash_lds_d<3>), dim3(sel.n), dim3(HOP_THREADS), 0, st, g, sel.list, act, gamma,
+                               out + r * B.stride, in + r * B.stride, U, C);
+        }
         return;
       }
     }

@@ -88,6 +88,12 @@ Lattice::Lattice(const int L[4], const int B[4], int tbc, int links32) {
   const char* dk = std::getenv("QWS_MI355X_DSLASH");
   g_.dslash_kernel = dk ? std::atoi(dk) : 4;
   if (g_.dslash_kernel != 0 && g_.dslash_kernel != 3) g_.dslash_kernel = 4;
+  // FP64 full operator (18-real links, 256-site blocks): LDS-staged for 2 waves per SIMD
+  // (32^3x64: 0.82 ms against 0.96-1.02 ms with global gathers, bit-identical), unless
+  // QWS_MI355X_DSLASH_D names another (0 gathers, 3 LDS-staged for 3 waves)
+  const char* dkd = std::getenv("QWS_MI355X_DSLASH_D");
+  g_.dslash_kernel_d = dkd ? std::atoi(dkd) : 2;
+  if (g_.dslash_kernel_d != 0 && g_.dslash_kernel_d != 3) g_.dslash_kernel_d = 2;
   nl32_ = links32;
   pin_chunk_ = pin_chunk_from_env();
   QWS_HIP(hipStreamCreate(&st_));

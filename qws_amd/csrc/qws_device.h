@@ -45,6 +45,7 @@ struct Geo {
   int xcd_remap;  // 1: map workgroups to blocks XCD by XCD (xcd_remap), 0: identity
   int dslash_kernel;  // FP32 full operator on 256-site blocks: 0 global gathers (k_dslash),
                       // 3/4: block spinor staged in LDS, compiled for that many waves per SIMD
+  int dslash_kernel_d;  // the same for FP64 (18-real links): 0 gathers, 2/3 LDS-staged
 };
 
 // MI355X dispatches the workgroups of a launch round-robin over its 8 XCDs, each

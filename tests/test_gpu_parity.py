@@ -551,6 +551,25 @@ def test_inner_bicgstab_stops_at_either_half_step():
     lat.close()
 
 
+def test_fp64_dslash_kernels_agree_bitwise(monkeypatch):
+    """The FP64 full operator on 256-site blocks: the LDS-staged kernels (default, and the
+    3-wave build) give the gathering kernel's result bit for bit, and match the oracle."""
+    p = Problem([16, 8, 8, 8], [4, 4, 4, 4], seed=12)
+    digests = []
+    for kd in ("0", "2", "3"):
+        monkeypatch.setenv("QWS_MI355X_DSLASH_D", kd)  # read when a lattice is created
+        lat = Q.Lattice(p.L, p.B, -1, 18, kappa=p.kappa)
+        lat.load_gauge(p.U)
+        lat.load_clover(p.C)
+        o = lat.spinor(Q.F64)
+        lat.dslash(o, lat.spinor(Q.F64, p.b))
+        digests.append(o.digest())
+        if kd == "2":
+            assert rel(o.download(), p.orc.dslash(p.b)) < TOL64
+        lat.close()
+    assert digests[0] == digests[1] == digests[2]
+
+
 def test_links12_reconstruction():
     """12-real FP32 links (third row rebuilt) equal 18-real storage for SU(3) links."""
     p = Problem([8, 8, 4, 8], [4, 4, 2, 4], su3=True, seed=11)
