@@ -603,6 +603,17 @@ def main():
                                   "GBs": V * BYTES_PER_SITE_F32_12 / (ms * 1e-3) / 1e9}), flush=True)
             ms = lat.time_dslash(Q.F64, a.dslash_sweep)
             print(json.dumps({"order": order, "f64_ms": ms, "GBs": V * 1536 / (ms * 1e-3) / 1e9}), flush=True)
+        # FP64 kernel variants (QWS_MI355X_DSLASH_D, read when a lattice is created):
+        # 0 global gathers, 2 / 3 LDS-staged for 2 / 3 waves per SIMD; index order
+        lat.close()
+        for kd in ("0", "2", "3"):
+            os.environ["QWS_MI355X_DSLASH_D"] = kd
+            lk, _ = setup(Q)
+            ms = lk.time_dslash(Q.F64, a.dslash_sweep)
+            print(json.dumps({"order": 0, "f64_kernel": int(kd), "f64_ms": ms,
+                              "GBs": V * 1536 / (ms * 1e-3) / 1e9}), flush=True)
+            lk.close()
+        os.environ.pop("QWS_MI355X_DSLASH_D")
         return
     if a.dslash_only:
         ms = lat.time_dslash(Q.F32, a.dslash_only)
