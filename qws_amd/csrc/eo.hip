@@ -77,7 +77,9 @@ k_deo(Geo g, int pe, int set, int clv, int mode, typename Store<S>::R fac, S* ou
       const S* in0, const S* __restrict__ U, const S* __restrict__ C) {
   using R = typename Store<S>::R;
   const long long Vh = g.V >> 1;
-  const long long h = blockIdx.x * (long long)EO_THREADS + threadIdx.x;
+  // XCD-contiguous ranges of sites: the z and t neighbours a workgroup reads were fetched
+  // by workgroups on the same XCD (its L2), not by ones spread round-robin over the XCDs
+  const long long h = xcd_remap(g, blockIdx.x, gridDim.x) * (long long)EO_THREADS + threadIdx.x;
   if (h >= Vh) return;
   EoSite e;
   eo_coords(g, pe, h, e);
